@@ -1,0 +1,176 @@
+"""Benchmark: batched SQP solves/s and fwd+bwd grads/s, fused vs per-launch GPU path.
+
+Workload (default): quadrotor nonlinear MPC, nx=13, nu=4, horizon T=50
+(N=50 dynamics steps), batch B=1024, 5 SQP iterations, PCG eps 1e-6 absolute,
+synthetic seeded inputs.  One step = one forward solve of the whole batch plus
+the implicit-differentiation backward of the tracking loss
+0.5 * sum ||x* - x_ref||^2 with respect to the shared state/control weights and
+the per-instance initial states.  `value` is the fused path's grads/s.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 is not supported
+(instances are independent; one process drives one GPU).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SQP solves/s & fwd+bwd grads/s, quadrotor T=50 B=1024, fused vs plain GPU"
+MI355X_FP64_PEAK_TFLOPS = 78.6
+
+
+def pcg_flops(N, nx):
+    """Algorithmic flops of one PCG iteration of one instance: two block-tridiagonal
+    matvecs over (3N+1) nx^2 entries plus dots and axpys."""
+    return 4 * (3 * N + 1) * nx * nx + 10 * (N + 1) * nx
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--horizon", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=96, help="instances in the CPU baseline sample")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: instances are independent and this build drives a single GPU "
+              "per process; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from diffmpc_amd import _lib
+    from diffmpc_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    name, prob, sp = W.quadrotor(batch=args.batch, seed=1, horizon=args.horizon)
+    N, nx = prob["horizon"], prob["x0"].shape[1]
+
+    def make(mode):
+        solver, params, guess = W.make_solver(name, prob, sp, device=dev, mode=mode)
+        return solver, params, guess
+
+    def fwd_bwd(solver, params, guess):
+        wx = params["weights_penalization_reference_state_trajectory"][0].clone().requires_grad_(True)
+        wu = params["weights_penalization_control_squared"][0].clone().requires_grad_(True)
+        x0 = params["initial_state"].clone().requires_grad_(True)
+        weights = {"weights_penalization_reference_state_trajectory": wx,
+                   "weights_penalization_control_squared": wu, "initial_state": x0}
+        sol = solver.solve(guess, params, weights)
+        loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+        grads = torch.autograd.grad(loss, [wx, wu, x0])
+        return sol, grads
+
+    def fwd(solver, params, guess):
+        with torch.no_grad():
+            return solver.solve(guess, params)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            out = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, out
+
+    # ---- fused path -------------------------------------------------------
+    solver, params, guess = make("fused")
+    t_fwd, sol = timed(lambda: fwd(solver, params, guess), args.steps, args.warmup)
+    lib = _lib.load()
+    _lib.profile_read(_lib.PROF_QP_ITERATION)
+    _lib.profile_read(_lib.PROF_BACKWARD)
+    _lib.profile_read(_lib.PROF_LINEARIZE)
+    lib.dmpc_profile_enable(1)
+    t_fb, (sol_fb, _) = timed(lambda: fwd_bwd(solver, params, guess), args.steps, args.warmup)
+    torch.cuda.synchronize()
+    lib.dmpc_profile_enable(0)
+    qp_ms, qp_n = _lib.profile_read(_lib.PROF_QP_ITERATION)
+    bw_ms, bw_n = _lib.profile_read(_lib.PROF_BACKWARD)
+    ln_ms, ln_n = _lib.profile_read(_lib.PROF_LINEARIZE)
+    pcg_fwd = int(sol_fb.num_pcg_iters.sum())  # over instances and SQP iterations of one step
+    pcg_bwd = int(solver.last_backward_pcg_iters.sum())
+    steps_prof = args.steps + args.warmup
+    flops_qp_launch = pcg_flops(N, nx) * pcg_fwd / sp["num_scp_iteration_max"]
+    qp_avg_ms = qp_ms / max(qp_n, 1)
+    achieved = flops_qp_launch / (qp_avg_ms * 1e-3) / 1e12
+
+    # ---- per-launch path --------------------------------------------------
+    psolver, pparams, pguess = make("plain")
+    t_pfwd, psol = timed(lambda: fwd(psolver, pparams, pguess), args.plain_steps, 1)
+    t_pfb, _ = timed(lambda: fwd_bwd(psolver, pparams, pguess), args.plain_steps, 1)
+    same = float((sol.states - psol.states).abs().max() / sol.states.abs().max())
+
+    B = args.batch
+    out = {
+        "metric": METRIC,
+        "value": B / t_fb,
+        "unit": "grads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_fb * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 dynamics/linearization, f64 Schur/PCG",
+        "data": "synthetic",
+        "config": {"workload": f"quadrotor nx=13 nu=4 T={N} B={B}, 5 SQP iterations, PCG eps 1e-6",
+                   "global_batch": B, "horizon": N, "parallelism": "single GPU"},
+        "fused": {"solves_per_s": B / t_fwd, "grads_per_s": B / t_fb, "ms_fwd": t_fwd * 1e3, "ms_fwd_bwd": t_fb * 1e3},
+        "plain": {"solves_per_s": B / t_pfwd, "grads_per_s": B / t_pfb, "ms_fwd": t_pfwd * 1e3,
+                  "ms_fwd_bwd": t_pfb * 1e3},
+        "fused_vs_plain": {"solves": t_pfwd / t_fwd, "grads": t_pfb / t_fb},
+        "fused_plain_max_rel_diff": same,
+        "pcg_iterations_per_step": {"forward": pcg_fwd, "backward": pcg_bwd},
+        "kernel_ms_per_step": {"qp_iteration": qp_ms / steps_prof, "backward": bw_ms / steps_prof,
+                               "linearize": ln_ms / steps_prof},
+        "roofline": {
+            "bound": "mfma",
+            "kernel": "qp_iteration_kernel (assembly + PCG + recovery, one SQP iteration of the batch)",
+            "achieved": achieved,
+            "peak": MI355X_FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / MI355X_FP64_PEAK_TFLOPS,
+            "traffic": None,
+            "launch_avg_ms": qp_avg_ms,
+            "flops_per_launch": flops_qp_launch,
+        },
+    }
+
+    # ---- CPU baseline: the float64 oracle on a bounded sample --------------
+    if not args.no_cpu_baseline:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import dynamics as odyn
+        from oracle import sqp as osqp
+
+        m = args.cpu_sample
+        sub = W.slice_problem(prob, slice(0, m))
+        sub["model"] = odyn.make_model(name)
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            ref = osqp.sqp_solve(sub, sp)
+            t1 = time.perf_counter()
+            osqp.sqp_backward(sub, sp, ref, ref["states"] - sub["x_ref"], np.zeros_like(ref["controls"]))
+            t2 = time.perf_counter()
+        out["cpu_baseline"] = {
+            "value": m / (t2 - t0), "unit": "grads/s", "cores": 1, "kind": "port",
+            "sample": f"{m} of the {B} instances, forward + backward, float64 numpy oracle, one thread",
+            "solves_per_s": m / (t1 - t0),
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+"""ctypes binding of libdmpc.so (the C ABI declared in include/dmpc.h).
+
+The library is built in-tree by ``diffmpc_amd/csrc/build.sh`` (or
+``__graft_entry__.build()``).  torch is imported first so that the HIP runtime
+torch ships is the one the library binds to (both use the SONAME
+libamdhip64.so.7).  There is no fallback: if the library is missing, every
+solver call raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must be loaded before libdmpc.so)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdmpc.so")
+
+DMPC_OK = 0
+DMPC_ERR_ARG = -1
+DMPC_ERR_UNSUPPORTED = -2
+DMPC_ERR_HIP = -3
+
+MODE_FUSED = 0
+MODE_PLAIN = 1
+
+MODEL_LINEAR = 0
+MODEL_QUADROTOR = 1
+MODEL_SPACECRAFT3 = 2
+MODEL_SPACECRAFT7 = 3
+MODEL_CARTPOLE = 4
+MODEL_CHAIN = 5
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+_c_int_p = ctypes.POINTER(ctypes.c_int32)
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int32), ("nx", ctypes.c_int32), ("nu", ctypes.c_int32),
+        ("horizon", ctypes.c_int32), ("batch", ctypes.c_int32), ("scheme", ctypes.c_int32),
+        ("n_params", ctypes.c_int32), ("penalize_control_reference", ctypes.c_int32),
+        ("dt", ctypes.c_double),
+    ]
+
+
+class SolverParams(ctypes.Structure):
+    _fields_ = [
+        ("num_scp_iteration_max", ctypes.c_int32), ("tol_convergence", ctypes.c_double),
+        ("linesearch", ctypes.c_int32), ("linesearch_eta", ctypes.c_double),
+        ("n_alphas", ctypes.c_int32), ("linesearch_alphas", ctypes.c_double * 8),
+        ("pcg_max_iter", ctypes.c_int32), ("pcg_tol_epsilon", ctypes.c_double),
+        ("warm_start_backward", ctypes.c_int32),
+    ]
+
+
+class ProblemData(ctypes.Structure):
+    _fields_ = [
+        ("x0", ctypes.c_void_p), ("x_final", ctypes.c_void_p), ("x_ref", ctypes.c_void_p),
+        ("u_ref", ctypes.c_void_p), ("w_x", ctypes.c_void_p), ("w_u", ctypes.c_void_p),
+        ("w_f", ctypes.c_void_p), ("dyn_params", ctypes.c_void_p),
+    ]
+
+
+class Solution(ctypes.Structure):
+    _fields_ = [
+        ("states", ctypes.c_void_p), ("controls", ctypes.c_void_p), ("dual", ctypes.c_void_p),
+        ("status", ctypes.c_void_p), ("num_iter", ctypes.c_void_p),
+        ("convergence_error", ctypes.c_void_p), ("convergence_errors", ctypes.c_void_p),
+        ("linesearch_alphas", ctypes.c_void_p), ("num_pcg_iters", ctypes.c_void_p),
+    ]
+
+
+class Saved(ctypes.Structure):
+    _fields_ = [("AB", ctypes.c_void_p), ("fd", ctypes.c_void_p), ("qinv", ctypes.c_void_p), ("rinv", ctypes.c_void_p)]
+
+
+class Grads(ctypes.Structure):
+    _fields_ = [
+        ("w_x", ctypes.c_void_p), ("w_u", ctypes.c_void_p), ("w_f", ctypes.c_void_p),
+        ("x0", ctypes.c_void_p), ("x_ref", ctypes.c_void_p), ("u_ref", ctypes.c_void_p),
+        ("x_final", ctypes.c_void_p), ("dual_backward", ctypes.c_void_p), ("pcg_iters", ctypes.c_void_p),
+    ]
+
+
+# name -> (restype, argtypes); the list is the public C ABI
+_SIGNATURES = {
+    "dmpc_version": (ctypes.c_char_p, []),
+    "dmpc_supported": (ctypes.c_int, [ctypes.c_int32] * 5),
+    "dmpc_sqp_solve": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(SolverParams), ctypes.POINTER(ProblemData),
+                                      ctypes.POINTER(Solution), ctypes.POINTER(Saved), ctypes.c_int32, ctypes.c_void_p]),
+    "dmpc_sqp_backward": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(SolverParams), ctypes.POINTER(ProblemData),
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Saved), ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.POINTER(Grads), ctypes.c_int32, ctypes.c_void_p]),
+    "dmpc_linearize": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(ProblemData)] + [ctypes.c_void_p] * 7),
+    "dmpc_pcg_assemble": (ctypes.c_int, [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 8),
+    "dmpc_pcg_gamma": (ctypes.c_int, [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 10),
+    "dmpc_pcg_solve": (ctypes.c_int, [ctypes.c_int32] * 3 + [ctypes.c_void_p] * 4 + [ctypes.c_int32, ctypes.c_double]
+                       + [ctypes.c_void_p] * 3),
+    "dmpc_pcg_recover": (ctypes.c_int, [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 11),
+    "dmpc_set_blocks_per_group": (ctypes.c_int, [ctypes.c_int32]),
+    "dmpc_profile_enable": (ctypes.c_int, [ctypes.c_int32]),
+    "dmpc_profile_read": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
+}
+
+PROF_QP_ITERATION = 0
+PROF_BACKWARD = 1
+PROF_LINEARIZE = 2
+
+
+def profile_read(kernel):
+    """(total ms, launches) of one instrumented kernel since the last read."""
+    lib = load()
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int32(0)
+    check(lib.dmpc_profile_read(kernel, ctypes.byref(ms), ctypes.byref(n)), "dmpc_profile_read")
+    return ms.value, n.value
+
+_lib = None
+
+
+class DmpcError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libdmpc.so (once); raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DmpcError(
+            f"{LIB_PATH} is missing: build it with diffmpc_amd/csrc/build.sh "
+            "(there is no CPU fallback for the solver)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def check(rc, what):
+    if rc == DMPC_OK:
+        return
+    names = {DMPC_ERR_ARG: "bad argument", DMPC_ERR_UNSUPPORTED: "unsupported configuration", DMPC_ERR_HIP: "HIP error"}
+    raise DmpcError(f"{what} failed: {names.get(rc, rc)} ({rc})")
+
+
+def ptr(t):
+    """Device pointer of a contiguous CUDA/HIP tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise DmpcError("libdmpc needs device tensors (got a CPU tensor)")
+    if not t.is_contiguous():
+        raise DmpcError("libdmpc needs contiguous tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

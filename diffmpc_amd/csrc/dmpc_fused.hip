@@ -1,0 +1,884 @@
+// Fused batched SQP solve and adjoint backward for CDNA4 (gfx950).
+//
+// One workgroup owns one problem instance for the whole SQP solve
+// (diffmpc/solvers/sqp.py:436-560): linearization, Schur/preconditioner
+// assembly, the PCG loop (pcg.py:350-445) and primal recovery all run inside
+// one persistent kernel with no host round trip.
+//
+// Work mapping.  Time block t (0..N) of the Schur system is owned by a group
+// of G lanes (G = next power of two >= nx, so a block never straddles a
+// wavefront); lane i of the group owns row i of the block.  A thread group
+// owns K blocks t = s * NG + group (s < K).  Block-local products (the
+// Cholesky factor of theta_t, y = theta_t^-1 r_t) use cross-lane shuffles;
+// only neighbour-block vectors go through LDS, so one PCG iteration costs
+// three workgroup barriers.
+//
+// Numerics.  Dynamics are integrated in float32 with forward-mode dual
+// numbers (one tangent per lane).  Everything of the QP is float64: the
+// Schur complement S = -G H^-1 G^T is applied matrix-free from the float32
+// A_t, B_t held in registers, and the block-Jacobi-stair preconditioner
+//   Phi^-1 r |_t = D_t (r_t - phi_t D_{t-1} r_{t-1} - phi_{t+1}^T D_{t+1} r_{t+1}),
+//   D_t = theta_t^-1,
+// which equals the explicit Phiinv rows of pcg.py:213-239, is applied in
+// factored form with D_t rows (float64) in registers.  As_next = -I and the
+// diagonal cost Hessians of optimal_control_problem.py:301-322 are used
+// in closed form.
+#include "dmpc_internal.h"
+#include "dmpc_models.h"
+
+namespace dmpc {
+
+int g_forced_K = 0;
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double nanmax(double a, double b) {
+  if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000ULL);
+  return a > b ? a : b;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum over the workgroup through one barrier; every thread gets the same,
+// bit-identical result (same summation order everywhere).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  double s = 0.0;
+  for (int k = 0; k < nw; ++k) s += red[k];
+  return s;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  double s = red[0];
+  for (int k = 1; k < nw; ++k) s = nanmax(s, red[k]);
+  return s;
+}
+
+__device__ __forceinline__ double nan_to_num(double v) {
+  if (v != v) return 0.0;
+  if (v == __longlong_as_double(0x7ff0000000000000ULL)) return 1.7976931348623157e308;
+  if (v == __longlong_as_double(0xfff0000000000000ULL)) return -1.7976931348623157e308;
+  return v;
+}
+
+// LDS carve-up.  Vectors over time blocks use stride NX per block; arrays
+// marked "padded" have a zero block before t = 0 and after t = N, so
+// neighbour reads need no branches: index (t + 1) * NX + i.
+template <int NX, int NU>
+struct LdsPlan {
+  static __host__ __device__ size_t doubles(int N) {
+    const size_t NB = N + 1, NBP = N + 3;
+    return NB * NX /*xb*/ + NB * NU /*ub*/ + NBP * NX /*lamS*/ + NB * NX /*qinv*/ + NB * NX /*qv*/ +
+           NB * NU /*rv*/ + NU /*rinv*/ + NX /*x0*/ + 3 * 16 /*red*/ + 5 * NBP * NX /*P RT Y YQ YX*/ +
+           NBP * NU /*YU*/ + NB * NX * NX /*LF*/ + NB * NX /*ID*/;
+  }
+  static __host__ __device__ size_t floats(int N, int P) { return (size_t)N * NX + P; }
+  static __host__ size_t bytes(int N, int P) { return doubles(N) * 8 + floats(N, P) * 4 + 16; }
+};
+
+struct Lds {
+  double *xb, *ub, *lamS, *qinv, *qv, *rv, *rinv, *x0, *redA, *redB, *redC;
+  double *P, *RT, *Y, *YQ, *YX, *YU;
+  double *LF, *ID;  // Cholesky factor rows of theta_t and 1 / L_kk
+  float *fd, *prm;
+};
+
+template <int NX, int NU>
+__device__ Lds carve(void* base, int N) {
+  const int NB = N + 1, NBP = N + 3;
+  double* d = reinterpret_cast<double*>(base);
+  Lds L;
+  L.xb = d; d += NB * NX;
+  L.ub = d; d += NB * NU;
+  L.lamS = d; d += NBP * NX;
+  L.qinv = d; d += NB * NX;
+  L.qv = d; d += NB * NX;
+  L.rv = d; d += NB * NU;
+  L.rinv = d; d += NU;
+  L.x0 = d; d += NX;
+  L.redA = d; d += 16;
+  L.redB = d; d += 16;
+  L.redC = d; d += 16;
+  L.P = d; d += NBP * NX;
+  L.RT = d; d += NBP * NX;
+  L.Y = d; d += NBP * NX;
+  L.YQ = d; d += NBP * NX;
+  L.YX = d; d += NBP * NX;
+  L.YU = d; d += NBP * NU;
+  L.LF = d; d += NB * NX * NX;
+  L.ID = d; d += NB * NX;
+  float* f = reinterpret_cast<float*>(d);
+  L.fd = f; f += N * NX;
+  L.prm = f;
+  return L;
+}
+
+// Per-thread register state of the K blocks it owns.
+template <int NX, int NU, int K>
+struct Regs {
+  float acol[K][NX];  // column `lane` of A_t          (zero for t = N)
+  float arow[K][NX];  // row `lane` of A_{t-1}         (zero for t = 0)
+  float brow[K][NU];  // row `lane` of B_{t-1}         (zero for t = 0)
+  float bcol[K][NX];  // column `lane` of B_t, lane < nu (zero otherwise)
+  double drow[K][NX]; // row `lane` of D_t = theta_t^-1
+  double qi[K];       // Qinv_t[lane, lane]
+  int t[K];
+  bool ok[K];         // t <= N and lane < nx
+};
+
+// Load A/B rows and columns of the saved linearization into registers.
+template <int NX, int NU, int K>
+__device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, const Lds& L, int N, int lane) {
+  constexpr int NZ = NX + NU;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = R.t[s];
+    const bool ok = R.ok[s];
+    const float* At = ABb + (size_t)t * NX * NZ;
+    const float* Ap = ABb + (size_t)(t - 1) * NX * NZ;
+#pragma unroll
+    for (int j = 0; j < NX; ++j) R.acol[s][j] = (ok && t < N) ? At[j * NZ + lane] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < NX; ++j) R.bcol[s][j] = (ok && t < N && lane < NU) ? At[j * NZ + NX + lane] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) R.arow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + k] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) R.brow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + NX + k] : 0.0f;
+    R.qi[s] = ok ? L.qinv[t * NX + lane] : 0.0;
+  }
+}
+
+// theta_t row `lane`, its Cholesky factor and D_t = theta_t^-1 (row `lane`),
+// float64, inside the lane group of block t:
+//   theta_0 = Qinv_0,
+//   theta_t = A_{t-1} Qinv_{t-1} A_{t-1}^T + B_{t-1} Rinv B_{t-1}^T + Qinv_t
+// (pcg.py:182-196 with As_next = -I).  The factor is computed in registers
+// with cross-lane shuffles, parked in LDS, and each lane then solves
+// L L^T d = e_lane for its row of D_t.
+template <int NX, int NU, int K>
+__device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const Lds& L, int lane, int gbase) {
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = R.t[s];
+    const bool ok = R.ok[s];
+    const int tp = t >= 1 ? t - 1 : 0;
+    double row[NX];
+    {
+      double aq[NX], bq[NU];
+#pragma unroll
+      for (int k = 0; k < NX; ++k) aq[k] = (double)R.arow[s][k] * (ok ? L.qinv[tp * NX + k] : 0.0);
+#pragma unroll
+      for (int k = 0; k < NU; ++k) bq[k] = (double)R.brow[s][k] * L.rinv[k];
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        double acc = (j == lane) ? R.qi[s] : 0.0;
+#pragma unroll
+        for (int k = 0; k < NX; ++k) acc += aq[k] * (double)__shfl(R.arow[s][k], gbase + j, 64);
+#pragma unroll
+        for (int k = 0; k < NU; ++k) acc += bq[k] * (double)__shfl(R.brow[s][k], gbase + j, 64);
+        row[j] = ok ? acc : ((j == lane) ? 1.0 : 0.0);
+      }
+    }
+    // right-looking Cholesky: lane i ends with row[k] = L[i][k], k <= i
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      const double dkk = __shfl(row[k], gbase + k, 64);
+      const double lkk = sqrt(dkk);
+      const double il = 1.0 / lkk;
+      if (ok && lane == k) L.ID[t * NX + k] = il;
+      const double lik = (lane == k) ? lkk : row[k] * il;
+      row[k] = lik;
+#pragma unroll
+      for (int j = k + 1; j < NX; ++j) {
+        const double ljk = __shfl(lik, gbase + j, 64);
+        row[j] -= lik * ljk;
+      }
+    }
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < NX; ++k) L.LF[(t * NX + lane) * NX + k] = row[k];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = R.t[s];
+    const double* LF = L.LF + (size_t)t * NX * NX;
+    const double* ID = L.ID + t * NX;
+    // column `lane` of theta^-1 = L^-T L^-1 e_lane
+    double w[NX];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) {
+      double acc = (j == lane) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < j; ++k) acc -= LF[j * NX + k] * w[k];
+      w[j] = acc * ID[j];
+    }
+#pragma unroll
+    for (int j = NX - 1; j >= 0; --j) {
+      double acc = w[j];
+#pragma unroll
+      for (int k = j + 1; k < NX; ++k) acc -= LF[k * NX + j] * R.drow[s][k];
+      R.drow[s][j] = acc * ID[j];
+    }
+#pragma unroll
+    for (int j = 0; j < NX; ++j) R.drow[s][j] = R.ok[s] ? R.drow[s][j] : 0.0;
+  }
+}
+
+// y = D_t v (v distributed over the lanes of the block group).
+template <int NX, int NU, int K>
+__device__ __forceinline__ double dmul(const Regs<NX, NU, K>& R, int s, double v, int gbase) {
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < NX; ++j) acc += R.drow[s][j] * __shfl(v, gbase + j, 64);
+  return acc;
+}
+
+// Block-Jacobi-preconditioned CG on S lam = gamma (pcg.py:350-445) with the
+// reference's stop rule: at least one iteration, then continue while
+// |eta| > eps and it <= max_iter - 1.  lam enters as the guess in L.lamS
+// (padded) and leaves in registers.  Returns the iteration count.
+template <int NX, int NU, int K>
+__device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K], const Lds& L, int N,
+                   int lane, int gbase, double eps, int max_iter, double* eta_out) {
+  double r[K], p[K], rt[K];
+  // ---- r = gamma - S lam0 -------------------------------------------------
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = R.t[s];
+    lam[s] = 0.0;
+    p[s] = 0.0;
+    if (R.ok[s]) {
+      const double pv = L.lamS[(t + 1) * NX + lane];
+      lam[s] = pv;
+      double wx = (t == 0) ? pv : -pv, wu = 0.0;
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        const double pn = L.lamS[(t + 2) * NX + j];
+        wx += (double)R.acol[s][j] * pn;
+        wu += (double)R.bcol[s][j] * pn;
+      }
+      L.YX[(t + 1) * NX + lane] = R.qi[s] * wx;
+      if (lane < NU && t < N) L.YU[(t + 1) * NU + lane] = L.rinv[lane] * wu;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = R.t[s];
+    double ups = 0.0;
+    if (R.ok[s]) {
+      double g = (t == 0 ? 1.0 : -1.0) * L.YX[(t + 1) * NX + lane];
+#pragma unroll
+      for (int j = 0; j < NX; ++j) g += (double)R.arow[s][j] * L.YX[t * NX + j];
+#pragma unroll
+      for (int k = 0; k < NU; ++k) g += (double)R.brow[s][k] * L.YU[t * NU + k];
+      ups = -g;
+    }
+    r[s] = R.ok[s] ? gam[s] - ups : 0.0;
+    const double y = dmul(R, s, r[s], gbase);
+    if (R.ok[s]) {
+      L.Y[(t + 1) * NX + lane] = y;
+      L.YQ[(t + 1) * NX + lane] = R.qi[s] * y;
+    }
+  }
+  __syncthreads();
+  double part = 0.0;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = R.t[s];
+    double z = 0.0;
+    if (R.ok[s]) {
+      double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        a1 += (double)R.arow[s][j] * L.YQ[t * NX + j];
+        a2 += (double)R.acol[s][j] * L.Y[(t + 2) * NX + j];
+      }
+      z = r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2;
+    }
+    rt[s] = dmul(R, s, z, gbase);
+    if (R.ok[s]) {
+      part += r[s] * rt[s];
+      L.RT[(t + 1) * NX + lane] = rt[s];
+      L.P[(t + 1) * NX + lane] = 0.0;
+    }
+  }
+  double eta = block_sum(part, L.redC);
+  double beta = 0.0;
+  int it = 0;
+  while (true) {
+    const bool cont = (fabs(eta) > eps && it <= max_iter - 1) || it < 1;
+    if (!cont) break;
+    // ---- phase A: p = rt + beta p; w = G^T p; y = H^-1 w; v = -w.y --------
+    double vpart = 0.0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const int t = R.t[s];
+      if (R.ok[s]) {
+        const double pv = rt[s] + beta * p[s];
+        p[s] = pv;
+        double wx = (t == 0) ? pv : -pv, wu = 0.0;
+#pragma unroll
+        for (int j = 0; j < NX; ++j) {
+          const int o = (t + 2) * NX + j;
+          const double pn = L.RT[o] + beta * L.P[o];
+          wx += (double)R.acol[s][j] * pn;
+          wu += (double)R.bcol[s][j] * pn;
+        }
+        const double yx = R.qi[s] * wx;
+        L.YX[(t + 1) * NX + lane] = yx;
+        vpart += wx * yx;
+        if (lane < NU && t < N) {
+          const double yu = L.rinv[lane] * wu;
+          L.YU[(t + 1) * NU + lane] = yu;
+          vpart += wu * yu;
+        }
+      }
+    }
+    const double v = -block_sum(vpart, L.redA);
+    const double alpha = eta / v;
+    // ---- phase B: ups = S p; lam += alpha p; r -= alpha ups; y = D r -----
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const int t = R.t[s];
+      if (R.ok[s]) {
+        double g = (t == 0 ? 1.0 : -1.0) * L.YX[(t + 1) * NX + lane];
+#pragma unroll
+        for (int j = 0; j < NX; ++j) g += (double)R.arow[s][j] * L.YX[t * NX + j];
+#pragma unroll
+        for (int k = 0; k < NU; ++k) g += (double)R.brow[s][k] * L.YU[t * NU + k];
+        lam[s] += alpha * p[s];
+        r[s] += alpha * g;  // r -= alpha * (-g)
+      }
+      const double y = dmul(R, s, r[s], gbase);
+      if (R.ok[s]) {
+        L.Y[(t + 1) * NX + lane] = y;
+        L.YQ[(t + 1) * NX + lane] = R.qi[s] * y;
+      }
+    }
+    __syncthreads();
+    // ---- phase C: rt = Phi^-1 r; eta' = r.rt ------------------------------
+    double epart = 0.0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const int t = R.t[s];
+      double z = 0.0;
+      if (R.ok[s]) {
+        double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < NX; ++j) {
+          a1 += (double)R.arow[s][j] * L.YQ[t * NX + j];
+          a2 += (double)R.acol[s][j] * L.Y[(t + 2) * NX + j];
+        }
+        z = r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2;
+      }
+      rt[s] = dmul(R, s, z, gbase);
+      if (R.ok[s]) {
+        epart += r[s] * rt[s];
+        L.RT[(t + 1) * NX + lane] = rt[s];
+        L.P[(t + 1) * NX + lane] = p[s];
+      }
+    }
+    const double eta_new = block_sum(epart, L.redB);
+    beta = eta_new / eta;
+    eta = eta_new;
+    ++it;
+  }
+  *eta_out = eta;
+  return it;
+}
+
+// gamma = C + G H^-1 q (pcg.py:245-288) for the rows this thread owns.
+// WITH_C: C = [x0; -f_d + A xbar + B ubar] (forward); otherwise C = 0.
+template <bool WITH_C, int NX, int NU, int K>
+__device__ __forceinline__ void gamma_rows(const Regs<NX, NU, K>& R, const Lds& L, int lane, double (&gam)[K]) {
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = R.t[s];
+    double g = 0.0;
+    if (R.ok[s]) {
+      if (t == 0) {
+        g = R.qi[s] * L.qv[lane] + (WITH_C ? L.x0[lane] : 0.0);
+      } else {
+        const int tp = t - 1;
+        g = -R.qi[s] * L.qv[t * NX + lane];
+        if (WITH_C) g -= (double)L.fd[tp * NX + lane];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+          double v = L.qinv[tp * NX + k] * L.qv[tp * NX + k];
+          if (WITH_C) v += L.xb[tp * NX + k];
+          g += (double)R.arow[s][k] * v;
+        }
+#pragma unroll
+        for (int k = 0; k < NU; ++k) {
+          double v = L.rinv[k] * L.rv[tp * NU + k];
+          if (WITH_C) v += L.ub[tp * NU + k];
+          g += (double)R.brow[s][k] * v;
+        }
+      }
+    }
+    gam[s] = g;
+  }
+}
+
+// Primal recovery from the multipliers in L.lamS (pcg.py:316-348).
+template <int NX, int NU, int K>
+__device__ __forceinline__ void recover_rows(const Regs<NX, NU, K>& R, const Lds& L, int N, int lane,
+                                             double (&xn)[K], double (&un)[K]) {
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    const int t = R.t[s];
+    xn[s] = 0.0;
+    un[s] = 0.0;
+    if (R.ok[s]) {
+      const double lt = L.lamS[(t + 1) * NX + lane];
+      double ax = 0.0, bx = 0.0;
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        const double ln = L.lamS[(t + 2) * NX + j];
+        ax += (double)R.acol[s][j] * ln;
+        bx += (double)R.bcol[s][j] * ln;
+      }
+      xn[s] = -R.qi[s] * (L.qv[t * NX + lane] + (t == 0 ? lt : -lt) + ax);
+      if (lane < NU && t < N) un[s] = -L.rinv[lane] * (L.rv[t * NU + lane] + bx);
+    }
+  }
+}
+
+template <int NX, int NU, int K>
+__device__ __forceinline__ void init_blocks(Regs<NX, NU, K>& R, int G, int N, int* lane_out, int* gbase_out) {
+  const int NG = blockDim.x / G;
+  const int grp = threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    R.t[s] = s * NG + grp;
+    R.ok[s] = (R.t[s] <= N) && (lane < NX);
+    if (R.t[s] > N) R.t[s] = N;  // keep indices in range for masked lanes
+  }
+  *lane_out = lane;
+  *gbase_out = (threadIdx.x & 63) - lane;
+}
+
+template <int NX, int NU>
+__device__ void zero_lds_vectors(const Lds& L, int N) {
+  const int NBP = N + 3;
+  for (int k = threadIdx.x; k < NBP * NX; k += blockDim.x) {
+    L.P[k] = 0.0; L.RT[k] = 0.0; L.Y[k] = 0.0; L.YQ[k] = 0.0; L.YX[k] = 0.0; L.lamS[k] = 0.0;
+  }
+  for (int k = threadIdx.x; k < NBP * NU; k += blockDim.x) L.YU[k] = 0.0;
+}
+
+// Closed-form quadratic model of the diagonal cost (sqp.py:160-197,
+// optimal_control_problem.py:249-323): Qinv = 1 / max(2 w, 1e-9),
+// q = -2 w ref (terminal: w_x + w_f and final_state), r = -2 w_u u_ref.
+template <int NX, int NU>
+__device__ void cost_terms(const Lds& L, int N, int b, const double* wx, const double* wu, const double* wf,
+                           const double* xref, const double* uref, const double* xf, int penalize) {
+  const int NB = N + 1;
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
+    const int t = k / NX, i = k % NX;
+    const double w = wx[b * NX + i];
+    if (t < N) {
+      L.qinv[k] = 1.0 / fmax(2.0 * w, kPsdMinEig);
+      L.qv[k] = -2.0 * w * xref[((size_t)b * NB + t) * NX + i];
+    } else {
+      const double wt = w + wf[b * NX + i];
+      L.qinv[k] = 1.0 / fmax(2.0 * wt, kPsdMinEig);
+      L.qv[k] = -2.0 * wt * xf[b * NX + i];
+    }
+  }
+  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) {
+    const int t = k / NU, i = k % NU;
+    L.rv[k] = (t < N && penalize) ? -2.0 * wu[b * NU + i] * uref[((size_t)b * NB + t) * NU + i] : 0.0;
+  }
+  for (int k = threadIdx.x; k < NU; k += blockDim.x) L.rinv[k] = 1.0 / fmax(2.0 * wu[b * NU + k], kPsdMinEig);
+}
+
+// ---------------------------------------------------------------------------
+// SQP driver kernels.  The SQP loop of sqp.py:460-533 runs as a fixed launch
+// sequence (linearize, QP) x num_scp_iteration_max followed by a finish
+// kernel, with no host synchronisation, so it can be captured in a hipGraph.
+// Each instance decides on the device whether it is still iterating:
+// (num_iter < maxit && err > tol) || num_iter < 1; finished instances return
+// at once and keep their last linearization, like the reference's vmapped
+// while_loop keeps frozen instances.
+__device__ __forceinline__ bool sqp_active(const SolveArgs& a, int b) {
+  const int it = a.num_iter[b];
+  return (it < a.maxit && a.conv_err[b] > a.tol) || it < 1;
+}
+
+__global__ void sqp_reset_kernel(int B, int32_t* num_iter) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < B) num_iter[b] = 0;
+}
+
+// f_d(x_t, u_t) and column j of [A_t B_t], one thread per (b, t, j)
+// (optimal_control_problem.py:360-383 with value_and_jacfwd).
+template <class Model>
+__global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
+  constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
+  const int N = a.N;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)a.B * N * NZ) return;
+  const int j = idx % NZ;
+  const int t = (idx / NZ) % N;
+  const int b = idx / ((long long)NZ * N);
+  if (!sqp_active(a, b)) return;
+  float xs[NX], us[NU], prm[Model::NPARAM];
+#pragma unroll
+  for (int i = 0; i < NX; ++i) xs[i] = (float)a.x[((size_t)b * (N + 1) + t) * NX + i];
+#pragma unroll
+  for (int k = 0; k < NU; ++k) us[k] = (float)a.u[((size_t)b * (N + 1) + t) * NU + k];
+#pragma unroll
+  for (int k = 0; k < Model::NPARAM; ++k) prm[k] = a.dyn[(size_t)b * a.P + k];
+  float fv[NX], col[NX];
+  step_jacobian_column<Model>(xs, us, prm, a.dt, a.scheme, j, fv, col);
+  float* At = a.AB + ((size_t)b * N + t) * NX * NZ;
+#pragma unroll
+  for (int i = 0; i < NX; ++i) At[i * NZ + j] = col[i];
+  if (j == 0) {
+#pragma unroll
+    for (int i = 0; i < NX; ++i) a.fd[((size_t)b * N + t) * NX + i] = fv[i];
+  }
+}
+
+// One SQP iteration of one instance per workgroup: Schur/preconditioner
+// assembly, gamma, the whole PCG loop, primal recovery and the step test
+// (sqp.py:262-299, 506-528).
+template <class Model, int G, int K, int MAXT>
+__global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
+  constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
+  extern __shared__ double smem_[];
+  const int b = blockIdx.x;
+  if (!sqp_active(a, b)) return;
+  const int it = a.num_iter[b];
+  const int N = a.N, NB = N + 1;
+  const Lds L = carve<NX, NU>(smem_, N);
+  Regs<NX, NU, K> R;
+  int lane, gbase;
+  init_blocks(R, G, N, &lane, &gbase);
+
+  zero_lds_vectors<NX, NU>(L, N);
+  __syncthreads();
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
+    L.xb[k] = a.x[(size_t)b * NB * NX + k];
+    L.lamS[NX + k] = a.lam[(size_t)b * NB * NX + k];
+  }
+  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) L.ub[k] = a.u[(size_t)b * NB * NU + k];
+  for (int k = threadIdx.x; k < N * NX; k += blockDim.x) L.fd[k] = a.fd[(size_t)b * N * NX + k];
+  for (int k = threadIdx.x; k < NX; k += blockDim.x) L.x0[k] = a.x0[b * NX + k];
+  cost_terms<NX, NU>(L, N, b, a.wx, a.wu, a.wf, a.xref, a.uref, a.xf, a.penalize);
+  __syncthreads();
+
+  const float* ABb = a.AB + (size_t)b * N * NX * NZ;
+  load_ab(R, ABb, L, N, lane);
+  assemble_dinv(R, L, lane, gbase);
+  double gam[K], lam[K];
+  gamma_rows<true>(R, L, lane, gam);
+  double eta;
+  const int npcg = pcg(R, gam, lam, L, N, lane, gbase, a.pcg_eps, a.pcg_max, &eta);
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+    if (R.ok[s]) L.lamS[(R.t[s] + 1) * NX + lane] = lam[s];
+  __syncthreads();
+  double xn[K], un[K];
+  recover_rows(R, L, N, lane, xn, un);
+  double e = 0.0;
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    if (R.ok[s]) {
+      const int t = R.t[s];
+      e = nanmax(e, fabs(xn[s] - L.xb[t * NX + lane]));
+      if (lane < NU) e = nanmax(e, fabs(un[s] - L.ub[t * NU + lane]));
+    }
+  }
+  const double err = block_max(e, L.redA);
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    if (R.ok[s]) {
+      const int t = R.t[s];
+      a.x[((size_t)b * NB + t) * NX + lane] = xn[s];
+      a.lam[((size_t)b * NB + t) * NX + lane] = lam[s];
+      if (lane < NU) a.u[((size_t)b * NB + t) * NU + lane] = un[s];
+    }
+  }
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) a.qinv[(size_t)b * NB * NX + k] = L.qinv[k];
+  for (int k = threadIdx.x; k < NU; k += blockDim.x) a.rinv[(size_t)b * NU + k] = L.rinv[k];
+  if (threadIdx.x == 0) {
+    a.conv_errs[(size_t)b * a.maxit + it] = err;
+    a.pcg_iters[(size_t)b * a.maxit + it] = npcg;
+    if (a.linesearch) a.ls_alphas[(size_t)b * a.maxit + it] = 1.0;
+    a.conv_err[b] = err;
+    a.num_iter[b] = it + 1;
+  }
+}
+
+// status and nan_to_num (sqp.py:535-559)
+__global__ void __launch_bounds__(256) sqp_finish_kernel(SolveArgs a, int nx, int nu) {
+  __shared__ double red[4];
+  const int b = blockIdx.x;
+  const int NB = a.N + 1;
+  double bad = 0.0;
+  for (int k = threadIdx.x; k < NB * nx; k += blockDim.x) {
+    const double v = a.x[(size_t)b * NB * nx + k];
+    if (v != v) bad = 1.0;
+  }
+  for (int k = threadIdx.x; k < NB * nu; k += blockDim.x) {
+    const double v = a.u[(size_t)b * NB * nu + k];
+    if (v != v) bad = 1.0;
+  }
+  bad = block_max(bad, red);
+  for (int k = threadIdx.x; k < NB * nx; k += blockDim.x) {
+    a.x[(size_t)b * NB * nx + k] = nan_to_num(a.x[(size_t)b * NB * nx + k]);
+    a.lam[(size_t)b * NB * nx + k] = nan_to_num(a.lam[(size_t)b * NB * nx + k]);
+  }
+  for (int k = threadIdx.x; k < NB * nu; k += blockDim.x)
+    a.u[(size_t)b * NB * nu + k] = nan_to_num(a.u[(size_t)b * NB * nu + k]);
+  if (threadIdx.x == 0) a.status[b] = (bad > 0.0 || a.conv_err[b] > a.tol) ? -100 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint backward (sqp.py:668-740): PCG on the saved linearization with
+// q~ = -dl/dx, r~ = -dl/du[:-1], C = 0, recovery of (x~, u~), then the
+// closed-form mixed-derivative contraction of the diagonal tracking cost.
+template <class Model, int G, int K, int MAXT>
+__global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
+  constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
+  extern __shared__ double smem_[];
+  const int b = blockIdx.x;
+  const int N = a.N, NB = N + 1;
+  const Lds L = carve<NX, NU>(smem_, N);
+  Regs<NX, NU, K> R;
+  int lane, gbase;
+  init_blocks(R, G, N, &lane, &gbase);
+
+  zero_lds_vectors<NX, NU>(L, N);
+  __syncthreads();
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
+    L.qinv[k] = a.qinv[(size_t)b * NB * NX + k];
+    L.qv[k] = -a.dl_dx[(size_t)b * NB * NX + k];
+    L.lamS[NX + k] = a.lam_bwd[(size_t)b * NB * NX + k];
+    L.xb[k] = a.x[(size_t)b * NB * NX + k];
+  }
+  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) {
+    const int t = k / NU;
+    L.rv[k] = t < N ? -a.dl_du[(size_t)b * NB * NU + k] : 0.0;
+    L.ub[k] = a.u[(size_t)b * NB * NU + k];
+  }
+  for (int k = threadIdx.x; k < NU; k += blockDim.x) L.rinv[k] = a.rinv[(size_t)b * NU + k];
+  __syncthreads();
+  const float* ABb = a.AB + (size_t)b * N * NX * NZ;
+  load_ab(R, ABb, L, N, lane);
+  assemble_dinv(R, L, lane, gbase);
+  double gam[K], lam[K];
+  gamma_rows<false>(R, L, lane, gam);
+  double eta;
+  const int npcg = pcg(R, gam, lam, L, N, lane, gbase, a.pcg_eps, a.pcg_max, &eta);
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+    if (R.ok[s]) L.lamS[(R.t[s] + 1) * NX + lane] = lam[s];
+  __syncthreads();
+  double xt[K], ut[K];
+  recover_rows(R, L, N, lane, xt, ut);
+  // per-row contraction terms -> LDS (reusing the PCG vectors), then one
+  // thread per component sums over time in a fixed order
+  double* TX = L.YX;  // (NB, NX): 2 (x - ref) x~
+  double* TU = L.YU;  // (NB, NU)
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    if (R.ok[s]) {
+      const int t = R.t[s];
+      const int i = lane;
+      const double ref = (t < N) ? a.xref[((size_t)b * NB + t) * NX + i] : a.xf[b * NX + i];
+      const double ex = L.xb[t * NX + i] - ref;
+      TX[t * NX + i] = 2.0 * ex * xt[s];
+      const double wxi = a.wx[b * NX + i];
+      a.g_xref[((size_t)b * NB + t) * NX + i] = (t < N) ? 2.0 * wxi * xt[s] : 0.0;
+      if (t == N) {
+        a.g_wf[b * NX + i] = -2.0 * ex * xt[s];
+        a.g_xf[b * NX + i] = 2.0 * (wxi + a.wf[b * NX + i]) * xt[s];
+      }
+      if (i < NU) {
+        const double ur = a.penalize ? a.uref[((size_t)b * NB + t) * NU + i] : 0.0;
+        TU[t * NU + i] = (t < N) ? 2.0 * (L.ub[t * NU + i] - ur) * ut[s] : 0.0;
+        a.g_uref[((size_t)b * NB + t) * NU + i] =
+            (t < N && a.penalize) ? 2.0 * a.wu[b * NU + i] * ut[s] : 0.0;
+      }
+      a.lam_bwd[((size_t)b * NB + t) * NX + i] = lam[s];
+      if (t == 0) a.g_x0[b * NX + i] = lam[s];
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NX + NU; i += blockDim.x) {
+    double acc = 0.0;
+    if (i < NX) {
+      for (int t = 0; t <= N; ++t) acc += TX[t * NX + i];
+      a.g_wx[b * NX + i] = -acc;
+    } else {
+      const int k = i - NX;
+      for (int t = 0; t < N; ++t) acc += TU[t * NU + k];
+      a.g_wu[b * NU + k] = -acc;
+    }
+  }
+  if (threadIdx.x == 0) a.pcg_iters[b] = npcg;
+}
+
+constexpr size_t kMaxLds = 160 * 1024;
+
+struct Launch {
+  int K, threads;
+  size_t lds;
+};
+
+// K blocks per thread group -> at most 1024 / 512 / 256 threads, so the
+// compiler may give each thread 128 / 256 / 512 registers.
+inline int max_threads(int K) { return K == 1 ? 1024 : (K == 2 ? 512 : 256); }
+
+template <class Model, int G>
+Launch plan(int N, int P) {
+  constexpr int NX = Model::NX, NU = Model::NU;
+  const int NB = N + 1;
+  Launch l{0, 0, LdsPlan<NX, NU>::bytes(N, P)};
+  const int cand[3] = {1, 2, 4};
+  for (int c = 0; c < 3; ++c) {
+    const int K = cand[c];
+    if (g_forced_K > 0 && K != g_forced_K) continue;
+    const int groups = (NB + K - 1) / K;
+    const int threads = ((groups * G + 63) / 64) * 64;
+    if (threads <= max_threads(K)) {
+      l.K = K;
+      l.threads = threads;
+      return l;
+    }
+  }
+  return l;  // K = 0: does not fit
+}
+
+template <class Model, int G, int K, int MAXT>
+int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
+  constexpr int NZ = Model::NX + Model::NU;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&qp_iteration_kernel<Model, G, K, MAXT>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) != hipSuccess)
+      return DMPC_ERR_HIP;
+    attr = true;
+  }
+  hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter);
+  const long long nlin = (long long)a.B * a.N * NZ;
+  for (int it = 0; it < a.maxit; ++it) {
+    prof_begin(PROF_LINEARIZE, st);
+    hipLaunchKernelGGL(linearize_kernel<Model>, dim3((unsigned)((nlin + 255) / 256)), dim3(256), 0, st, a);
+    prof_end(PROF_LINEARIZE, st);
+    prof_begin(PROF_QP_ITERATION, st);
+    hipLaunchKernelGGL((qp_iteration_kernel<Model, G, K, MAXT>), dim3(a.B), dim3(l.threads), l.lds, st, a);
+    prof_end(PROF_QP_ITERATION, st);
+  }
+  hipLaunchKernelGGL(sqp_finish_kernel, dim3(a.B), dim3(256), 0, st, a, Model::NX, Model::NU);
+  return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
+}
+
+template <class Model, int G, int K, int MAXT>
+int launch_bwd_k(const BackwardArgs& a, const Launch& l, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&fused_backward_kernel<Model, G, K, MAXT>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) != hipSuccess)
+      return DMPC_ERR_HIP;
+    attr = true;
+  }
+  prof_begin(PROF_BACKWARD, st);
+  hipLaunchKernelGGL((fused_backward_kernel<Model, G, K, MAXT>), dim3(a.B), dim3(l.threads), l.lds, st, a);
+  prof_end(PROF_BACKWARD, st);
+  return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
+}
+
+template <class Model, int G>
+int fwd_model(const SolveArgs& a, hipStream_t st) {
+  Launch l = plan<Model, G>(a.N, a.P);
+  if (l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
+  switch (l.K) {
+    case 1: return launch_fwd_k<Model, G, 1, 1024>(a, l, st);
+    case 2: return launch_fwd_k<Model, G, 2, 512>(a, l, st);
+    case 4: return launch_fwd_k<Model, G, 4, 256>(a, l, st);
+    default: return DMPC_ERR_UNSUPPORTED;
+  }
+}
+
+template <class Model, int G>
+int bwd_model(const BackwardArgs& a, hipStream_t st) {
+  Launch l = plan<Model, G>(a.N, 0);
+  if (l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
+  switch (l.K) {
+    case 1: return launch_bwd_k<Model, G, 1, 1024>(a, l, st);
+    case 2: return launch_bwd_k<Model, G, 2, 512>(a, l, st);
+    case 4: return launch_bwd_k<Model, G, 4, 256>(a, l, st);
+    default: return DMPC_ERR_UNSUPPORTED;
+  }
+}
+
+template <class Model, int G>
+bool fits(int N) {
+  Launch l = plan<Model, G>(N, 64);
+  return l.K > 0 && l.lds <= kMaxLds;
+}
+
+}  // namespace
+
+bool fused_supported(int model, int nx, int nu, int N) {
+  if (N < 1) return false;
+  switch (model) {
+    case MODEL_LINEAR: return nx == 4 && nu == 2 && fits<LinearModel<4, 2>, 4>(N);
+    case MODEL_QUADROTOR: return nx == 13 && nu == 4 && fits<QuadrotorModel, 16>(N);
+    case MODEL_SPACECRAFT3: return nx == 3 && nu == 3 && fits<Spacecraft3Model, 4>(N);
+    case MODEL_SPACECRAFT7: return nx == 7 && nu == 3 && fits<Spacecraft7Model, 8>(N);
+    case MODEL_CARTPOLE: return nx == 4 && nu == 1 && fits<CartpoleModel, 4>(N);
+    default: return false;
+  }
+}
+
+int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st) {
+  if (!fused_supported(model, nx, nu, a.N)) return DMPC_ERR_UNSUPPORTED;
+  switch (model) {
+    case MODEL_LINEAR: return fwd_model<LinearModel<4, 2>, 4>(a, st);
+    case MODEL_QUADROTOR: return fwd_model<QuadrotorModel, 16>(a, st);
+    case MODEL_SPACECRAFT3: return fwd_model<Spacecraft3Model, 4>(a, st);
+    case MODEL_SPACECRAFT7: return fwd_model<Spacecraft7Model, 8>(a, st);
+    case MODEL_CARTPOLE: return fwd_model<CartpoleModel, 4>(a, st);
+    default: return DMPC_ERR_UNSUPPORTED;
+  }
+}
+
+int fused_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t st) {
+  if (!fused_supported(model, nx, nu, a.N)) return DMPC_ERR_UNSUPPORTED;
+  switch (model) {
+    case MODEL_LINEAR: return bwd_model<LinearModel<4, 2>, 4>(a, st);
+    case MODEL_QUADROTOR: return bwd_model<QuadrotorModel, 16>(a, st);
+    case MODEL_SPACECRAFT3: return bwd_model<Spacecraft3Model, 4>(a, st);
+    case MODEL_SPACECRAFT7: return bwd_model<Spacecraft7Model, 8>(a, st);
+    case MODEL_CARTPOLE: return bwd_model<CartpoleModel, 4>(a, st);
+    default: return DMPC_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace dmpc

@@ -1,0 +1,71 @@
+// Internal declarations shared by the fused kernels, the per-launch kernels
+// and the C ABI.  Not part of the public interface (include/dmpc.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/dmpc.h"
+
+namespace dmpc {
+
+constexpr double kPsdMinEig = 1e-9;  // diffmpc/solvers/sqp.py:191-196
+constexpr int kMaxAlphas = 8;
+
+// Everything one batched SQP solve needs, flattened for kernel arguments.
+struct SolveArgs {
+  int B, N, P, scheme, penalize;
+  float dt;
+  int maxit;
+  double tol;
+  int pcg_max;
+  double pcg_eps;
+  int linesearch;
+  double ls_eta;
+  int n_alphas;
+  double alphas[kMaxAlphas];
+  // problem data (device)
+  const double *x0, *xf, *xref, *uref, *wx, *wu, *wf;
+  const float* dyn;
+  // solution (device, in: guess and guess histories, out: solution)
+  double *x, *u, *lam;
+  int32_t *status, *num_iter;
+  double *conv_err, *conv_errs, *ls_alphas;
+  int32_t* pcg_iters;
+  // last linearization point, kept for the backward
+  float* AB;      // (B, N, nx, nx+nu): [A_t | B_t]
+  float* fd;      // (B, N, nx): f_d(x_t, u_t)
+  double* qinv;   // (B, N+1, nx) diagonal of Qinv
+  double* rinv;   // (B, nu) diagonal of Rinv (the same for every t)
+};
+
+struct BackwardArgs {
+  int B, N, penalize, pcg_max;
+  double pcg_eps;
+  const double *x0, *xf, *xref, *uref, *wx, *wu, *wf;
+  const double *x, *u;                 // solution
+  const float* AB;                     // saved linearization
+  const double *qinv, *rinv;
+  const double *dl_dx, *dl_du;         // cotangents (B,N+1,nx), (B,N+1,nu)
+  double* lam_bwd;                     // in: adjoint warm start, out: lam~
+  double *g_wx, *g_wu, *g_wf, *g_x0, *g_xref, *g_uref, *g_xf;
+  int32_t* pcg_iters;                  // (B,)
+};
+
+// fused path (dmpc_fused.hip)
+int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t s);
+int fused_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t s);
+bool fused_supported(int model, int nx, int nu, int N);
+
+// per-launch path (dmpc_plain.hip)
+int plain_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t s);
+int plain_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t s);
+
+// tuning knob: thread groups handle K time blocks each (0 = automatic)
+extern int g_forced_K;
+
+// optional HIP-event timing of individual kernel launches (dmpc_profile_*)
+enum ProfKernel { PROF_QP_ITERATION = 0, PROF_BACKWARD = 1, PROF_LINEARIZE = 2, PROF_NKERNELS = 3 };
+void prof_begin(int kernel, hipStream_t s);
+void prof_end(int kernel, hipStream_t s);
+
+}  // namespace dmpc

@@ -1,0 +1,274 @@
+// Dynamics models and integrators as device templates over the scalar type.
+//
+// Every model is a struct with NX, NU and a templated
+//   f(const T* x, const T* u, const float* prm, T* xdot)
+// where T is float or Dual (forward-mode derivative with one tangent).  The
+// flat parameter layouts are the ones documented in include/dmpc.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace dmpc {
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual number with one tangent direction.
+struct Dual {
+  float v, d;
+};
+__device__ __forceinline__ Dual mk(float v) { return Dual{v, 0.0f}; }
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator*(float s, Dual a) { return {s * a.v, s * a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, float s) { return {s * a.v, s * a.d}; }
+__device__ __forceinline__ Dual operator+(Dual a, float s) { return {a.v + s, a.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, float s) { return {a.v - s, a.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  float inv = 1.0f / b.v;
+  float q = a.v * inv;
+  return {q, (a.d - q * b.d) * inv};
+}
+__device__ __forceinline__ Dual operator/(Dual a, float s) { return {a.v / s, a.d / s}; }
+__device__ __forceinline__ Dual dsqrt(Dual a) {
+  float s = sqrtf(a.v);
+  return {s, a.d * (0.5f / s)};
+}
+__device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
+__device__ __forceinline__ Dual dsin(Dual a) { return {sinf(a.v), a.d * cosf(a.v)}; }
+__device__ __forceinline__ Dual dcos(Dual a) { return {cosf(a.v), -a.d * sinf(a.v)}; }
+__device__ __forceinline__ float dsin(float a) { return sinf(a); }
+__device__ __forceinline__ float dcos(float a) { return cosf(a); }
+__device__ __forceinline__ float val(float a) { return a; }
+__device__ __forceinline__ float val(Dual a) { return a.v; }
+__device__ __forceinline__ float tan_(Dual a) { return a.d; }
+
+template <class T>
+__device__ __forceinline__ T sc(float v);
+template <>
+__device__ __forceinline__ float sc<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ Dual sc<Dual>(float v) { return Dual{v, 0.0f}; }
+
+enum ModelId : int {
+  MODEL_LINEAR = 0,
+  MODEL_QUADROTOR = 1,
+  MODEL_SPACECRAFT3 = 2,
+  MODEL_SPACECRAFT7 = 3,
+  MODEL_CARTPOLE = 4,
+  MODEL_CHAIN = 5,
+};
+
+// ---------------------------------------------------------------------------
+// xdot = A x + B u + b   (diffmpc/dynamics/linear_dynamics.py:46-70)
+template <int NX_, int NU_>
+struct LinearModel {
+  static constexpr int NX = NX_, NU = NU_;
+  static constexpr int NPARAM = NX * NX + NX * NU + NX;
+  template <class T>
+  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+#pragma unroll
+    for (int i = 0; i < NX; ++i) {
+      T acc = sc<T>(p[NX * NX + NX * NU + i]);
+#pragma unroll
+      for (int j = 0; j < NX; ++j) acc = acc + p[i * NX + j] * x[j];
+#pragma unroll
+      for (int k = 0; k < NU; ++k) acc = acc + p[NX * NX + i * NU + k] * u[k];
+      xd[i] = acc;
+    }
+  }
+};
+
+// Quadrotor, 13 states [p, v, q (scalar first), w], 4 controls [f, tau].
+// diffmpc/dynamics/quadrotor_dynamics.py:84-133.  Parameters: mass, J (3x3).
+struct QuadrotorModel {
+  static constexpr int NX = 13, NU = 4, NPARAM = 10;
+  template <class T>
+  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+    const float m = p[0];
+    const float* J = p + 1;
+    // inverse of J by the adjugate (J is constant: plain floats)
+    float c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
+    float det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+    float id = 1.0f / det;
+    float Ji[9] = {c00 * id, (J[2] * J[7] - J[1] * J[8]) * id, (J[1] * J[5] - J[2] * J[4]) * id,
+                   c01 * id, (J[0] * J[8] - J[2] * J[6]) * id, (J[2] * J[3] - J[0] * J[5]) * id,
+                   c02 * id, (J[1] * J[6] - J[0] * J[7]) * id, (J[0] * J[4] - J[1] * J[3]) * id};
+    T qw = x[6], qx = x[7], qy = x[8], qz = x[9];
+    T inv = sc<T>(1.0f) / dsqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+    qw = qw * inv; qx = qx * inv; qy = qy * inv; qz = qz * inv;
+    T wx = x[10], wy = x[11], wz = x[12];
+    T fm = u[0] * (1.0f / m);
+    xd[0] = x[3]; xd[1] = x[4]; xd[2] = x[5];
+    // v' = -g e3 + (f/m) R(q) e3
+    xd[3] = fm * (2.0f * (qx * qz + qw * qy));
+    xd[4] = fm * (2.0f * (qy * qz - qw * qx));
+    xd[5] = fm * (qw * qw - qx * qx - qy * qy + qz * qz) - 9.81f;
+    // q' = 1/2 L(q) [0; w]
+    xd[6] = 0.5f * (-(qx * wx) - qy * wy - qz * wz);
+    xd[7] = 0.5f * (qw * wx - qz * wy + qy * wz);
+    xd[8] = 0.5f * (qz * wx + qw * wy - qx * wz);
+    xd[9] = 0.5f * (qx * wy - qy * wx + qw * wz);
+    // w' = J^-1 ((J w) x w + tau)
+    T hx = J[0] * wx + J[1] * wy + J[2] * wz;
+    T hy = J[3] * wx + J[4] * wy + J[5] * wz;
+    T hz = J[6] * wx + J[7] * wy + J[8] * wz;
+    T ex = hy * wz - hz * wy + u[1];
+    T ey = hz * wx - hx * wz + u[2];
+    T ez = hx * wy - hy * wx + u[3];
+    xd[10] = Ji[0] * ex + Ji[1] * ey + Ji[2] * ez;
+    xd[11] = Ji[3] * ex + Ji[4] * ey + Ji[5] * ez;
+    xd[12] = Ji[6] * ex + Ji[7] * ey + Ji[8] * ez;
+  }
+};
+
+// Rate-only spacecraft: w' = J^-1 (tau - w x J w), diagonal J.
+// diffmpc/dynamics/spacecraft_dynamics.py:31-66.
+struct Spacecraft3Model {
+  static constexpr int NX = 3, NU = 3, NPARAM = 3;
+  template <class T>
+  __device__ static void f(const T* w, const T* u, const float* p, T* xd) {
+    T hx = p[0] * w[0], hy = p[1] * w[1], hz = p[2] * w[2];
+    xd[0] = (u[0] - (w[1] * hz - w[2] * hy)) * (1.0f / p[0]);
+    xd[1] = (u[1] - (w[2] * hx - w[0] * hz)) * (1.0f / p[1]);
+    xd[2] = (u[2] - (w[0] * hy - w[1] * hx)) * (1.0f / p[2]);
+  }
+};
+
+// Attitude spacecraft, 7 states [q (scalar first), w], 3 torques: the
+// normalised quaternion kinematics of quadrotor_dynamics.py:122,129 with
+// Euler's equation of spacecraft_dynamics.py:63 (diagonal J).
+struct Spacecraft7Model {
+  static constexpr int NX = 7, NU = 3, NPARAM = 3;
+  template <class T>
+  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+    T qw = x[0], qx = x[1], qy = x[2], qz = x[3];
+    T inv = sc<T>(1.0f) / dsqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+    qw = qw * inv; qx = qx * inv; qy = qy * inv; qz = qz * inv;
+    T wx = x[4], wy = x[5], wz = x[6];
+    xd[0] = 0.5f * (-(qx * wx) - qy * wy - qz * wz);
+    xd[1] = 0.5f * (qw * wx - qz * wy + qy * wz);
+    xd[2] = 0.5f * (qz * wx + qw * wy - qx * wz);
+    xd[3] = 0.5f * (qx * wy - qy * wx + qw * wz);
+    T hx = p[0] * wx, hy = p[1] * wy, hz = p[2] * wz;
+    xd[4] = (u[0] - (wy * hz - wz * hy)) * (1.0f / p[0]);
+    xd[5] = (u[1] - (wz * hx - wx * hz)) * (1.0f / p[1]);
+    xd[6] = (u[2] - (wx * hy - wy * hx)) * (1.0f / p[2]);
+  }
+};
+
+// Cart-pole (diffmpc/dynamics/cartpole_dynamics.py:36-80).
+// Parameters: masscart, masspole, length, gravity.
+struct CartpoleModel {
+  static constexpr int NX = 4, NU = 1, NPARAM = 4;
+  template <class T>
+  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+    const float mc = p[0], mp = p[1], len = p[2], g = p[3];
+    const float total = mc + mp, pml = mp * len;
+    T s = dsin(x[2]), c = dcos(x[2]);
+    T temp = (u[0] + pml * (x[3] * x[3]) * s) * (1.0f / total);
+    T th_acc = (g * s - c * temp) / (len * (sc<T>(4.0f / 3.0f) - (mp / total) * (c * c)));
+    T x_acc = temp - (pml / total) * (th_acc * c);
+    xd[0] = x[1];
+    xd[1] = x_acc;
+    xd[2] = x[3];
+    xd[3] = th_acc;
+  }
+};
+
+// Planar chain of 12 unit masses hanging from a wall; link j (spring k_j,
+// damper c_j, zero rest length) joins mass j-1 and j; u_j pushes mass j
+// along x.  State [p_1x, p_1y, ..., p_12y, v_1x, ..., v_12y].
+struct ChainModel {
+  static constexpr int M = 12;
+  static constexpr int NX = 4 * M, NU = M, NPARAM = 2 * M;
+  template <class T>
+  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+#pragma unroll
+    for (int i = 0; i < 2 * M; ++i) xd[i] = x[2 * M + i];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        T dp = x[2 * j + d] - (j > 0 ? x[2 * (j - 1) + d] : sc<T>(0.0f));
+        T dv = x[2 * M + 2 * j + d] - (j > 0 ? x[2 * M + 2 * (j - 1) + d] : sc<T>(0.0f));
+        T fj = -(p[j] * dp) - p[M + j] * dv;
+        T acc = fj;
+        if (j + 1 < M) {
+          T dp1 = x[2 * (j + 1) + d] - x[2 * j + d];
+          T dv1 = x[2 * M + 2 * (j + 1) + d] - x[2 * M + 2 * j + d];
+          acc = acc + p[j + 1] * dp1 + p[M + j + 1] * dv1;
+        }
+        if (d == 0) acc = acc + u[j];
+        xd[2 * M + 2 * j + d] = acc;
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// One integration step (diffmpc/dynamics/integrators.py:20-71):
+// scheme 0 = Euler, 1 = midpoint, 2 = RK4.
+template <class Model, class T>
+__device__ void step(const T* x, const T* u, const float* p, float dt, int scheme, T* out) {
+  constexpr int NX = Model::NX;
+  T k1[NX], xt[NX];
+  Model::f(x, u, p, k1);
+  if (scheme == 0) {
+#pragma unroll
+    for (int i = 0; i < NX; ++i) out[i] = x[i] + dt * k1[i];
+    return;
+  }
+  if (scheme == 1) {
+#pragma unroll
+    for (int i = 0; i < NX; ++i) xt[i] = x[i] + (0.5f * dt) * k1[i];
+    T k2[NX];
+    Model::f(xt, u, p, k2);
+#pragma unroll
+    for (int i = 0; i < NX; ++i) out[i] = x[i] + dt * k2[i];
+    return;
+  }
+  T acc[NX];
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    acc[i] = k1[i];
+    xt[i] = x[i] + (0.5f * dt) * k1[i];
+  }
+  T k[NX];
+  Model::f(xt, u, p, k);
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    acc[i] = acc[i] + 2.0f * k[i];
+    xt[i] = x[i] + (0.5f * dt) * k[i];
+  }
+  Model::f(xt, u, p, k);
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    acc[i] = acc[i] + 2.0f * k[i];
+    xt[i] = x[i] + dt * k[i];
+  }
+  Model::f(xt, u, p, k);
+#pragma unroll
+  for (int i = 0; i < NX; ++i) out[i] = x[i] + (acc[i] + k[i]) * (dt / 6.0f);
+}
+
+// Value and column j of the Jacobian d f_d / d[x; u] at (x, u) in float.
+// Mirrors value_and_jacfwd over next_state (optimal_control_problem.py:360-383)
+// with one forward-mode tangent e_j.
+template <class Model>
+__device__ void step_jacobian_column(const float* x, const float* u, const float* p, float dt, int scheme, int j,
+                                     float* fval, float* col) {
+  constexpr int NX = Model::NX, NU = Model::NU;
+  Dual xd[NX], ud[NU], out[NX];
+#pragma unroll
+  for (int i = 0; i < NX; ++i) xd[i] = Dual{x[i], (i == j) ? 1.0f : 0.0f};
+#pragma unroll
+  for (int k = 0; k < NU; ++k) ud[k] = Dual{u[k], (NX + k == j) ? 1.0f : 0.0f};
+  step<Model, Dual>(xd, ud, p, dt, scheme, out);
+#pragma unroll
+  for (int i = 0; i < NX; ++i) {
+    fval[i] = out[i].v;
+    col[i] = out[i].d;
+  }
+}
+
+}  // namespace dmpc

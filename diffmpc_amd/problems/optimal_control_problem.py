@@ -1,0 +1,181 @@
+"""Optimal control problem (diffmpc/problems/optimal_control_problem.py:16-398).
+
+Holds the dynamics and the parameter dict and offers the host-side pieces of
+the reference class: sizes, the straight-line initial guess, the cost and the
+equality constraints (torch, any device, batched over leading axes).  The
+linearization itself runs on the GPU inside the solver; ``linearize`` exposes
+it for parity checks.
+"""
+import copy
+from typing import Any, Dict, Tuple
+
+import torch
+
+from .. import _lib
+from ..dynamics.base_dynamics import Dynamics, batched
+from ..dynamics.integrators import DiscretizationScheme, predict_next_state
+from ..utils.load_params import check_parameters_dictionary_or_raise_errors
+
+DYNAMICS_PARAM_KEYS = {
+    "linear": ("A", "B", "b"),
+}
+
+
+def _as_tensor(v, device=None, dtype=torch.float64):
+    return torch.as_tensor(v, dtype=dtype, device=device)
+
+
+class OptimalControlProblem:
+    """Optimal control problem class."""
+
+    def __init__(self, dynamics: Dynamics, params: Dict[str, Any] = None, check_parameters_are_valid: bool = True):
+        self._dynamics = dynamics
+        if check_parameters_are_valid:
+            check_parameters_dictionary_or_raise_errors(params)
+        params = copy.deepcopy(params)
+        self._params = params
+        self._discretization_scheme = DiscretizationScheme(int(params["discretization_scheme"]))
+        self._num_variables = (self.num_control_variables + self.num_state_variables) * (self.horizon + 1)
+        self._num_constraints = self.num_state_variables * (self.horizon + 1)
+        self.name = type(self).__name__
+
+    @property
+    def dynamics(self) -> Dynamics:
+        return self._dynamics
+
+    @property
+    def params(self) -> Dict:
+        return self._params
+
+    @property
+    def horizon(self) -> int:
+        return int(self.params["horizon"])
+
+    @property
+    def discretization_scheme(self) -> DiscretizationScheme:
+        return self._discretization_scheme
+
+    @property
+    def num_variables(self) -> int:
+        return self._num_variables
+
+    @property
+    def num_constraints(self) -> int:
+        return self._num_constraints
+
+    @property
+    def num_state_variables(self) -> int:
+        return self.dynamics.num_states
+
+    @property
+    def num_control_variables(self) -> int:
+        return self.dynamics.num_controls
+
+    def initial_guess(self, params: Dict[str, Any] = None, device=None) -> Tuple[torch.Tensor, torch.Tensor]:
+        """Straight line from initial to final state plus 1e-6; controls 1e-6 (ocp.py:82-102).
+
+        A batched ``initial_state`` (B, nx) gives (B, N+1, nx) / (B, N+1, nu).
+        """
+        if params is None:
+            params = self.params
+        x0 = _as_tensor(params["initial_state"], device)
+        xf = _as_tensor(params["final_state"], device)
+        N = self.horizon
+        t = torch.arange(N + 1, dtype=torch.float64, device=x0.device).unsqueeze(-1)
+        a1 = (N - t) / N
+        a2 = t / N
+        if x0.dim() == 2 or xf.dim() == 2:
+            x0 = x0 if x0.dim() == 2 else x0.unsqueeze(0)
+            xf = xf if xf.dim() == 2 else xf.unsqueeze(0)
+            states = x0.unsqueeze(1) * a1 + xf.unsqueeze(1) * a2 + 1e-6
+            controls = torch.zeros(states.shape[0], N + 1, self.num_control_variables, dtype=torch.float64,
+                                   device=x0.device) + 1e-6
+        else:
+            states = x0 * a1 + xf * a2 + 1e-6
+            controls = torch.zeros(N + 1, self.num_control_variables, dtype=torch.float64, device=x0.device) + 1e-6
+        return states, controls
+
+    def equality_constraints(self, states, controls, params: Dict[str, Any]) -> torch.Tensor:
+        """g = [x_0 - x_init; f_d(x_t, u_t) - x_{t+1}] flattened (ocp.py:104-151)."""
+        N = self.horizon
+        x0 = _as_tensor(params["initial_state"], states.device)
+        nxt = predict_next_state(self.dynamics, float(params["discretization_resolution"]), self.discretization_scheme,
+                                 params, states[..., :N, :], controls[..., :N, :])
+        g0 = states[..., 0, :] - x0
+        gd = (nxt - states[..., 1:, :]).reshape(states.shape[:-2] + (-1,))
+        return torch.cat([g0, gd], -1)
+
+    def cost(self, states, controls, params: Dict[str, Any]) -> torch.Tensor:
+        """Sum of w (z - ref)^2 over steps plus (w_x + w_f)(x_N - x_final)^2, no 1/2 (ocp.py:153-247)."""
+        N = self.horizon
+        dev = states.device
+        xref = _as_tensor(params["reference_state_trajectory"], dev)
+        uref = _as_tensor(params["reference_control_trajectory"], dev)
+        if not self.params["penalize_control_reference"]:
+            uref = torch.zeros_like(uref)
+        wx = _as_tensor(params["weights_penalization_reference_state_trajectory"], dev)
+        wu = _as_tensor(params["weights_penalization_control_squared"], dev)
+        wf = _as_tensor(params["weights_penalization_final_state"], dev)
+        xf = _as_tensor(params["final_state"], dev)
+        c = (wx.unsqueeze(-2) * (states[..., :N, :] - xref[..., :N, :]) ** 2).sum((-1, -2))
+        c = c + (wu.unsqueeze(-2) * (controls[..., :N, :] - uref[..., :N, :]) ** 2).sum((-1, -2))
+        c = c + ((wx + wf) * (states[..., N, :] - xf) ** 2).sum(-1)
+        return c
+
+    def device_problem(self, params: Dict[str, Any], batch: int, device) -> Dict[str, torch.Tensor]:
+        """Batched, contiguous device tensors of the C ABI's dmpc_problem_data."""
+        N, nx, nu = self.horizon, self.num_state_variables, self.num_control_variables
+
+        def b(key, nd):
+            return batched(params[key], batch, nd, torch.float64, device)
+
+        return dict(
+            x0=b("initial_state", 1), x_final=b("final_state", 1),
+            x_ref=b("reference_state_trajectory", 2), u_ref=b("reference_control_trajectory", 2),
+            w_x=b("weights_penalization_reference_state_trajectory", 1),
+            w_u=b("weights_penalization_control_squared", 1),
+            w_f=b("weights_penalization_final_state", 1),
+            dyn=self.dynamics.flat_params(params, batch, device),
+        )
+
+    def c_problem(self, batch: int) -> "_lib.Problem":
+        p = _lib.Problem()
+        p.model = self.dynamics.model_id
+        p.nx = self.num_state_variables
+        p.nu = self.num_control_variables
+        p.horizon = self.horizon
+        p.batch = batch
+        p.scheme = int(self.discretization_scheme)
+        p.n_params = self.dynamics.num_params
+        p.penalize_control_reference = int(bool(self.params["penalize_control_reference"]))
+        p.dt = float(torch.as_tensor(self.params["discretization_resolution"]).reshape(-1)[0])
+        return p
+
+    def linearize(self, states, controls, params: Dict[str, Any] = None):
+        """As_next, As, Bs, Cs at (states, controls) on the GPU (ocp.py:325-398).
+
+        states (B, N+1, nx), controls (B, N+1, nu) device tensors.
+        Returns float64 tensors As_next (B,N,nx,nx), As, Bs (B,N,nx,nu), Cs (B,N+1,nx).
+        """
+        if params is None:
+            params = self.params
+        lib = _lib.load()
+        B = states.shape[0]
+        N, nx, nu = self.horizon, self.num_state_variables, self.num_control_variables
+        dev = states.device
+        d = self.device_problem(params, B, dev)
+        d = {k: v.contiguous() for k, v in d.items()}
+        data = _lib.ProblemData(_lib.ptr(d["x0"]), _lib.ptr(d["x_final"]), _lib.ptr(d["x_ref"]), _lib.ptr(d["u_ref"]),
+                                _lib.ptr(d["w_x"]), _lib.ptr(d["w_u"]), _lib.ptr(d["w_f"]), _lib.ptr(d["dyn"]))
+        fd = torch.empty(B, N, nx, dtype=torch.float64, device=dev)
+        A = torch.empty(B, N, nx, nx, dtype=torch.float64, device=dev)
+        Bm = torch.empty(B, N, nx, nu, dtype=torch.float64, device=dev)
+        C = torch.empty(B, N + 1, nx, dtype=torch.float64, device=dev)
+        prob = self.c_problem(B)
+        xs = states.to(torch.float64).contiguous()
+        us = controls.to(torch.float64).contiguous()
+        rc = lib.dmpc_linearize(prob, data, _lib.ptr(xs), _lib.ptr(us), _lib.ptr(fd), _lib.ptr(A), _lib.ptr(Bm),
+                                _lib.ptr(C), _lib.stream_ptr(dev))
+        _lib.check(rc, "dmpc_linearize")
+        As_next = -torch.eye(nx, dtype=torch.float64, device=dev).expand(B, N, nx, nx).contiguous()
+        return As_next, A, Bm, C

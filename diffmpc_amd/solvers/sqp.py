@@ -1,0 +1,363 @@
+"""Batched SQP solver with an implicit-differentiation backward, on the GPU.
+
+Mirrors diffmpc/solvers/sqp.py: ``SQPSolver(program, name, params)``,
+``solver.initial_guess(params)``, ``solver.solve(initial_guess, problem_params,
+weights) -> SQPSolution`` and the ``SolverReturnStatus`` codes.  Differences
+that follow from the batch axis being explicit here instead of a caller's
+``jax.vmap``:
+
+* tensors carry a leading batch axis B (an unbatched guess is accepted and
+  the result is returned unbatched);
+* problem parameters may be shared (no batch axis) or per instance;
+* gradients flow, through ``torch.autograd``, to the entries of ``weights``
+  only (problem_params are stop-gradient, sqp.py:456-458); shared weights get
+  the batch-summed gradient, like ``vmap(in_axes=None)``.
+
+The solve runs in libdmpc.so: mode "fused" (default) is one persistent
+workgroup per instance per SQP iteration, "plain" is the per-launch op graph
+of the reference.  Solver parameters use the reference's keys plus the
+optional ``"mode"``.
+"""
+import copy
+import enum
+from typing import Any, Dict, NamedTuple, Optional
+
+import torch
+
+from .. import _lib
+from ..dynamics.base_dynamics import batched
+from ..problems.optimal_control_problem import OptimalControlProblem
+from ..utils.load_params import load_solver_params
+from .pcg import (PCGOptimalControl, QPCostParameters, QPDynamicsParameters, QPParameters,
+                  SchurComplementMatrices)
+
+DEFAULT_SOLVER_PARAMS = load_solver_params("sqp.yaml")
+
+# weights keys the backward differentiates (closed-form mixed derivatives)
+WEIGHT_KEYS = {
+    "weights_penalization_reference_state_trajectory": "w_x",
+    "weights_penalization_control_squared": "w_u",
+    "weights_penalization_final_state": "w_f",
+    "initial_state": "x0",
+    "reference_state_trajectory": "x_ref",
+    "reference_control_trajectory": "u_ref",
+    "final_state": "x_final",
+}
+
+
+class SolverReturnStatus(enum.IntEnum):
+    SUCCESS = 0
+    ERROR = -100  # NaN detected or not converged
+
+
+class SavedLinearization(NamedTuple):
+    """QP data of the last linearization point (sqp.py:487-492)."""
+
+    AB: torch.Tensor  # (B, N, nx, nx+nu) float32: [A_t | B_t]
+    fd: torch.Tensor  # (B, N, nx) float32: f_d(x_t, u_t)
+    qinv: torch.Tensor  # (B, N+1, nx) float64: diag Qinv_t
+    rinv: torch.Tensor  # (B, nu) float64: diag Rinv
+
+
+class SQPSolution(NamedTuple):
+    states: torch.Tensor
+    controls: torch.Tensor
+    dual: torch.Tensor
+    qp_parameters: Optional[SavedLinearization]
+    dynamics_cost_premultiplied_matrices: Any
+    schur_complement_matrices: Any
+    status: Any
+    num_iter: Any
+    convergence_error: Any
+    convergence_errors: torch.Tensor
+    linesearch_alphas: torch.Tensor
+    num_pcg_iters: torch.Tensor
+    dual_backward_guess: torch.Tensor = None
+
+
+def _c_solver_params(params: Dict[str, Any]) -> "_lib.SolverParams":
+    sp = _lib.SolverParams()
+    sp.num_scp_iteration_max = int(params["num_scp_iteration_max"])
+    sp.tol_convergence = float(params["tol_convergence"])
+    sp.linesearch = int(bool(params["linesearch"]))
+    sp.linesearch_eta = float(params["linesearch_eta"])
+    alphas = [float(a) for a in params["linesearch_alphas"]]
+    if len(alphas) > 8:
+        raise ValueError("at most 8 line-search candidates are supported")
+    sp.n_alphas = len(alphas)
+    for i, a in enumerate(alphas):
+        sp.linesearch_alphas[i] = a
+    sp.pcg_max_iter = int(params["pcg"]["max_iter"])
+    sp.pcg_tol_epsilon = float(params["pcg"]["tol_epsilon"])
+    sp.warm_start_backward = int(bool(params["warm_start_backward"]))
+    return sp
+
+
+def _mode(params) -> int:
+    m = params.get("mode", "fused")
+    if m not in ("fused", "plain"):
+        raise ValueError(f"mode must be 'fused' or 'plain', got {m!r}")
+    return _lib.MODE_FUSED if m == "fused" else _lib.MODE_PLAIN
+
+
+class _SQPSolveFn(torch.autograd.Function):
+    """custom_vjp of sqp.py:562-802 as a torch.autograd.Function."""
+
+    @staticmethod
+    def forward(ctx, solver, guess, x0, x_final, x_ref, u_ref, w_x, w_u, w_f, dyn):
+        lib = _lib.load()
+        prog = solver.program
+        B = guess["states"].shape[0]
+        dev = guess["states"].device
+        N, nx, nu = prog.horizon, prog.num_state_variables, prog.num_control_variables
+        maxit = solver.maxiter
+        data_t = [t.contiguous() for t in (x0, x_final, x_ref, u_ref, w_x, w_u, w_f)] + [dyn.contiguous()]
+        data = _lib.ProblemData(*[_lib.ptr(t) for t in data_t])
+        x = guess["states"].to(torch.float64).clone().contiguous()
+        u = guess["controls"].to(torch.float64).clone().contiguous()
+        lam = guess["dual"].to(torch.float64).clone().contiguous()
+        status = torch.zeros(B, dtype=torch.int32, device=dev)
+        num_iter = torch.zeros(B, dtype=torch.int32, device=dev)
+        conv_err = guess["convergence_error"].to(torch.float64).clone().contiguous()
+        conv_errs = guess["convergence_errors"].to(torch.float64).clone().contiguous()
+        ls_alphas = guess["linesearch_alphas"].to(torch.float64).clone().contiguous()
+        pcg_iters = guess["num_pcg_iters"].to(torch.int32).clone().contiguous()
+        AB = torch.zeros(B, N, nx, nx + nu, dtype=torch.float32, device=dev)
+        fd = torch.zeros(B, N, nx, dtype=torch.float32, device=dev)
+        qinv = torch.zeros(B, N + 1, nx, dtype=torch.float64, device=dev)
+        rinv = torch.zeros(B, nu, dtype=torch.float64, device=dev)
+        sol = _lib.Solution(_lib.ptr(x), _lib.ptr(u), _lib.ptr(lam), _lib.ptr(status), _lib.ptr(num_iter),
+                            _lib.ptr(conv_err), _lib.ptr(conv_errs), _lib.ptr(ls_alphas), _lib.ptr(pcg_iters))
+        saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv))
+        prob = prog.c_problem(B)
+        sp = _c_solver_params(solver.params)
+        mode = _mode(solver.params)
+        rc = lib.dmpc_sqp_solve(prob, sp, data, sol, saved, mode, _lib.stream_ptr(dev))
+        _lib.check(rc, "dmpc_sqp_solve")
+        dual_bwd_in = guess.get("dual_backward_guess")
+        dual_bwd_out = (dual_bwd_in.to(torch.float64).clone() if dual_bwd_in is not None
+                        else torch.zeros(B, N + 1, nx, dtype=torch.float64, device=dev)).contiguous()
+        ctx.solver = solver
+        ctx.prob_args = (prob, sp, mode)
+        ctx.data_t = data_t
+        ctx.saved_lin = (AB, fd, qinv, rinv)
+        ctx.dual_bwd_in = dual_bwd_in
+        ctx.dual_bwd_out = dual_bwd_out
+        ctx.save_for_backward(x, u)
+        ctx.mark_non_differentiable(lam, status, num_iter, conv_err, conv_errs, ls_alphas, pcg_iters, AB, fd, qinv,
+                                    rinv, dual_bwd_out)
+        return x, u, lam, status, num_iter, conv_err, conv_errs, ls_alphas, pcg_iters, AB, fd, qinv, rinv, dual_bwd_out
+
+    @staticmethod
+    def backward(ctx, g_x, g_u, *unused):
+        lib = _lib.load()
+        x, u = ctx.saved_tensors
+        prob, sp, mode = ctx.prob_args
+        B = x.shape[0]
+        N = prob.horizon
+        nx, nu = prob.nx, prob.nu
+        dev = x.device
+        dl_dx = (g_x if g_x is not None else torch.zeros_like(x)).to(torch.float64).contiguous()
+        dl_du = (g_u if g_u is not None else torch.zeros_like(u)).to(torch.float64).contiguous()
+        # adjoint warm start: the cotangent that reached this solve's output
+        # dual_backward_guess (written by the backward of the solve that used
+        # this solution as its guess), sqp.py:748-752
+        if ctx.solver.params["warm_start_backward"]:
+            lam_b = ctx.dual_bwd_out.clone().contiguous()
+        else:
+            lam_b = torch.zeros(B, N + 1, nx, dtype=torch.float64, device=dev)
+        g = {k: torch.empty(s, dtype=torch.float64, device=dev) for k, s in (
+            ("w_x", (B, nx)), ("w_u", (B, nu)), ("w_f", (B, nx)), ("x0", (B, nx)), ("x_ref", (B, N + 1, nx)),
+            ("u_ref", (B, N + 1, nu)), ("x_final", (B, nx)))}
+        iters = torch.zeros(B, dtype=torch.int32, device=dev)
+        grads = _lib.Grads(_lib.ptr(g["w_x"]), _lib.ptr(g["w_u"]), _lib.ptr(g["w_f"]), _lib.ptr(g["x0"]),
+                           _lib.ptr(g["x_ref"]), _lib.ptr(g["u_ref"]), _lib.ptr(g["x_final"]), _lib.ptr(lam_b),
+                           _lib.ptr(iters))
+        data = _lib.ProblemData(*[_lib.ptr(t) for t in ctx.data_t])
+        AB, fd, qinv, rinv = ctx.saved_lin
+        saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv))
+        rc = lib.dmpc_sqp_backward(prob, sp, data, _lib.ptr(x), _lib.ptr(u), saved, _lib.ptr(dl_dx), _lib.ptr(dl_du),
+                                   grads, mode, _lib.stream_ptr(dev))
+        _lib.check(rc, "dmpc_sqp_backward")
+        if ctx.dual_bwd_in is not None:
+            with torch.no_grad():
+                ctx.dual_bwd_in.copy_(lam_b.to(ctx.dual_bwd_in.dtype))
+        ctx.solver.last_backward_pcg_iters = iters
+        return (None, None, g["x0"], g["x_final"], g["x_ref"], g["u_ref"], g["w_x"], g["w_u"], g["w_f"], None)
+
+
+class SQPSolver:
+    """SQP for optimal control (equality constraints only) on the GPU."""
+
+    _supported_program_types = [OptimalControlProblem]
+
+    def __init__(self, program: OptimalControlProblem, name: str = "SQPSolver",
+                 params: Dict = DEFAULT_SOLVER_PARAMS):
+        if params["verbose"]:
+            print("Initializing Solver with\n> name    =", name, "\n> program =", program)
+        if not any(isinstance(program, t) for t in self._supported_program_types):
+            raise NotImplementedError(str(getattr(program, "name", program)) + " is not supported.")
+        self._verbose = params["verbose"]
+        self._params = copy.deepcopy(params)
+        self._program = program
+        self._name = name
+        self._pcg = PCGOptimalControl(program.horizon, program.num_state_variables, program.num_control_variables,
+                                      self._params["pcg"])
+        self.last_backward_pcg_iters = None
+        self.solve = self._differentiable_solve
+
+    @property
+    def name(self) -> str:
+        return self._name
+
+    @property
+    def params(self) -> Dict:
+        return self._params
+
+    @property
+    def problem_params(self) -> Dict:
+        return self._program.params
+
+    @property
+    def maxiter(self) -> int:
+        return int(self.params["num_scp_iteration_max"])
+
+    @property
+    def program(self) -> OptimalControlProblem:
+        return self._program
+
+    @property
+    def verbose(self) -> bool:
+        return self._verbose
+
+    @property
+    def pcg(self) -> PCGOptimalControl:
+        return self._pcg
+
+    def initial_guess_primal(self, params: Dict[str, Any] = None, device=None):
+        return self.program.initial_guess(params, device=device)
+
+    def initial_guess_dual(self, batch: Optional[int] = None, device=None) -> torch.Tensor:
+        shape = (self.program.horizon + 1, self.program.num_state_variables)
+        if batch is not None:
+            shape = (batch,) + shape
+        return torch.zeros(shape, dtype=torch.float64, device=device)
+
+    def initial_guess(self, params: Dict[str, Any] = None, device=None) -> SQPSolution:
+        """Straight-line guess (sqp.py:143-158); batched if initial_state is (B, nx)."""
+        x, u = self.initial_guess_primal(params, device=device)
+        batch = x.shape[0] if x.dim() == 3 else None
+        lead = (batch,) if batch is not None else ()
+        dev = x.device
+        return SQPSolution(
+            x, u, self.initial_guess_dual(batch, dev), None, None, None,
+            SolverReturnStatus.SUCCESS, 0, 0.0,
+            torch.zeros(lead + (self.maxiter,), dtype=torch.float64, device=dev),
+            torch.zeros(lead + (self.maxiter,), dtype=torch.float64, device=dev),
+            torch.zeros(lead + (self.maxiter,), dtype=torch.int32, device=dev),
+            self.initial_guess_dual(batch, dev),
+        )
+
+    def make_params_with_weights(self, weights, problem_params=None):
+        new_params = copy.copy(self.problem_params if problem_params is None else problem_params)
+        for key, value in weights.items():
+            new_params[key] = value
+        return new_params
+
+    def _differentiable_solve(self, initial_guess: SQPSolution, problem_params: Dict[str, Any] = None,
+                              weights: Dict[str, Any] = None, device=None) -> SQPSolution:
+        """Solve the batch of problems (sqp.py:436-560, 572-587)."""
+        if problem_params is None:
+            problem_params = self.problem_params
+        weights = {} if weights is None else weights
+        for k in weights:
+            if k not in WEIGHT_KEYS:
+                raise NotImplementedError(f"no gradient rule for weights key {k!r}; differentiable keys: "
+                                          f"{sorted(WEIGHT_KEYS)}")
+        prog = self.program
+        N, nx, nu = prog.horizon, prog.num_state_variables, prog.num_control_variables
+        states = torch.as_tensor(initial_guess.states)
+        unbatched = states.dim() == 2
+        if device is None:
+            device = states.device if states.is_cuda else torch.device("cuda")
+
+        def lead(t, nd, dtype=torch.float64):
+            t = torch.as_tensor(t, device=device)
+            t = t.unsqueeze(0) if unbatched and t.dim() == nd else t
+            return t.to(dtype)
+
+        g = {
+            "states": lead(initial_guess.states, 2),
+            "controls": lead(initial_guess.controls, 2),
+            "dual": lead(initial_guess.dual, 2),
+        }
+        B = g["states"].shape[0]
+        if g["states"].shape[1:] != (N + 1, nx) or g["controls"].shape[1:] != (N + 1, nu):
+            raise ValueError("initial guess must have (B, N+1, nx) states and (B, N+1, nu) controls")
+
+        def hist(v, dtype):
+            t = torch.as_tensor(v, device=device)
+            if t.dim() == 0 or t.numel() == 0:
+                return torch.zeros(B, self.maxiter, dtype=dtype, device=device)
+            t = t.reshape(-1, t.shape[-1]) if t.dim() >= 1 else t
+            return t.expand(B, self.maxiter).to(dtype) if t.shape[0] == 1 else t.to(dtype)
+
+        ce = torch.as_tensor(initial_guess.convergence_error, dtype=torch.float64, device=device).reshape(-1)
+        g["convergence_error"] = ce.expand(B) if ce.numel() == 1 else ce
+        g["convergence_errors"] = hist(initial_guess.convergence_errors, torch.float64)
+        g["linesearch_alphas"] = hist(initial_guess.linesearch_alphas, torch.float64)
+        g["num_pcg_iters"] = hist(initial_guess.num_pcg_iters, torch.int32)
+        if initial_guess.dual_backward_guess is not None:
+            dbg = torch.as_tensor(initial_guess.dual_backward_guess)
+            g["dual_backward_guess"] = dbg if (dbg.dim() == 3 and dbg.is_cuda) else lead(dbg, 2)
+        else:
+            g["dual_backward_guess"] = None
+
+        # problem_params are stop-gradient (sqp.py:456-457); weights override them
+        params = {k: (v.detach() if isinstance(v, torch.Tensor) else v) for k, v in problem_params.items()}
+        params = self.make_params_with_weights(weights, params)
+
+        def b(key, nd):
+            return batched(params[key], B, nd, torch.float64, device)
+
+        x0 = b("initial_state", 1)
+        xf = b("final_state", 1)
+        xref = b("reference_state_trajectory", 2)
+        uref = b("reference_control_trajectory", 2)
+        wx = b("weights_penalization_reference_state_trajectory", 1)
+        wu = b("weights_penalization_control_squared", 1)
+        wf = b("weights_penalization_final_state", 1)
+        dyn = prog.dynamics.flat_params({k: (v.detach() if isinstance(v, torch.Tensor) else v)
+                                         for k, v in params.items()}, B, device)
+        out = _SQPSolveFn.apply(self, g, x0, xf, xref, uref, wx, wu, wf, dyn)
+        (x, u, lam, status, num_iter, conv_err, conv_errs, ls_alphas, pcg_iters, AB, fd, qinv, rinv,
+         dual_bwd_out) = out
+        saved = SavedLinearization(AB, fd, qinv, rinv)
+        if unbatched:
+            x, u, lam = x[0], u[0], lam[0]
+            status, num_iter, conv_err = status[0], num_iter[0], conv_err[0]
+            conv_errs, ls_alphas, pcg_iters, dual_bwd_out = conv_errs[0], ls_alphas[0], pcg_iters[0], dual_bwd_out[0]
+        return SQPSolution(x, u, lam, saved, None, None, status, num_iter, conv_err, conv_errs, ls_alphas,
+                           pcg_iters, dual_bwd_out)
+
+    def qp_parameters(self, solution: SQPSolution) -> QPParameters:
+        """Dense QP blocks of the last linearization point of a batched solution
+        (sqp.py:230-260); C and q, r are not kept by the solve and come back zero."""
+        s = solution.qp_parameters
+        B, N, nx, nz = s.AB.shape
+        nu = nz - nx
+        dev = s.AB.device
+        As = s.AB[..., :nx].to(torch.float64)
+        Bs = s.AB[..., nx:].to(torch.float64)
+        As_next = -torch.eye(nx, dtype=torch.float64, device=dev).expand(B, N, nx, nx).contiguous()
+        Qinv = torch.diag_embed(s.qinv)
+        Rinv = torch.diag_embed(s.rinv.unsqueeze(1).expand(B, N, nu))
+        z = torch.zeros
+        return QPParameters(QPDynamicsParameters(As_next, As.contiguous(), Bs.contiguous(),
+                                                 z(B, N + 1, nx, dtype=torch.float64, device=dev)),
+                            QPCostParameters(Qinv.contiguous(), z(B, N + 1, nx, dtype=torch.float64, device=dev),
+                                             Rinv.contiguous(), z(B, N, nu, dtype=torch.float64, device=dev)))
+
+    def schur_complement_matrices(self, solution: SQPSolution) -> SchurComplementMatrices:
+        """Explicit S, Phiinv (reference layout) of the last linearization point."""
+        return self.pcg.compute_S_Phiinv(self.qp_parameters(solution))

@@ -1,0 +1,162 @@
+/* dmpc.h -- C ABI of the MI355X batched SQP/PCG solver (libdmpc.so).
+ *
+ * All data pointers are DEVICE pointers to contiguous, batch-major arrays;
+ * every call is asynchronous on the given HIP stream (passed as void*) and
+ * returns DMPC_OK or a negative error code.  Nothing here allocates on the
+ * solve path except the per-launch (plain) mode's scratch, which it frees
+ * before returning.  N is the reference's `horizon`: trajectories have N+1
+ * knots and the last control row is zero.
+ *
+ * Reference interfaces replaced (ToyotaResearchInstitute/diffmpc):
+ *   dmpc_sqp_solve      SQPSolver.solve / custom_vjp primal   diffmpc/solvers/sqp.py:572-603 (+ _solve :436-560)
+ *   dmpc_sqp_backward   custom_vjp backward                    diffmpc/solvers/sqp.py:668-798
+ *   dmpc_linearize      get_dynamics_linearized_matrices_...   diffmpc/problems/optimal_control_problem.py:325-398
+ *   dmpc_pcg_assemble   PCGOptimalControl.compute_S_Phiinv     diffmpc/solvers/pcg.py:160-243
+ *   dmpc_pcg_gamma      PCGOptimalControl.compute_gamma        diffmpc/solvers/pcg.py:245-288
+ *   dmpc_pcg_solve      PCGOptimalControl.solve_KKT_Schur      diffmpc/solvers/pcg.py:350-445
+ *   dmpc_pcg_recover    get_states_controls_from_kkt_mult...   diffmpc/solvers/pcg.py:290-348
+ */
+#ifndef DMPC_H_
+#define DMPC_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMPC_OK 0
+#define DMPC_ERR_ARG (-1)          /* null pointer, bad size or option */
+#define DMPC_ERR_UNSUPPORTED (-2)  /* model / size / option not built in this mode */
+#define DMPC_ERR_HIP (-3)          /* a HIP call failed */
+
+/* dynamics models; flat per-instance parameter layouts (float32, n_params) */
+#define DMPC_MODEL_LINEAR 0      /* A (nx*nx, row-major), B (nx*nu), b (nx)   linear_dynamics.py:46-70 */
+#define DMPC_MODEL_QUADROTOR 1   /* mass, inertia 3x3 row-major  (13/4)        quadrotor_dynamics.py:84-133 */
+#define DMPC_MODEL_SPACECRAFT3 2 /* inertia_vector (3)           (3/3)         spacecraft_dynamics.py:31-66 */
+#define DMPC_MODEL_SPACECRAFT7 3 /* inertia diagonal (3)         (7/3)  quaternion + rate, no reference counterpart */
+#define DMPC_MODEL_CARTPOLE 4    /* masscart, masspole, length, gravity (4/1)  cartpole_dynamics.py:36-80 */
+#define DMPC_MODEL_CHAIN 5       /* spring k[12], damper c[12]   (48/12) planar chain, no reference counterpart */
+
+#define DMPC_MODE_FUSED 0 /* one persistent workgroup per instance for the whole SQP solve */
+#define DMPC_MODE_PLAIN 1 /* one kernel per operation, host-driven loops (the reference's op graph) */
+
+typedef struct dmpc_problem {
+  int32_t model;     /* DMPC_MODEL_* */
+  int32_t nx, nu;    /* must match the model (any for LINEAR with a built instantiation) */
+  int32_t horizon;   /* N >= 1 */
+  int32_t batch;     /* B >= 1 */
+  int32_t scheme;    /* 0 Euler, 1 midpoint, 2 RK4 (integrators.py:12-17) */
+  int32_t n_params;  /* floats per instance in dyn_params */
+  int32_t penalize_control_reference;
+  double dt;         /* discretization_resolution */
+} dmpc_problem;
+
+typedef struct dmpc_solver_params { /* keys of diffmpc/solvers/params/sqp.yaml */
+  int32_t num_scp_iteration_max;
+  double tol_convergence;
+  int32_t linesearch;
+  double linesearch_eta;
+  int32_t n_alphas; /* <= 8 */
+  double linesearch_alphas[8];
+  int32_t pcg_max_iter;
+  double pcg_tol_epsilon;
+  int32_t warm_start_backward;
+} dmpc_solver_params;
+
+typedef struct dmpc_problem_data {
+  const double* x0;        /* (B, nx)      initial_state */
+  const double* x_final;   /* (B, nx)      final_state */
+  const double* x_ref;     /* (B, N+1, nx) reference_state_trajectory */
+  const double* u_ref;     /* (B, N+1, nu) reference_control_trajectory */
+  const double* w_x;       /* (B, nx)      weights_penalization_reference_state_trajectory */
+  const double* w_u;       /* (B, nu)      weights_penalization_control_squared */
+  const double* w_f;       /* (B, nx)      weights_penalization_final_state */
+  const float* dyn_params; /* (B, n_params) */
+} dmpc_problem_data;
+
+typedef struct dmpc_solution { /* SQPSolution fields, sqp.py:38-57 */
+  double* states;             /* (B, N+1, nx)  in: initial guess, out: solution */
+  double* controls;           /* (B, N+1, nu)  in: initial guess, out: solution */
+  double* dual;               /* (B, N+1, nx)  in: multiplier guess, out: multipliers */
+  int32_t* status;            /* (B,) 0 SUCCESS, -100 ERROR */
+  int32_t* num_iter;          /* (B,) */
+  double* convergence_error;  /* (B,) */
+  double* convergence_errors; /* (B, maxit) in: carried from the guess, out */
+  double* linesearch_alphas;  /* (B, maxit) in/out */
+  int32_t* num_pcg_iters;     /* (B, maxit) in/out */
+} dmpc_solution;
+
+typedef struct dmpc_saved { /* QP data of the last linearization point (sqp.py:487-492) */
+  float* AB;    /* (B, N, nx, nx+nu): [A_t | B_t] */
+  float* fd;    /* (B, N, nx): f_d(x_t, u_t) */
+  double* qinv; /* (B, N+1, nx): diagonal of Qinv_t */
+  double* rinv; /* (B, nu): diagonal of Rinv_t (the same for every t) */
+} dmpc_saved;
+
+typedef struct dmpc_grads { /* d loss / d input, per instance */
+  double* w_x;     /* (B, nx) */
+  double* w_u;     /* (B, nu) */
+  double* w_f;     /* (B, nx) */
+  double* x0;      /* (B, nx) */
+  double* x_ref;   /* (B, N+1, nx) */
+  double* u_ref;   /* (B, N+1, nu) */
+  double* x_final; /* (B, nx) */
+  double* dual_backward; /* (B, N+1, nx) in: adjoint warm start, out: lambda~ */
+  int32_t* pcg_iters;    /* (B,) */
+} dmpc_grads;
+
+/* Library version and build target ("gfx950"). */
+const char* dmpc_version(void);
+
+/* 1 if the mode supports (model, nx, nu, horizon), else 0. */
+int dmpc_supported(int32_t mode, int32_t model, int32_t nx, int32_t nu, int32_t horizon);
+
+/* Batched SQP solve (sqp.py:436-560).  Solution arrays carry the initial guess in. */
+int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
+                   dmpc_solution* sol, dmpc_saved* saved, int32_t mode, void* stream);
+
+/* Implicit-differentiation backward (sqp.py:668-798): one adjoint PCG on the
+ * saved linearization with q = -dl/dx, r = -dl/du[:-1], C = 0, then the
+ * closed-form contraction with the cost's mixed derivatives. */
+int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
+                      const double* states, const double* controls, const dmpc_saved* saved, const double* dl_dstates,
+                      const double* dl_dcontrols, dmpc_grads* grads, int32_t mode, void* stream);
+
+/* f_d, A, B, C at (states, controls): fd (B,N,nx), A (B,N,nx,nx), Bm (B,N,nx,nu), C (B,N+1,nx). */
+int dmpc_linearize(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,
+                   const double* controls, double* fd, double* A, double* Bm, double* C, void* stream);
+
+/* PCGOptimalControl layer, dense general blocks, float64.
+ * As_next, As (B,N,nx,nx); Bs (B,N,nx,nu); Qinv (B,N+1,nx,nx); Rinv (B,N,nu,nu);
+ * S, Phiinv (B,N+1,nx,3nx). */
+int dmpc_pcg_assemble(int32_t B, int32_t N, int32_t nx, int32_t nu, const double* As_next, const double* As,
+                      const double* Bs, const double* Qinv, const double* Rinv, double* S, double* Phiinv,
+                      void* stream);
+/* gamma (B,N+1,nx) from Cs (B,N+1,nx), qs (B,N+1,nx), rs (B,N,nu). */
+int dmpc_pcg_gamma(int32_t B, int32_t N, int32_t nx, int32_t nu, const double* As_next, const double* As,
+                   const double* Bs, const double* Qinv, const double* Rinv, const double* Cs, const double* qs,
+                   const double* rs, double* gamma, void* stream);
+/* PCG on S lam = gamma; lam (B,N+1,nx) in: guess, out: solution; iters (B,), eta (B,).
+ * Synchronises the stream once per PCG iteration (host-driven loop). */
+int dmpc_pcg_solve(int32_t B, int32_t N, int32_t nx, const double* S, const double* Phiinv, const double* gamma,
+                   double* lam, int32_t max_iter, double tol_epsilon, int32_t* iters, double* eta, void* stream);
+/* states (B,N+1,nx), controls (B,N+1,nu) from lam. */
+int dmpc_pcg_recover(int32_t B, int32_t N, int32_t nx, int32_t nu, const double* lam, const double* As_next,
+                     const double* As, const double* Bs, const double* Qinv, const double* qs, const double* Rinv,
+                     const double* rs, double* states, double* controls, void* stream);
+
+/* Tuning: time blocks per thread group of the fused kernels (0 = automatic). */
+int dmpc_set_blocks_per_group(int32_t k);
+
+/* Instrumentation (no reference counterpart): when enabled, the library
+ * records a HIP event pair on the launch stream around every launch of the
+ * fused kernels; dmpc_profile_read sums the elapsed times of one kernel
+ * (0 = QP iteration, 1 = backward, 2 = linearize) and clears them. */
+int dmpc_profile_enable(int32_t on);
+int dmpc_profile_read(int32_t kernel, double* total_ms, int32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMPC_H_ */

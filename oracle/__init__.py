@@ -1,0 +1,28 @@
+"""FP64 CPU oracle for the diffmpc SQP/PCG hot path -- TEST INFRASTRUCTURE ONLY.
+
+This package restates, in batched float64 numpy, the algorithm of the reference
+library's SQP solver (diffmpc/solvers/sqp.py), its Schur-complement PCG QP
+solver (diffmpc/solvers/pcg.py), the optimal-control problem linearization
+(diffmpc/problems/optimal_control_problem.py) and the dynamics models
+(diffmpc/dynamics/*.py).  Each function cites the reference lines it follows.
+
+It is the checker, never the product: only ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` may import it.  The product path
+(``diffmpc_amd``) never imports or calls anything from here.
+
+Independent techniques are used on purpose so that a mistake shared with the GPU
+kernels is unlikely: Jacobians come from complex-step differentiation (the GPU
+uses forward-mode dual numbers), block inverses from LU (``np.linalg.inv``; the GPU
+uses Cholesky), and the whole QP path is pinned against a dense KKT solve.
+
+Parity pinning (what checks each function; see tests/test_oracle_*.py):
+  dynamics.*            reference invariants (skew, rotation, q_left), central FD
+  qp.compute_S_Phiinv   dense -G H^-1 G^T, symmetry, Phi^-1 == (-S)^-1 when phi=0
+  qp.compute_gamma      dense C + G H^-1 q
+  qp.solve_kkt_schur    dense KKT solve; warm-start invariant of the reference test
+  qp.recover            stationarity H z + q + G^T lam = 0
+  sqp.sqp_solve         LTI closed form (one iteration == exact QP), notebook value
+  sqp.sqp_backward      central finite differences of sqp_solve (FP64)
+  sqp.linesearch        parity unpinned beyond the alpha=[1.0] no-op and a
+                        hand-checked decrease rule on a 1-D case
+"""

@@ -1,0 +1,71 @@
+"""Shared problem set-ups for the tests (oracle and GPU sides)."""
+import numpy as np
+
+from oracle import dynamics as odyn
+from diffmpc_amd import workloads as W
+
+
+def oracle_problem(name, prob):
+    p = dict(prob)
+    if name == "linear":
+        p["model"] = odyn.make_model("linear", prob["x0"].shape[1], prob["u_ref"].shape[2])
+    else:
+        p["model"] = odyn.make_model(name)
+    return p
+
+
+def notebook_spacecraft():
+    """notebooks/RL_spacecraft.ipynb cells 5, 7, 8: B=2, horizon 40 (N=39), 20 sim steps."""
+    rs = np.random.RandomState(0)
+    inertia = 1.0 + rs.rand(3) * 9.0
+    x_init = -0.1 + rs.rand(2, 3) * 0.2
+    B, N = 2, 39
+    prob = dict(horizon=N, dt=0.1, scheme=0, penalize_control_reference=False,
+                x0=x_init.copy(), x_final=np.zeros((B, 3)), x_ref=np.zeros((B, N + 1, 3)),
+                u_ref=np.zeros((B, N + 1, 3)), w_x=np.ones((B, 3)), w_u=np.ones((B, 3)), w_f=np.zeros((B, 3)),
+                dyn=np.repeat(inertia[None], B, 0))
+    params = W.solver_params(1, 1e-8, 1e-12, linesearch=True, alphas=(1.0,))
+    return inertia, x_init, prob, params
+
+
+def small_random_qp(T=3, nx=4, nu=5, seed=0, B=2):
+    """Random QP of the reference's PCG warm-start test (tests/test_solver_pcg.py:179-227), batched."""
+    rng = np.random.default_rng(seed)
+    As = np.stack([np.broadcast_to(np.diag(rng.random(nx)), (T, nx, nx)) for _ in range(B)])
+    As_next = np.stack([np.broadcast_to(np.diag(rng.random(nx)), (T, nx, nx)) for _ in range(B)])
+    Bs = np.ones((B, T, nx, nu))
+    Qinv = np.stack([np.tile(np.diag(rng.random(nx)), (T + 1, 1, 1)) for _ in range(B)])
+    Rinv = np.tile(np.eye(nu), (B, T, 1, 1))
+    Cs = np.ones((B, T + 1, nx))
+    qs = rng.normal(size=(B, T + 1, nx))
+    rs = rng.normal(size=(B, T, nu))
+    return [np.ascontiguousarray(a) for a in (As_next, As, Bs, Qinv, Rinv, Cs, qs, rs)]
+
+
+def dense_kkt(As_next, As, Bs, Qinv, qvec, Rinv, rvec, Cs):
+    """Dense KKT solve [H G^T; G 0][z; lam] = [-q; C] for one instance."""
+    N, nx, nu = As.shape[0], As.shape[1], Bs.shape[2]
+    nz = (N + 1) * (nx + nu)
+    H = np.zeros((nz, nz))
+    q = np.zeros(nz)
+    G = np.zeros(((N + 1) * nx, nz))
+    xi = lambda t: t * (nx + nu)  # noqa: E731
+    ui = lambda t: t * (nx + nu) + nx  # noqa: E731
+    for t in range(N + 1):
+        H[xi(t):xi(t) + nx, xi(t):xi(t) + nx] = np.linalg.inv(Qinv[t])
+        q[xi(t):xi(t) + nx] = qvec[t]
+        if t < N:
+            H[ui(t):ui(t) + nu, ui(t):ui(t) + nu] = np.linalg.inv(Rinv[t])
+            q[ui(t):ui(t) + nu] = rvec[t]
+        else:
+            H[ui(t):ui(t) + nu, ui(t):ui(t) + nu] = np.eye(nu)
+    G[0:nx, xi(0):xi(0) + nx] = np.eye(nx)
+    for t in range(N):
+        r = (t + 1) * nx
+        G[r:r + nx, xi(t):xi(t) + nx] = As[t]
+        G[r:r + nx, ui(t):ui(t) + nu] = Bs[t]
+        G[r:r + nx, xi(t + 1):xi(t + 1) + nx] = As_next[t]
+    K = np.block([[H, G.T], [G, np.zeros((G.shape[0],) * 2)]])
+    s = np.linalg.solve(K, np.concatenate([-q, Cs.reshape(-1)]))
+    z = s[:nz].reshape(N + 1, nx + nu)
+    return z[:, :nx], z[:N, nx:], s[nz:].reshape(N + 1, nx), H, G

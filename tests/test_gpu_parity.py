@@ -1,0 +1,230 @@
+"""GPU parity: the HIP path through the C ABI against the float64 oracle.
+
+Tolerances (relative to the largest entry of the instance's reference array):
+trajectories and multipliers 1e-5, gradients 1e-4, as the solver's contract
+states; PCG-layer kernels (all float64) 1e-10.  The dynamics are integrated in
+float32 on the GPU, which the oracle emulation puts at ~2e-7 on these cases.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from diffmpc_amd import workloads as W
+from oracle import dynamics as odyn
+from oracle import qp as oqp
+from oracle import sqp as osqp
+from tests.helpers import notebook_spacecraft, oracle_problem, small_random_qp
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "notebook_golden.json")))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    axes = tuple(range(1, a.ndim))
+    return float(np.max(np.max(np.abs(a - b), axis=axes) / np.maximum(np.max(np.abs(b), axis=axes), 1e-30)))
+
+
+CASES = {
+    "lti": lambda: W.lti(batch=4, seed=0),
+    "quadrotor": lambda: W.quadrotor(batch=8, seed=1),
+    "spacecraft7": lambda: W.spacecraft7(batch=4, seed=2),
+    "spacecraft3": lambda: W.spacecraft3(batch=3, seed=5),
+    "cartpole": lambda: W.cartpole(batch=3, seed=6),
+}
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+@pytest.mark.parametrize("case", list(CASES))
+def test_sqp_solve_matches_oracle(cuda, case, mode):
+    say(f"[sqp] case={case} mode={mode}")
+    name, prob, sp = CASES[case]()
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda, mode=mode)
+    sol = solver.solve(guess, params)
+    torch.cuda.synchronize()
+    ref = osqp.sqp_solve(oracle_problem(name, prob), sp)
+    say("  rel states", rel(_np(sol.states), ref["states"]), "controls", rel(_np(sol.controls), ref["controls"]),
+        "dual", rel(_np(sol.dual), ref["dual"]))
+    assert rel(_np(sol.states), ref["states"]) <= 1e-5
+    assert rel(_np(sol.controls), ref["controls"]) <= 1e-5
+    assert rel(_np(sol.dual), ref["dual"]) <= 1e-5
+    assert np.array_equal(_np(sol.num_iter), ref["num_iter"])
+    assert np.array_equal(_np(sol.status), ref["status"])
+    it_gpu, it_ref = _np(sol.num_pcg_iters), ref["num_pcg_iters"]
+    say("  pcg iters gpu", it_gpu[:2].tolist(), "oracle", it_ref[:2].tolist())
+    assert np.max(np.abs(it_gpu - it_ref)) <= 1
+    assert np.allclose(_np(sol.convergence_errors), ref["convergence_errors"], rtol=1e-4, atol=1e-9)
+
+
+@pytest.mark.parametrize("case", ["quadrotor", "spacecraft7", "lti"])
+def test_fused_and_plain_agree(cuda, case):
+    say(f"[fused-vs-plain] case={case}")
+    name, prob, sp = CASES[case]()
+    out = {}
+    for mode in ("fused", "plain"):
+        solver, params, guess = W.make_solver(name, prob, sp, device=cuda, mode=mode)
+        out[mode] = solver.solve(guess, params)
+    torch.cuda.synchronize()
+    assert rel(_np(out["fused"].states), _np(out["plain"].states)) <= 1e-9
+    assert rel(_np(out["fused"].dual), _np(out["plain"].dual)) <= 1e-8
+    assert torch.equal(out["fused"].num_pcg_iters, out["plain"].num_pcg_iters)
+    for f, p in zip(out["fused"].qp_parameters, out["plain"].qp_parameters):
+        assert rel(_np(f), _np(p)) <= 1e-6
+
+
+KEYS = {"weights_penalization_reference_state_trajectory": "w_x", "weights_penalization_control_squared": "w_u",
+        "weights_penalization_final_state": "w_f", "initial_state": "x0", "reference_state_trajectory": "x_ref",
+        "reference_control_trajectory": "u_ref", "final_state": "x_final"}
+
+
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+@pytest.mark.parametrize("case", ["lti", "quadrotor", "spacecraft7", "cartpole"])
+def test_backward_matches_oracle(cuda, case, mode):
+    say(f"[bwd] case={case} mode={mode}")
+    name, prob, sp = CASES[case]()
+    if case == "quadrotor":
+        prob["w_f"] = np.full_like(prob["w_f"], 0.5)
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda, mode=mode)
+    weights = {k: params[k].clone().requires_grad_(True) for k in KEYS}
+    sol = solver.solve(guess, params, weights)
+    rng = np.random.default_rng(0)
+    cx = rng.normal(size=tuple(sol.states.shape))
+    cu = rng.normal(size=tuple(sol.controls.shape))
+    loss = (sol.states * torch.tensor(cx, device=cuda)).sum() + (sol.controls * torch.tensor(cu, device=cuda)).sum()
+    grads = torch.autograd.grad(loss, [weights[k] for k in KEYS])
+    op = oracle_problem(name, prob)
+    ref = osqp.sqp_solve(op, sp)
+    g = osqp.sqp_backward(op, sp, ref, cx, cu)
+    for (k, short), gr in zip(KEYS.items(), grads):
+        e = rel(_np(gr), g[short]) if np.max(np.abs(g[short])) > 0 else float(np.max(np.abs(_np(gr))))
+        say(f"  {short}: rel {e:.2e}")
+        assert e <= 1e-4, short
+
+
+def test_shared_weights_get_batch_summed_gradient(cuda):
+    say("[bwd-shared] quadrotor")
+    name, prob, sp = CASES["quadrotor"]()
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    wx = params["weights_penalization_reference_state_trajectory"][0].clone().requires_grad_(True)
+    sol = solver.solve(guess, params, {"weights_penalization_reference_state_trajectory": wx})
+    (g,) = torch.autograd.grad(sol.states.square().sum(), [wx])
+    op = oracle_problem(name, prob)
+    ref = osqp.sqp_solve(op, sp)
+    go = osqp.sqp_backward(op, sp, ref, 2 * ref["states"], np.zeros_like(ref["controls"]))
+    assert rel(_np(g)[None], go["w_x"].sum(0)[None]) <= 1e-4
+
+
+def test_pcg_layer_matches_oracle(cuda):
+    say("[pcg-layer] random QP")
+    from diffmpc_amd.solvers.pcg import PCGOptimalControl, QPCostParameters, QPDynamicsParameters, QPParameters
+
+    As_next, As, Bs, Qinv, Rinv, Cs, qs, rs = small_random_qp(6, 4, 3, seed=11, B=3)
+    As = As + 0.05 * np.random.default_rng(2).normal(size=As.shape)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=cuda)  # noqa: E731
+    qpp = QPParameters(QPDynamicsParameters(t(As_next), t(As), t(Bs), t(Cs)), QPCostParameters(t(Qinv), t(qs), t(Rinv), t(rs)))
+    pcg = PCGOptimalControl(6, 4, 3, {"max_iter": 100, "tol_epsilon": 1e-14})
+    sc = pcg.compute_S_Phiinv(qpp)
+    gam = pcg.compute_gamma(qpp)
+    sol, dbg = pcg.solve_KKT_Schur(qpp, sc, gam, torch.zeros_like(gam))
+    S, Phi = oqp.compute_S_Phiinv(As_next, As, Bs, Qinv, Rinv)
+    AQ, AnQ, BR = oqp.premultiplied(As_next, As, Bs, Qinv, Rinv)
+    go = oqp.compute_gamma(AQ, AnQ, BR, Cs, Qinv[:, 0], qs, rs)
+    lam, it, eta = oqp.solve_kkt_schur(S, Phi, go, np.zeros_like(go), 1e-14, 100)
+    x, u = oqp.recover(lam, As_next, As, Bs, Qinv, qs, Rinv, rs)
+    assert rel(_np(sc.S), S) <= 1e-12 and rel(_np(sc.preconditioner_Phiinv), Phi) <= 1e-10
+    assert rel(_np(gam), go) <= 1e-12
+    assert np.array_equal(_np(dbg.num_iterations), it)
+    assert rel(_np(sol.kkt_multipliers), lam) <= 1e-9
+    assert rel(_np(sol.states), x) <= 1e-9 and rel(_np(sol.controls), u) <= 1e-9
+
+
+@pytest.mark.parametrize("case", ["quadrotor", "cartpole", "spacecraft7"])
+def test_linearization_matches_complex_step(cuda, case):
+    say(f"[linearize] {case}")
+    name, prob, sp = CASES[case]()
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    rng = np.random.default_rng(1)
+    x = guess.states + 0.1 * torch.tensor(rng.normal(size=tuple(guess.states.shape)), device=cuda)
+    u = guess.controls + torch.tensor(rng.normal(size=tuple(guess.controls.shape)), device=cuda)
+    As_next, A, Bm, C = solver.program.linearize(x, u, params)
+    op = oracle_problem(name, prob)
+    _, Ao, Bo, Co = osqp.linearize(op, _np(x), _np(u))
+    say("  rel A", rel(_np(A), Ao), "B", rel(_np(Bm), Bo), "C", rel(_np(C), Co))
+    assert rel(_np(A), Ao) <= 2e-6 and rel(_np(Bm), Bo) <= 2e-6 and rel(_np(C), Co) <= 2e-6
+
+
+def test_warm_start_keeps_solution_with_fewer_pcg_iterations(cuda):
+    say("[warm-start] spacecraft7")
+    name, prob, sp = CASES["spacecraft7"]()
+    sp["num_scp_iteration_max"] = 1
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    s1 = solver.solve(guess, params)
+    # a second solve from s1 linearizes at s1; solve that QP twice: cold and warm multipliers
+    cold = solver.solve(s1._replace(dual=torch.zeros_like(s1.dual)), params)
+    warm = solver.solve(cold._replace(states=s1.states, controls=s1.controls), params)
+    assert rel(_np(warm.states), _np(cold.states)) <= 1e-6
+    assert torch.all(warm.num_pcg_iters[:, 0] < cold.num_pcg_iters[:, 0])
+
+
+def test_nan_input_gives_error_status_and_finite_outputs(cuda):
+    say("[nan] lti")
+    name, prob, sp = CASES["lti"]()
+    prob["x0"][1, 0] = np.nan
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    sol = solver.solve(guess, params)
+    assert int(sol.status[1]) == -100
+    assert torch.isfinite(sol.states).all() and torch.isfinite(sol.dual).all()
+
+
+def test_notebook_spacecraft_rollout_and_gradient(cuda):
+    """notebooks/RL_spacecraft.ipynb cells 12/14 on the GPU path, two training steps."""
+    say("[notebook] spacecraft rollout")
+    from diffmpc_amd import OptimalControlProblem, SpacecraftDynamics, SQPSolver
+
+    inertia, x_init, prob, sp = notebook_spacecraft()
+    solver, params, _ = W.make_solver("spacecraft3", prob, sp, device=cuda)
+    params = {k: (v[0] if isinstance(v, torch.Tensor) and k not in ("initial_state",) else v) for k, v in params.items()}
+    params["inertia_vector"] = torch.tensor(inertia, dtype=torch.float64, device=cuda)
+    dyn = SpacecraftDynamics()
+    wx = torch.ones(3, dtype=torch.float64, device=cuda)
+    wu = torch.ones(3, dtype=torch.float64, device=cuda)
+    ref = GOLDEN["spacecraft"]
+    for step in range(2):
+        wx_ = wx.clone().requires_grad_(True)
+        wu_ = wu.clone().requires_grad_(True)
+        weights = {"weights_penalization_reference_state_trajectory": wx_, "weights_penalization_control_squared": wu_}
+        state = torch.tensor(x_init, dtype=torch.float64, device=cuda)
+        guess = solver.initial_guess({**params, "initial_state": state}, device=cuda)
+        total = 0.0
+        for _ in range(20):
+            p = {**params, "initial_state": state}
+            sol = solver.solve(guess, p, weights)
+            u0 = sol.controls[:, 0]
+            nxt = state + 0.1 * dyn.state_dot(state, u0, p)
+            total = total - (10 * nxt.square().sum() + u0.square().sum())
+            state = nxt
+            guess = sol
+        gx, gu = torch.autograd.grad(total, [wx_, wu_])
+        loss = float(total)
+        say(f"  step {step + 1}: loss {loss!r} reference {ref['losses'][step]!r}")
+        assert abs(loss - ref["losses"][step]) <= 2e-6 * abs(ref["losses"][step])
+        wx = torch.clamp(wx + gx, min=1e-10)
+        wu = torch.clamp(wu + gu, min=1e-10)
+        if step == 0:
+            say("  w after step 1", wx.tolist(), wu.tolist())
+            assert np.allclose(_np(wx), ref["w_x_after_step1"], atol=3e-5, rtol=0)
+            assert np.allclose(_np(wu), ref["w_u_after_step1"], atol=3e-5, rtol=0)

@@ -1,0 +1,111 @@
+"""Host-side API: parameter files, problem class, guesses, parameter layouts."""
+import numpy as np
+import pytest
+import torch
+
+from diffmpc_amd import (CartpoleDynamics, ChainDynamics, LinearDynamics, OptimalControlProblem, QuadrotorDynamics,
+                         SpacecraftAttitudeDynamics, SpacecraftDynamics, SQPSolver, load_problem_params,
+                         load_solver_params)
+from diffmpc_amd.dynamics.quadrotor_dynamics import S, get_rotation, q_conj, q_left
+from oracle import dynamics as odyn
+from oracle import sqp as osqp
+
+
+def test_load_params_repeats_references():
+    p = load_problem_params("spacecraft.yaml")
+    assert p["horizon"] == 25
+    assert p["reference_state_trajectory"].shape == (26, 3)
+    assert p["reference_control_trajectory"].shape == (26, 3)
+    assert p["initial_state"].dtype == torch.float64
+    s = load_solver_params("sqp.yaml")
+    assert s["num_scp_iteration_max"] == 15 and s["pcg"]["max_iter"] == 10000
+    assert s["linesearch_alphas"] == [0.01, 0.1, 0.3, 0.7, 1.0]
+
+
+def test_missing_required_key_raises():
+    p = load_problem_params("spacecraft.yaml")
+    del p["final_state"]
+    with pytest.raises(KeyError):
+        OptimalControlProblem(SpacecraftDynamics(), p)
+
+
+def test_solver_rejects_other_programs_and_unknown_weight_keys():
+    with pytest.raises(NotImplementedError):
+        SQPSolver(program=object(), params=load_solver_params("sqp.yaml"))
+    p = load_problem_params("spacecraft.yaml")
+    solver = SQPSolver(OptimalControlProblem(SpacecraftDynamics(), p), params=load_solver_params("sqp.yaml"))
+    with pytest.raises(NotImplementedError):
+        solver.solve(solver.initial_guess(), p, {"inertia_vector": p["inertia_vector"]})
+
+
+def test_initial_guess_matches_oracle():
+    p = load_problem_params("quadrotor.yaml")
+    prob = OptimalControlProblem(QuadrotorDynamics(), p)
+    B = 3
+    x0 = torch.randn(B, 13, dtype=torch.float64)
+    xs, us = prob.initial_guess({**p, "initial_state": x0})
+    op = dict(horizon=prob.horizon, x0=x0.numpy(), x_final=np.repeat(p["final_state"].numpy()[None], B, 0),
+              model=odyn.make_model("quadrotor"))
+    ox, ou, _ = osqp.initial_guess(op)
+    assert np.allclose(xs.numpy(), ox, atol=1e-15) and np.allclose(us.numpy(), ou)
+    x1, u1 = prob.initial_guess()
+    assert x1.shape == (26, 13) and u1.shape == (26, 4)
+
+
+def test_flat_parameter_layouts_match_the_oracle_models():
+    rng = np.random.default_rng(0)
+    cases = [
+        (QuadrotorDynamics(), {"mass": 0.7, "inertia": np.diag([0.01, 0.02, 0.03])}, "quadrotor"),
+        (SpacecraftDynamics(), {"inertia_vector": [5.0, 2.0, 1.0]}, "spacecraft3"),
+        (SpacecraftAttitudeDynamics(), {"inertia_vector": [0.5, 1.0, 1.5]}, "spacecraft7"),
+        (CartpoleDynamics(), {"masscart": 1.0, "masspole": 0.1, "length": 0.5, "gravity": 9.81}, "cartpole"),
+        (ChainDynamics(), {"spring_constants": np.linspace(0.5, 2, 12), "damping": np.linspace(0.05, 0.2, 12)},
+         "chain"),
+    ]
+    for dyn, params, name in cases:
+        flat = dyn.flat_params(params, 2).numpy().astype(np.float64)
+        assert flat.shape == (2, dyn.num_params)
+        m = odyn.make_model(name)
+        x = rng.normal(size=(2, m.nx))
+        u = rng.normal(size=(2, m.nu))
+        if name in ("quadrotor",):
+            x[:, 6:10] += [1, 0, 0, 0]
+        if name == "spacecraft7":
+            x[:, 0:4] += [1, 0, 0, 0]
+        host = dyn.state_dot(torch.tensor(x), torch.tensor(u), params).numpy()
+        assert np.allclose(host, m.state_dot(x, u, flat), rtol=1e-5, atol=1e-6), name
+    lin = LinearDynamics()
+    A, Bm, b = rng.normal(size=(4, 4)), rng.normal(size=(4, 2)), rng.normal(size=4)
+    flat = lin.flat_params({"A": A, "B": Bm, "b": b}, 1).numpy()[0]
+    assert np.allclose(flat, np.concatenate([A.ravel(), Bm.ravel(), b]), atol=1e-6)
+
+
+def test_quadrotor_helpers():
+    a, b = torch.randn(3, dtype=torch.float64), torch.randn(3, dtype=torch.float64)
+    assert torch.allclose(S(a) @ b, torch.linalg.cross(a, b))
+    q = torch.randn(4, dtype=torch.float64)
+    q = q / q.norm()
+    R = get_rotation(q)
+    assert torch.allclose(R.T @ R, torch.eye(3, dtype=torch.float64), atol=1e-12)
+    u = torch.randn(3, dtype=torch.float64)
+    w = q_left(q_left(q) @ torch.cat([torch.zeros(1, dtype=torch.float64), u])) @ q_conj(q)
+    assert torch.allclose(R @ u, w[1:], atol=1e-12)
+
+
+def test_cost_and_constraints_match_oracle():
+    p = load_problem_params("spacecraft.yaml")
+    prob = OptimalControlProblem(SpacecraftDynamics(), p)
+    N = prob.horizon
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(N + 1, 3))
+    u = rng.normal(size=(N + 1, 3))
+    c = prob.cost(torch.tensor(x), torch.tensor(u), p).item()
+    g = prob.equality_constraints(torch.tensor(x), torch.tensor(u), p).numpy()
+    op = dict(model=odyn.make_model("spacecraft3"), horizon=N, dt=0.1, scheme=0, penalize_control_reference=False,
+              x0=p["initial_state"].numpy()[None], x_final=p["final_state"].numpy()[None],
+              x_ref=p["reference_state_trajectory"].numpy()[None], u_ref=p["reference_control_trajectory"].numpy()[None],
+              w_x=p["weights_penalization_reference_state_trajectory"].numpy()[None],
+              w_u=p["weights_penalization_control_squared"].numpy()[None],
+              w_f=p["weights_penalization_final_state"].numpy()[None], dyn=p["inertia_vector"].numpy()[None])
+    assert np.isclose(c, osqp.cost(op, x[None], u[None])[0], rtol=1e-13)
+    assert np.allclose(g, osqp.equality_constraints(op, x[None], u[None])[0], atol=1e-13)
