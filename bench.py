@@ -60,6 +60,10 @@ def main():
         return solver, params, guess
 
     def fwd_bwd(solver, params, guess):
+        # cold adjoint start every step: the backward writes its multipliers
+        # into the guess (the reference's warm-start channel), which would
+        # otherwise hand the next step a converged adjoint
+        guess.dual_backward_guess.zero_()
         wx = params["weights_penalization_reference_state_trajectory"][0].clone().requires_grad_(True)
         wu = params["weights_penalization_control_squared"][0].clone().requires_grad_(True)
         x0 = params["initial_state"].clone().requires_grad_(True)

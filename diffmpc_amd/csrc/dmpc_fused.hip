@@ -89,7 +89,7 @@ struct LdsPlan {
            NB * NU /*rv*/ + NU /*rinv*/ + NX /*x0*/ + 3 * 16 /*red*/ + 5 * NBP * NX /*P RT Y YQ YX*/ +
            NBP * NU /*YU*/ + NB * NX * NX /*LF*/ + NB * NX /*ID*/;
   }
-  static __host__ __device__ size_t floats(int N, int P) { return (size_t)N * NX + P; }
+  static __host__ __device__ size_t floats(int N, int P) { return (size_t)N * NX + P + (size_t)(N + 1) * NU * NX; }
   static __host__ size_t bytes(int N, int P) { return doubles(N) * 8 + floats(N, P) * 4 + 16; }
 };
 
@@ -98,6 +98,7 @@ struct Lds {
   double *P, *RT, *Y, *YQ, *YX, *YU;
   double *LF, *ID;  // Cholesky factor rows of theta_t and 1 / L_kk
   float *fd, *prm;
+  float* BT;        // BT[t][k][j] = B_t[j][k], k < nu (row N unused)
 };
 
 template <int NX, int NU>
@@ -126,8 +127,28 @@ __device__ Lds carve(void* base, int N) {
   L.ID = d; d += NB * NX;
   float* f = reinterpret_cast<float*>(d);
   L.fd = f; f += N * NX;
+  L.BT = f; f += (N + 1) * NU * NX;
   L.prm = f;
   return L;
+}
+
+// Makes a register value opaque to the optimizer (no instruction is emitted),
+// so float -> double conversions of the A/B registers stay inside the PCG loop
+// instead of being hoisted out of it as double copies, which would not fit.
+__device__ __forceinline__ void opaque(float& v) { asm volatile("" : "+v"(v)); }
+
+// Dot products of a register row with an LDS vector.  The scheduler barrier
+// every 4 terms keeps it from hoisting all loads of the row at once, which
+// would need more registers than the 13-wave workgroup can give a thread.
+template <int N>
+__device__ __forceinline__ double dot_rl(const float* a, const double* v) {
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    acc += (double)a[j] * v[j];
+    if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+  }
+  return acc;
 }
 
 // Per-thread register state of the K blocks it owns.
@@ -136,8 +157,6 @@ struct Regs {
   float acol[K][NX];  // column `lane` of A_t          (zero for t = N)
   float arow[K][NX];  // row `lane` of A_{t-1}         (zero for t = 0)
   float brow[K][NU];  // row `lane` of B_{t-1}         (zero for t = 0)
-  float bcol[K][NX];  // column `lane` of B_t, lane < nu (zero otherwise)
-  double drow[K][NX]; // row `lane` of D_t = theta_t^-1
   double qi[K];       // Qinv_t[lane, lane]
   int t[K];
   bool ok[K];         // t <= N and lane < nx
@@ -155,8 +174,10 @@ __device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, co
     const float* Ap = ABb + (size_t)(t - 1) * NX * NZ;
 #pragma unroll
     for (int j = 0; j < NX; ++j) R.acol[s][j] = (ok && t < N) ? At[j * NZ + lane] : 0.0f;
+    if (ok && t < N && lane < NU) {
 #pragma unroll
-    for (int j = 0; j < NX; ++j) R.bcol[s][j] = (ok && t < N && lane < NU) ? At[j * NZ + NX + lane] : 0.0f;
+      for (int j = 0; j < NX; ++j) L.BT[(t * NU + lane) * NX + j] = At[j * NZ + NX + lane];
+    }
 #pragma unroll
     for (int k = 0; k < NX; ++k) R.arow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + k] : 0.0f;
 #pragma unroll
@@ -173,26 +194,33 @@ __device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, co
 // with cross-lane shuffles, parked in LDS, and each lane then solves
 // L L^T d = e_lane for its row of D_t.
 template <int NX, int NU, int K>
-__device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const Lds& L, int lane, int gbase) {
+__device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* ABb, const Lds& L, int lane,
+                                              int gbase) {
+  constexpr int NZ = NX + NU;
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     const int t = R.t[s];
     const bool ok = R.ok[s];
+    const bool has_prev = ok && t >= 1;
     const int tp = t >= 1 ? t - 1 : 0;
+    const float* Ap = ABb + (size_t)tp * NX * NZ;  // [A_{t-1} | B_{t-1}], L1-cached broadcast reads
     double row[NX];
     {
       double aq[NX], bq[NU];
 #pragma unroll
-      for (int k = 0; k < NX; ++k) aq[k] = (double)R.arow[s][k] * (ok ? L.qinv[tp * NX + k] : 0.0);
+      for (int k = 0; k < NX; ++k) aq[k] = has_prev ? (double)Ap[lane * NZ + k] * L.qinv[tp * NX + k] : 0.0;
 #pragma unroll
-      for (int k = 0; k < NU; ++k) bq[k] = (double)R.brow[s][k] * L.rinv[k];
+      for (int k = 0; k < NU; ++k) bq[k] = has_prev ? (double)Ap[lane * NZ + NX + k] * L.rinv[k] : 0.0;
+      const double qi = ok ? L.qinv[t * NX + lane] : 0.0;
 #pragma unroll
       for (int j = 0; j < NX; ++j) {
-        double acc = (j == lane) ? R.qi[s] : 0.0;
+        double acc = (j == lane) ? qi : 0.0;
+        if (has_prev) {
 #pragma unroll
-        for (int k = 0; k < NX; ++k) acc += aq[k] * (double)__shfl(R.arow[s][k], gbase + j, 64);
+          for (int k = 0; k < NX; ++k) acc += aq[k] * (double)Ap[j * NZ + k];
 #pragma unroll
-        for (int k = 0; k < NU; ++k) acc += bq[k] * (double)__shfl(R.brow[s][k], gbase + j, 64);
+          for (int k = 0; k < NU; ++k) acc += bq[k] * (double)Ap[j * NZ + NX + k];
+        }
         row[j] = ok ? acc : ((j == lane) ? 1.0 : 0.0);
       }
     }
@@ -223,7 +251,7 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const Lds& L, 
     const double* LF = L.LF + (size_t)t * NX * NX;
     const double* ID = L.ID + t * NX;
     // column `lane` of theta^-1 = L^-T L^-1 e_lane
-    double w[NX];
+    double w[NX], d[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
       double acc = (j == lane) ? 1.0 : 0.0;
@@ -235,20 +263,29 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const Lds& L, 
     for (int j = NX - 1; j >= 0; --j) {
       double acc = w[j];
 #pragma unroll
-      for (int k = j + 1; k < NX; ++k) acc -= LF[k * NX + j] * R.drow[s][k];
-      R.drow[s][j] = acc * ID[j];
+      for (int k = j + 1; k < NX; ++k) acc -= LF[k * NX + j] * d[k];
+      d[j] = acc * ID[j];
     }
+    // D_t replaces the factor of its own block: all lanes of the group are in
+    // one wavefront and have finished reading the factor above.
+    if (R.ok[s]) {
 #pragma unroll
-    for (int j = 0; j < NX; ++j) R.drow[s][j] = R.ok[s] ? R.drow[s][j] : 0.0;
+      for (int j = 0; j < NX; ++j) L.LF[(t * NX + lane) * NX + j] = d[j];
+    }
   }
+  __syncthreads();
 }
 
 // y = D_t v (v distributed over the lanes of the block group).
 template <int NX, int NU, int K>
-__device__ __forceinline__ double dmul(const Regs<NX, NU, K>& R, int s, double v, int gbase) {
+__device__ __forceinline__ double dmul(const Regs<NX, NU, K>& R, const Lds& L, int s, double v, int lane, int gbase) {
+  const double* D = L.LF + ((size_t)R.t[s] * NX + lane) * NX;  // row `lane` of D_t
   double acc = 0.0;
 #pragma unroll
-  for (int j = 0; j < NX; ++j) acc += R.drow[s][j] * __shfl(v, gbase + j, 64);
+  for (int j = 0; j < NX; ++j) {
+    acc += (lane < NX ? D[j] : 0.0) * __shfl(v, gbase + j, 64);
+    if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+  }
   return acc;
 }
 
@@ -257,9 +294,14 @@ __device__ __forceinline__ double dmul(const Regs<NX, NU, K>& R, int s, double v
 // |eta| > eps and it <= max_iter - 1.  lam enters as the guess in L.lamS
 // (padded) and leaves in registers.  Returns the iteration count.
 template <int NX, int NU, int K>
-__device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K], const Lds& L, int N,
+__device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K], const Lds& L, int N,
                    int lane, int gbase, double eps, int max_iter, double* eta_out) {
-  double r[K], p[K], rt[K];
+  double r[K], p[K], rt[K], ap[K], bp[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) {
+    ap[s] = 0.0;
+    bp[s] = 0.0;
+  }
   // ---- r = gamma - S lam0 -------------------------------------------------
 #pragma unroll
   for (int s = 0; s < K; ++s) {
@@ -274,7 +316,7 @@ __device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&la
       for (int j = 0; j < NX; ++j) {
         const double pn = L.lamS[(t + 2) * NX + j];
         wx += (double)R.acol[s][j] * pn;
-        wu += (double)R.bcol[s][j] * pn;
+        if (lane < NU && t < N) wu += (double)L.BT[(t * NU + lane) * NX + j] * pn;
       }
       L.YX[(t + 1) * NX + lane] = R.qi[s] * wx;
       if (lane < NU && t < N) L.YU[(t + 1) * NU + lane] = L.rinv[lane] * wu;
@@ -287,14 +329,13 @@ __device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&la
     double ups = 0.0;
     if (R.ok[s]) {
       double g = (t == 0 ? 1.0 : -1.0) * L.YX[(t + 1) * NX + lane];
-#pragma unroll
-      for (int j = 0; j < NX; ++j) g += (double)R.arow[s][j] * L.YX[t * NX + j];
+      g += dot_rl<NX>(R.arow[s], L.YX + t * NX);
 #pragma unroll
       for (int k = 0; k < NU; ++k) g += (double)R.brow[s][k] * L.YU[t * NU + k];
       ups = -g;
     }
     r[s] = R.ok[s] ? gam[s] - ups : 0.0;
-    const double y = dmul(R, s, r[s], gbase);
+    const double y = dmul(R, L, s, r[s], lane, gbase);
     if (R.ok[s]) {
       L.Y[(t + 1) * NX + lane] = y;
       L.YQ[(t + 1) * NX + lane] = R.qi[s] * y;
@@ -308,14 +349,11 @@ __device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&la
     double z = 0.0;
     if (R.ok[s]) {
       double a1 = 0.0, a2 = 0.0;
-#pragma unroll
-      for (int j = 0; j < NX; ++j) {
-        a1 += (double)R.arow[s][j] * L.YQ[t * NX + j];
-        a2 += (double)R.acol[s][j] * L.Y[(t + 2) * NX + j];
-      }
+      a1 = dot_rl<NX>(R.arow[s], L.YQ + t * NX);
+      a2 = dot_rl<NX>(R.acol[s], L.Y + (t + 2) * NX);
       z = r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2;
     }
-    rt[s] = dmul(R, s, z, gbase);
+    rt[s] = dmul(R, L, s, z, lane, gbase);
     if (R.ok[s]) {
       part += r[s] * rt[s];
       L.RT[(t + 1) * NX + lane] = rt[s];
@@ -328,7 +366,19 @@ __device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&la
   while (true) {
     const bool cont = (fabs(eta) > eps && it <= max_iter - 1) || it < 1;
     if (!cont) break;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        opaque(R.acol[s][j]);
+        opaque(R.arow[s][j]);
+      }
+#pragma unroll
+      for (int k = 0; k < NU; ++k) opaque(R.brow[s][k]);
+    }
     // ---- phase A: p = rt + beta p; w = G^T p; y = H^-1 w; v = -w.y --------
+    // A_t^T p_{t+1} = A_t^T rt_{t+1} + beta A_t^T p_{t+1}(previous), so only
+    // the neighbour's rt is read; the previous product is kept in registers.
     double vpart = 0.0;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
@@ -336,21 +386,18 @@ __device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&la
       if (R.ok[s]) {
         const double pv = rt[s] + beta * p[s];
         p[s] = pv;
-        double wx = (t == 0) ? pv : -pv, wu = 0.0;
-#pragma unroll
-        for (int j = 0; j < NX; ++j) {
-          const int o = (t + 2) * NX + j;
-          const double pn = L.RT[o] + beta * L.P[o];
-          wx += (double)R.acol[s][j] * pn;
-          wu += (double)R.bcol[s][j] * pn;
-        }
+        const double art = dot_rl<NX>(R.acol[s], L.RT + (t + 2) * NX);
+        const double brt = (lane < NU && t < N) ? dot_rl<NX>(L.BT + (t * NU + lane) * NX, L.RT + (t + 2) * NX) : 0.0;
+        ap[s] = art + beta * ap[s];
+        bp[s] = brt + beta * bp[s];
+        const double wx = ((t == 0) ? pv : -pv) + ap[s];
         const double yx = R.qi[s] * wx;
         L.YX[(t + 1) * NX + lane] = yx;
         vpart += wx * yx;
         if (lane < NU && t < N) {
-          const double yu = L.rinv[lane] * wu;
+          const double yu = L.rinv[lane] * bp[s];
           L.YU[(t + 1) * NU + lane] = yu;
-          vpart += wu * yu;
+          vpart += bp[s] * yu;
         }
       }
     }
@@ -369,7 +416,7 @@ __device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&la
         lam[s] += alpha * p[s];
         r[s] += alpha * g;  // r -= alpha * (-g)
       }
-      const double y = dmul(R, s, r[s], gbase);
+      const double y = dmul(R, L, s, r[s], lane, gbase);
       if (R.ok[s]) {
         L.Y[(t + 1) * NX + lane] = y;
         L.YQ[(t + 1) * NX + lane] = R.qi[s] * y;
@@ -391,7 +438,7 @@ __device__ int pcg(const Regs<NX, NU, K>& R, const double (&gam)[K], double (&la
         }
         z = r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2;
       }
-      rt[s] = dmul(R, s, z, gbase);
+      rt[s] = dmul(R, L, s, z, lane, gbase);
       if (R.ok[s]) {
         epart += r[s] * rt[s];
         L.RT[(t + 1) * NX + lane] = rt[s];
@@ -456,7 +503,7 @@ __device__ __forceinline__ void recover_rows(const Regs<NX, NU, K>& R, const Lds
       for (int j = 0; j < NX; ++j) {
         const double ln = L.lamS[(t + 2) * NX + j];
         ax += (double)R.acol[s][j] * ln;
-        bx += (double)R.bcol[s][j] * ln;
+        if (lane < NU && t < N) bx += (double)L.BT[(t * NU + lane) * NX + j] * ln;
       }
       xn[s] = -R.qi[s] * (L.qv[t * NX + lane] + (t == 0 ? lt : -lt) + ax);
       if (lane < NU && t < N) un[s] = -L.rinv[lane] * (L.rv[t * NU + lane] + bx);
@@ -591,8 +638,8 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   __syncthreads();
 
   const float* ABb = a.AB + (size_t)b * N * NX * NZ;
+  assemble_dinv(R, ABb, L, lane, gbase);
   load_ab(R, ABb, L, N, lane);
-  assemble_dinv(R, L, lane, gbase);
   double gam[K], lam[K];
   gamma_rows<true>(R, L, lane, gam);
   double eta;
@@ -688,8 +735,8 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   for (int k = threadIdx.x; k < NU; k += blockDim.x) L.rinv[k] = a.rinv[(size_t)b * NU + k];
   __syncthreads();
   const float* ABb = a.AB + (size_t)b * N * NX * NZ;
+  assemble_dinv(R, ABb, L, lane, gbase);
   load_ab(R, ABb, L, N, lane);
-  assemble_dinv(R, L, lane, gbase);
   double gam[K], lam[K];
   gamma_rows<false>(R, L, lane, gam);
   double eta;

@@ -491,7 +491,8 @@ __global__ void finish_kernel(int B, int N, int nx, int nu, double tol, double* 
 
 // saved linearization for the backward: AB (float), diag Qinv, diag Rinv
 __global__ void save_kernel(int B, int N, int nx, int nu, const double* As, const double* Bs, const double* Qinv,
-                            const double* Rinv, const int* active, float* AB, double* qinv, double* rinv) {
+                            const double* Rinv, const double* fd, const int* active, float* AB, float* fdo,
+                            double* qinv, double* rinv) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int nz = nx + nu;
   const long long nab = (long long)B * N * nx * nz;
@@ -504,6 +505,9 @@ __global__ void save_kernel(int B, int N, int nx, int nu, const double* As, cons
     const long long blk = idx / nx;
     const int i = idx % nx;
     if (active[blk / (N + 1)]) qinv[idx] = Qinv[blk * nx * nx + i * nx + i];
+  }
+  if (idx < (long long)B * N * nx) {
+    if (active[idx / ((long long)N * nx)]) fdo[idx] = (float)fd[idx];
   }
   if (idx < (long long)B * nu) {
     const int b = idx / nu, i = idx % nu;
@@ -779,7 +783,7 @@ int plain_forward_t(const SolveArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(recover_kernel, dim3(nblk(nst + (long long)B * (N + 1) * nu)), dim3(256), 0, st, B, N, nx, nu,
                        q.lam, q.As_next, q.As, q.Bs, q.Qinv, q.qv, q.Rinv, q.rv, q.xn, q.un);
     hipLaunchKernelGGL(save_kernel, dim3(nblk((long long)B * N * nx * (nx + nu) + nst)), dim3(256), 0, st, B, N, nx,
-                       nu, q.As, q.Bs, q.Qinv, q.Rinv, q.sqp_active, a.AB, a.qinv, a.rinv);
+                       nu, q.As, q.Bs, q.Qinv, q.Rinv, q.fd, q.sqp_active, a.AB, a.fd, a.qinv, a.rinv);
     hipLaunchKernelGGL(sqp_accept_kernel, dim3(B), dim3(256), 0, st, B, N, nx, nu, a.maxit, sqp_it, q.sqp_active, a.x,
                        a.u, a.lam, q.xn, q.un, q.lam, a.conv_err, a.conv_errs, a.pcg_iters, q.iters, a.ls_alphas,
                        a.linesearch, a.num_iter);
