@@ -32,10 +32,35 @@ int g_forced_K = 0;
 
 namespace {
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// Sum over the G lanes of a block group, result in every lane of the group
+// (DPP butterflies: quad swaps, then half-row and row mirrors).
+template <int ctrl>
+__device__ __forceinline__ double dpp_mov(double v) {
+  long long x = __double_as_longlong(v);
+  x = __builtin_amdgcn_mov_dpp(x, ctrl, 0xf, 0xf, true);
+  return __longlong_as_double(x);
+}
+template <int G>
+__device__ __forceinline__ double gsum(double v) {
+  v += dpp_mov<0xB1>(v);                        // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);                        // quad_perm [2,3,0,1]
+  if constexpr (G >= 8) v += dpp_mov<0x141>(v);  // row_half_mirror
+  if constexpr (G >= 16) v += dpp_mov<0x140>(v); // row_mirror
   return v;
+}
+
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const long long x = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over a full wave: DPP sums inside each 16-lane row, then the four row
+// sums are read as scalars.  The result is wave-uniform.
+__device__ __forceinline__ double wave_sum(double v) {
+  v = gsum<16>(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
 __device__ __forceinline__ double nanmax(double a, double b) {
@@ -86,19 +111,18 @@ struct LdsPlan {
   static __host__ __device__ size_t doubles(int N) {
     const size_t NB = N + 1, NBP = N + 3;
     return NB * NX /*xb*/ + NB * NU /*ub*/ + NBP * NX /*lamS*/ + NB * NX /*qinv*/ + NB * NX /*qv*/ +
-           NB * NU /*rv*/ + NU /*rinv*/ + NX /*x0*/ + 3 * 16 /*red*/ + 5 * NBP * NX /*P RT Y YQ YX*/ +
-           NBP * NU /*YU*/ + NB * NX * NX /*LF*/ + NB * NX /*ID*/;
+           NB * NU /*rv*/ + NU /*rinv*/ + NX /*x0*/ + 3 * 16 /*red*/ + 4 * NBP * NX /*RT Y YQ YX*/ +
+           NB * NX * NX /*LF*/ + NB * NX /*ID*/;
   }
-  static __host__ __device__ size_t floats(int N, int P) { return (size_t)N * NX + P + (size_t)(N + 1) * NU * NX; }
+  static __host__ __device__ size_t floats(int N, int P) { return (size_t)N * NX + P; }
   static __host__ size_t bytes(int N, int P) { return doubles(N) * 8 + floats(N, P) * 4 + 16; }
 };
 
 struct Lds {
   double *xb, *ub, *lamS, *qinv, *qv, *rv, *rinv, *x0, *redA, *redB, *redC;
-  double *P, *RT, *Y, *YQ, *YX, *YU;
+  double *RT, *Y, *YQ, *YX;
   double *LF, *ID;  // Cholesky factor rows of theta_t and 1 / L_kk
   float *fd, *prm;
-  float* BT;        // BT[t][k][j] = B_t[j][k], k < nu (row N unused)
 };
 
 template <int NX, int NU>
@@ -117,49 +141,118 @@ __device__ Lds carve(void* base, int N) {
   L.redA = d; d += 16;
   L.redB = d; d += 16;
   L.redC = d; d += 16;
-  L.P = d; d += NBP * NX;
   L.RT = d; d += NBP * NX;
   L.Y = d; d += NBP * NX;
   L.YQ = d; d += NBP * NX;
   L.YX = d; d += NBP * NX;
-  L.YU = d; d += NBP * NU;
   L.LF = d; d += NB * NX * NX;
   L.ID = d; d += NB * NX;
   float* f = reinterpret_cast<float*>(d);
   L.fd = f; f += N * NX;
-  L.BT = f; f += (N + 1) * NU * NX;
   L.prm = f;
   return L;
 }
 
-// Makes a register value opaque to the optimizer (no instruction is emitted),
-// so float -> double conversions of the A/B registers stay inside the PCG loop
-// instead of being hoisted out of it as double copies, which would not fit.
-__device__ __forceinline__ void opaque(float& v) { asm volatile("" : "+v"(v)); }
+// float -> double conversion that the optimizer cannot move out of the PCG
+// loop: the asm takes the (uniform) iteration counter as an operand, so it is
+// not loop-invariant, but it has no side effects and schedules freely.
+// Without it the compiler hoists the conversions of the A/B registers and
+// keeps the double copies, which do not fit in the register file.
+__device__ __forceinline__ double cvt_in_loop(float f, int salt) {
+  double d;
+  asm("v_cvt_f64_f32 %0, %1" : "=v"(d) : "v"(f), "s"(salt));
+  return d;
+}
 
-// Dot products of a register row with an LDS vector.  The scheduler barrier
-// every 4 terms keeps it from hoisting all loads of the row at once, which
-// would need more registers than the 13-wave workgroup can give a thread.
-template <int N>
-__device__ __forceinline__ double dot_rl(const float* a, const double* v) {
-  double acc = 0.0;
-#pragma unroll
-  for (int j = 0; j < N; ++j) {
-    acc += (double)a[j] * v[j];
-    if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+// ---------------------------------------------------------------------------
+// Broadcast of lane n of each G-lane block group to the whole group with DPP
+// (no LDS traffic): G = 16 is one DPP row (row_newbcast:n, one 64-bit move on
+// gfx950); G = 4 a quad (quad_perm [n,n,n,n]); G = 8 takes row_newbcast of n
+// and 8 + n and selects by the lane's half of the row.
+template <int n>
+__device__ __forceinline__ double dpp_rowbcast(double v) {
+  long long x = __double_as_longlong(v);
+  x = __builtin_amdgcn_mov_dpp(x, 0x150 + n, 0xf, 0xf, true);
+  return __longlong_as_double(x);
+}
+template <int n>
+__device__ __forceinline__ double dpp_quadbcast(double v) {
+  long long x = __double_as_longlong(v);
+  x = __builtin_amdgcn_mov_dpp(x, n | (n << 2) | (n << 4) | (n << 6), 0xf, 0xf, true);
+  return __longlong_as_double(x);
+}
+
+template <int G, int n>
+__device__ __forceinline__ double gbcast(double v, int lane) {
+  static_assert(n < G, "source lane outside the group");
+  if constexpr (G == 16) {
+    return dpp_rowbcast<n>(v);
+  } else if constexpr (G == 4) {
+    return dpp_quadbcast<n>(v);
+  } else {
+    static_assert(G == 8, "block groups of 4, 8 or 16 lanes");
+    // groups are 8-aligned in the workgroup: the upper group of a DPP row
+    // is the one with bit 3 of the thread index set (`lane` is below 8)
+    const double lo = dpp_rowbcast<n>(v);
+    const double hi = dpp_rowbcast<n + 8>(v);
+    return (threadIdx.x & 8) ? hi : lo;
   }
-  return acc;
+}
+
+// sum_j a[j] * v_j where v_j lives in lane j of the group (j < M <= NX).
+template <int G, int M, int j = 0>
+__device__ __forceinline__ double gdot(const float* a, double v, int lane, int salt, double acc = 0.0) {
+  if constexpr (j == M) {
+    return acc;
+  } else {
+    acc += cvt_in_loop(a[j], salt) * gbcast<G, j>(v, lane);
+    // keep the scheduler from converting and broadcasting a whole row ahead
+    // of its FMAs (each in-flight term holds four VGPRs)
+    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    return gdot<G, M, j + 1>(a, v, lane, salt, acc);
+  }
+}
+template <int G, int M, int j = 0>
+__device__ __forceinline__ double gdotd(const double* a, double v, int lane, double acc = 0.0) {
+  if constexpr (j == M) {
+    return acc;
+  } else {
+    acc += a[j] * gbcast<G, j>(v, lane);
+    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    return gdotd<G, M, j + 1>(a, v, lane, acc);
+  }
+}
+
+// Where D_t = theta_t^-1 lives during PCG: in registers (row `lane` per lane)
+// for small blocks, in LDS for nx > 8, where the registers are needed for the
+// A/B rows.  The LDS copy is stored transposed, DT[t][j][i] = D_t[i][j] as
+// computed by lane i, so the lanes of a group read consecutive words.
+template <int NX>
+__host__ __device__ constexpr bool d_in_lds() { return NX > 8; }
+
+// sum_j a[j * STRIDE] * v_j (a in LDS), v_j in lane j of the group.
+template <int G, int M, int STRIDE, int j = 0>
+__device__ __forceinline__ double gdot_strided(const double* a, double v, int lane, double acc = 0.0) {
+  if constexpr (j == M) {
+    return acc;
+  } else {
+    acc += a[j * STRIDE] * gbcast<G, j>(v, lane);
+    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    return gdot_strided<G, M, STRIDE, j + 1>(a, v, lane, acc);
+  }
 }
 
 // Per-thread register state of the K blocks it owns.
 template <int NX, int NU, int K>
 struct Regs {
-  float acol[K][NX];  // column `lane` of A_t          (zero for t = N)
-  float arow[K][NX];  // row `lane` of A_{t-1}         (zero for t = 0)
-  float brow[K][NU];  // row `lane` of B_{t-1}         (zero for t = 0)
-  double qi[K];       // Qinv_t[lane, lane]
+  float acol[K][NX];   // column `lane` of A_t            (zero for t = N)
+  float arow[K][NX];   // row `lane` of A_{t-1}           (zero for t = 0)
+  float brow[K][NU];   // row `lane` of B_{t-1}           (zero for t = 0)
+                       // (B_{t-1} terms of S are formed by block t alone)
+  double drow[K][d_in_lds<NX>() ? 1 : NX];  // row `lane` of D_t = theta_t^-1 (registers mode)
+  double qi[K];        // Qinv_t[lane, lane]
   int t[K];
-  bool ok[K];         // t <= N and lane < nx
+  bool ok[K];          // t <= N and lane < nx
 };
 
 // Load A/B rows and columns of the saved linearization into registers.
@@ -174,10 +267,6 @@ __device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, co
     const float* Ap = ABb + (size_t)(t - 1) * NX * NZ;
 #pragma unroll
     for (int j = 0; j < NX; ++j) R.acol[s][j] = (ok && t < N) ? At[j * NZ + lane] : 0.0f;
-    if (ok && t < N && lane < NU) {
-#pragma unroll
-      for (int j = 0; j < NX; ++j) L.BT[(t * NU + lane) * NX + j] = At[j * NZ + NX + lane];
-    }
 #pragma unroll
     for (int k = 0; k < NX; ++k) R.arow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + k] : 0.0f;
 #pragma unroll
@@ -266,76 +355,98 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* A
       for (int k = j + 1; k < NX; ++k) acc -= LF[k * NX + j] * d[k];
       d[j] = acc * ID[j];
     }
-    // D_t replaces the factor of its own block: all lanes of the group are in
-    // one wavefront and have finished reading the factor above.
-    if (R.ok[s]) {
+    if constexpr (d_in_lds<NX>()) {
+      __syncthreads();  // every lane is done reading the factors
+      if (R.ok[s]) {
 #pragma unroll
-      for (int j = 0; j < NX; ++j) L.LF[(t * NX + lane) * NX + j] = d[j];
+        for (int j = 0; j < NX; ++j) L.LF[((size_t)t * NX + j) * NX + lane] = d[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NX; ++j) R.drow[s][j] = R.ok[s] ? d[j] : 0.0;
     }
   }
-  __syncthreads();
+  if constexpr (d_in_lds<NX>()) __syncthreads();
 }
 
-// y = D_t v (v distributed over the lanes of the block group).
-template <int NX, int NU, int K>
-__device__ __forceinline__ double dmul(const Regs<NX, NU, K>& R, const Lds& L, int s, double v, int lane, int gbase) {
-  const double* D = L.LF + ((size_t)R.t[s] * NX + lane) * NX;  // row `lane` of D_t
-  double acc = 0.0;
-#pragma unroll
-  for (int j = 0; j < NX; ++j) {
-    acc += (lane < NX ? D[j] : 0.0) * __shfl(v, gbase + j, 64);
-    if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+// D_t v for block t: sum_j D_t[lane][j] v_j with v_j in lane j of the group.
+// Zero on masked lanes.
+template <int G, int NX, int NU, int K>
+__device__ __forceinline__ double dmul(const Regs<NX, NU, K>& R, int s, const Lds& L, double v, int lane) {
+  double acc;
+  if constexpr (d_in_lds<NX>()) {
+    const double* col = L.LF + (size_t)R.t[s] * NX * NX + (lane < NX ? lane : 0);
+    acc = gdot_strided<G, NX, NX>(col, v, lane);
+  } else {
+    acc = gdotd<G, NX>(R.drow[s], v, lane);
   }
-  return acc;
+  return R.ok[s] ? acc : 0.0;
+}
+
+// Entry `lane` of block tb of an LDS vector (padded layout), 0 outside the block.
+template <int NX>
+__device__ __forceinline__ double own(const double* V, int tb, int lane) {
+  return lane < NX ? V[(tb + 1) * NX + lane] : 0.0;
+}
+
+// B_{t-1} Rinv B_{t-1}^T v_t for block t from the lane's own entry v of v_t:
+// the NU components of B_{t-1}^T v_t are group sums.  Also returns
+// (B^T v) . Rinv (B^T v) (identical in every lane of the group).
+template <int G, int NX, int NU, int K>
+__device__ __forceinline__ double bgram(const Regs<NX, NU, K>& R, int s, const Lds& L, double v, int salt,
+                                        double* quad) {
+  double g = 0.0, q = 0.0;
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const double bk = cvt_in_loop(R.brow[s][k], salt);
+    const double sk = gsum<G>(bk * v);
+    const double yk = L.rinv[k] * sk;
+    g += bk * yk;
+    q += sk * yk;
+  }
+  *quad = q;
+  return g;
 }
 
 // Block-Jacobi-preconditioned CG on S lam = gamma (pcg.py:350-445) with the
 // reference's stop rule: at least one iteration, then continue while
 // |eta| > eps and it <= max_iter - 1.  lam enters as the guess in L.lamS
 // (padded) and leaves in registers.  Returns the iteration count.
-template <int NX, int NU, int K>
-__device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K], const Lds& L, int N,
-                   int lane, int gbase, double eps, int max_iter, double* eta_out) {
-  double r[K], p[K], rt[K], ap[K], bp[K];
-#pragma unroll
-  for (int s = 0; s < K; ++s) {
-    ap[s] = 0.0;
-    bp[s] = 0.0;
-  }
+//
+// S = -G H^-1 G^T is applied matrix-free.  Row block t of G H^-1 G^T p is
+//   (t == 0 ? 1 : -1) Qinv_t w_t + A_{t-1} Qinv_{t-1} w_{t-1} + B_{t-1} Rinv B_{t-1}^T p_t
+// with w_t = (t == 0 ? p_t : -p_t) + A_t^T p_{t+1}.  Each lane loads only its
+// own entry of a neighbour block's vector from LDS; products with a block's
+// matrices broadcast the entries inside the lane group with DPP, and the B
+// term is local to the block.  Three workgroup barriers per iteration.
+template <int G, int NX, int NU, int K>
+__device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K], const Lds& L, int N, int lane,
+                   double eps, int max_iter, double* eta_out) {
+  double r[K], p[K], rt[K], ap[K], yxo[K], gb[K];
+  const bool lead = lane == 0;
   // ---- r = gamma - S lam0 -------------------------------------------------
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     const int t = R.t[s];
-    lam[s] = 0.0;
+    const double pv = own<NX>(L.lamS, t, lane);
+    const double pn = own<NX>(L.lamS, t + 1, lane);
+    const double wx = ((t == 0) ? pv : -pv) + gdot<G, NX>(R.acol[s], pn, lane, 0);
+    double q;
+    gb[s] = bgram<G>(R, s, L, pv, 0, &q);
+    lam[s] = R.ok[s] ? pv : 0.0;
     p[s] = 0.0;
-    if (R.ok[s]) {
-      const double pv = L.lamS[(t + 1) * NX + lane];
-      lam[s] = pv;
-      double wx = (t == 0) ? pv : -pv, wu = 0.0;
-#pragma unroll
-      for (int j = 0; j < NX; ++j) {
-        const double pn = L.lamS[(t + 2) * NX + j];
-        wx += (double)R.acol[s][j] * pn;
-        if (lane < NU && t < N) wu += (double)L.BT[(t * NU + lane) * NX + j] * pn;
-      }
-      L.YX[(t + 1) * NX + lane] = R.qi[s] * wx;
-      if (lane < NU && t < N) L.YU[(t + 1) * NU + lane] = L.rinv[lane] * wu;
-    }
+    ap[s] = 0.0;
+    yxo[s] = R.qi[s] * wx;
+    if (R.ok[s]) L.YX[(t + 1) * NX + lane] = yxo[s];
   }
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     const int t = R.t[s];
-    double ups = 0.0;
-    if (R.ok[s]) {
-      double g = (t == 0 ? 1.0 : -1.0) * L.YX[(t + 1) * NX + lane];
-      g += dot_rl<NX>(R.arow[s], L.YX + t * NX);
-#pragma unroll
-      for (int k = 0; k < NU; ++k) g += (double)R.brow[s][k] * L.YU[t * NU + k];
-      ups = -g;
-    }
-    r[s] = R.ok[s] ? gam[s] - ups : 0.0;
-    const double y = dmul(R, L, s, r[s], lane, gbase);
+    const double u = own<NX>(L.YX, t - 1, lane);
+    const double g = (t == 0 ? 1.0 : -1.0) * yxo[s] + gdot<G, NX>(R.arow[s], u, lane, 0) + gb[s];
+    r[s] = R.ok[s] ? gam[s] + g : 0.0;  // gamma - S lam0, S lam0 = -g
+    const double y = dmul<G>(R, s, L, r[s], lane);
     if (R.ok[s]) {
       L.Y[(t + 1) * NX + lane] = y;
       L.YQ[(t + 1) * NX + lane] = R.qi[s] * y;
@@ -346,18 +457,13 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     const int t = R.t[s];
-    double z = 0.0;
-    if (R.ok[s]) {
-      double a1 = 0.0, a2 = 0.0;
-      a1 = dot_rl<NX>(R.arow[s], L.YQ + t * NX);
-      a2 = dot_rl<NX>(R.acol[s], L.Y + (t + 2) * NX);
-      z = r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2;
-    }
-    rt[s] = dmul(R, L, s, z, lane, gbase);
+    const double a1 = gdot<G, NX>(R.arow[s], own<NX>(L.YQ, t - 1, lane), lane, 0);
+    const double a2 = gdot<G, NX>(R.acol[s], own<NX>(L.Y, t + 1, lane), lane, 0);
+    const double z = R.ok[s] ? r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2 : 0.0;
+    rt[s] = dmul<G>(R, s, L, z, lane);
     if (R.ok[s]) {
       part += r[s] * rt[s];
       L.RT[(t + 1) * NX + lane] = rt[s];
-      L.P[(t + 1) * NX + lane] = 0.0;
     }
   }
   double eta = block_sum(part, L.redC);
@@ -366,39 +472,25 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
   while (true) {
     const bool cont = (fabs(eta) > eps && it <= max_iter - 1) || it < 1;
     if (!cont) break;
-#pragma unroll
-    for (int s = 0; s < K; ++s) {
-#pragma unroll
-      for (int j = 0; j < NX; ++j) {
-        opaque(R.acol[s][j]);
-        opaque(R.arow[s][j]);
-      }
-#pragma unroll
-      for (int k = 0; k < NU; ++k) opaque(R.brow[s][k]);
-    }
     // ---- phase A: p = rt + beta p; w = G^T p; y = H^-1 w; v = -w.y --------
     // A_t^T p_{t+1} = A_t^T rt_{t+1} + beta A_t^T p_{t+1}(previous), so only
-    // the neighbour's rt is read; the previous product is kept in registers.
+    // the neighbour's rt is read; the previous product stays in registers.
     double vpart = 0.0;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
       const int t = R.t[s];
+      const double pv = rt[s] + beta * p[s];
+      p[s] = pv;
+      const double rn = own<NX>(L.RT, t + 1, lane);
+      ap[s] = gdot<G, NX>(R.acol[s], rn, lane, it) + beta * ap[s];
+      double q;
+      gb[s] = bgram<G>(R, s, L, pv, it, &q);
+      const double wx = ((t == 0) ? pv : -pv) + ap[s];
+      yxo[s] = R.qi[s] * wx;
       if (R.ok[s]) {
-        const double pv = rt[s] + beta * p[s];
-        p[s] = pv;
-        const double art = dot_rl<NX>(R.acol[s], L.RT + (t + 2) * NX);
-        const double brt = (lane < NU && t < N) ? dot_rl<NX>(L.BT + (t * NU + lane) * NX, L.RT + (t + 2) * NX) : 0.0;
-        ap[s] = art + beta * ap[s];
-        bp[s] = brt + beta * bp[s];
-        const double wx = ((t == 0) ? pv : -pv) + ap[s];
-        const double yx = R.qi[s] * wx;
-        L.YX[(t + 1) * NX + lane] = yx;
-        vpart += wx * yx;
-        if (lane < NU && t < N) {
-          const double yu = L.rinv[lane] * bp[s];
-          L.YU[(t + 1) * NU + lane] = yu;
-          vpart += bp[s] * yu;
-        }
+        L.YX[(t + 1) * NX + lane] = yxo[s];
+        vpart += wx * yxo[s];
+        if (lead) vpart += q;  // u_{t-1} term, once per block
       }
     }
     const double v = -block_sum(vpart, L.redA);
@@ -407,16 +499,13 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
 #pragma unroll
     for (int s = 0; s < K; ++s) {
       const int t = R.t[s];
+      const double u = own<NX>(L.YX, t - 1, lane);
+      const double g = (t == 0 ? 1.0 : -1.0) * yxo[s] + gdot<G, NX>(R.arow[s], u, lane, it) + gb[s];
       if (R.ok[s]) {
-        double g = (t == 0 ? 1.0 : -1.0) * L.YX[(t + 1) * NX + lane];
-#pragma unroll
-        for (int j = 0; j < NX; ++j) g += (double)R.arow[s][j] * L.YX[t * NX + j];
-#pragma unroll
-        for (int k = 0; k < NU; ++k) g += (double)R.brow[s][k] * L.YU[t * NU + k];
         lam[s] += alpha * p[s];
-        r[s] += alpha * g;  // r -= alpha * (-g)
+        r[s] += alpha * g;  // r -= alpha * ups, ups = -g
       }
-      const double y = dmul(R, L, s, r[s], lane, gbase);
+      const double y = dmul<G>(R, s, L, r[s], lane);
       if (R.ok[s]) {
         L.Y[(t + 1) * NX + lane] = y;
         L.YQ[(t + 1) * NX + lane] = R.qi[s] * y;
@@ -428,21 +517,13 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
 #pragma unroll
     for (int s = 0; s < K; ++s) {
       const int t = R.t[s];
-      double z = 0.0;
-      if (R.ok[s]) {
-        double a1 = 0.0, a2 = 0.0;
-#pragma unroll
-        for (int j = 0; j < NX; ++j) {
-          a1 += (double)R.arow[s][j] * L.YQ[t * NX + j];
-          a2 += (double)R.acol[s][j] * L.Y[(t + 2) * NX + j];
-        }
-        z = r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2;
-      }
-      rt[s] = dmul(R, L, s, z, lane, gbase);
+      const double a1 = gdot<G, NX>(R.arow[s], own<NX>(L.YQ, t - 1, lane), lane, it);
+      const double a2 = gdot<G, NX>(R.acol[s], own<NX>(L.Y, t + 1, lane), lane, it);
+      const double z = R.ok[s] ? r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2 : 0.0;
+      rt[s] = dmul<G>(R, s, L, z, lane);
       if (R.ok[s]) {
         epart += r[s] * rt[s];
         L.RT[(t + 1) * NX + lane] = rt[s];
-        L.P[(t + 1) * NX + lane] = p[s];
       }
     }
     const double eta_new = block_sum(epart, L.redB);
@@ -488,28 +569,36 @@ __device__ __forceinline__ void gamma_rows(const Regs<NX, NU, K>& R, const Lds& 
 }
 
 // Primal recovery from the multipliers in L.lamS (pcg.py:316-348).
-template <int NX, int NU, int K>
+// Block t yields state row t and control row t - 1 (lanes < nu):
+//   x_t = -Qinv_t (q_t + (t == 0 ? lam_t : -lam_t) + A_t^T lam_{t+1}),
+//   u_{t-1} = -Rinv (r_{t-1} + B_{t-1}^T lam_t);
+// block 0 yields the zero last control row instead (control_row()).
+template <int G, int NX, int NU, int K>
 __device__ __forceinline__ void recover_rows(const Regs<NX, NU, K>& R, const Lds& L, int N, int lane,
                                              double (&xn)[K], double (&un)[K]) {
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     const int t = R.t[s];
+    const double lt = own<NX>(L.lamS, t, lane);
+    const double ln = own<NX>(L.lamS, t + 1, lane);
+    const double ax = gdot<G, NX>(R.acol[s], ln, lane, 0);
+    double bl = 0.0;  // component `lane` of B_{t-1}^T lam_t
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      const double sk = gsum<G>((double)R.brow[s][k] * lt);
+      bl = (lane == k) ? sk : bl;
+    }
     xn[s] = 0.0;
     un[s] = 0.0;
     if (R.ok[s]) {
-      const double lt = L.lamS[(t + 1) * NX + lane];
-      double ax = 0.0, bx = 0.0;
-#pragma unroll
-      for (int j = 0; j < NX; ++j) {
-        const double ln = L.lamS[(t + 2) * NX + j];
-        ax += (double)R.acol[s][j] * ln;
-        if (lane < NU && t < N) bx += (double)L.BT[(t * NU + lane) * NX + j] * ln;
-      }
       xn[s] = -R.qi[s] * (L.qv[t * NX + lane] + (t == 0 ? lt : -lt) + ax);
-      if (lane < NU && t < N) un[s] = -L.rinv[lane] * (L.rv[t * NU + lane] + bx);
+      if (lane < NU && t >= 1) un[s] = -L.rinv[lane] * (L.rv[(t - 1) * NU + lane] + bl);
     }
   }
 }
+
+// Control row produced by block t (see recover_rows).
+__device__ __forceinline__ int control_row(int t, int N) { return t == 0 ? N : t - 1; }
 
 template <int NX, int NU, int K>
 __device__ __forceinline__ void init_blocks(Regs<NX, NU, K>& R, int G, int N, int* lane_out, int* gbase_out) {
@@ -530,9 +619,8 @@ template <int NX, int NU>
 __device__ void zero_lds_vectors(const Lds& L, int N) {
   const int NBP = N + 3;
   for (int k = threadIdx.x; k < NBP * NX; k += blockDim.x) {
-    L.P[k] = 0.0; L.RT[k] = 0.0; L.Y[k] = 0.0; L.YQ[k] = 0.0; L.YX[k] = 0.0; L.lamS[k] = 0.0;
+    L.RT[k] = 0.0; L.Y[k] = 0.0; L.YQ[k] = 0.0; L.YX[k] = 0.0; L.lamS[k] = 0.0;
   }
-  for (int k = threadIdx.x; k < NBP * NU; k += blockDim.x) L.YU[k] = 0.0;
 }
 
 // Closed-form quadratic model of the diagonal cost (sqp.py:160-197,
@@ -643,20 +731,20 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   double gam[K], lam[K];
   gamma_rows<true>(R, L, lane, gam);
   double eta;
-  const int npcg = pcg(R, gam, lam, L, N, lane, gbase, a.pcg_eps, a.pcg_max, &eta);
+  const int npcg = pcg<G>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta);
 #pragma unroll
   for (int s = 0; s < K; ++s)
     if (R.ok[s]) L.lamS[(R.t[s] + 1) * NX + lane] = lam[s];
   __syncthreads();
   double xn[K], un[K];
-  recover_rows(R, L, N, lane, xn, un);
+  recover_rows<G>(R, L, N, lane, xn, un);
   double e = 0.0;
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     if (R.ok[s]) {
       const int t = R.t[s];
       e = nanmax(e, fabs(xn[s] - L.xb[t * NX + lane]));
-      if (lane < NU) e = nanmax(e, fabs(un[s] - L.ub[t * NU + lane]));
+      if (lane < NU) e = nanmax(e, fabs(un[s] - L.ub[control_row(t, N) * NU + lane]));
     }
   }
   const double err = block_max(e, L.redA);
@@ -666,7 +754,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
       const int t = R.t[s];
       a.x[((size_t)b * NB + t) * NX + lane] = xn[s];
       a.lam[((size_t)b * NB + t) * NX + lane] = lam[s];
-      if (lane < NU) a.u[((size_t)b * NB + t) * NU + lane] = un[s];
+      if (lane < NU) a.u[((size_t)b * NB + control_row(t, N)) * NU + lane] = un[s];
     }
   }
   for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) a.qinv[(size_t)b * NB * NX + k] = L.qinv[k];
@@ -740,17 +828,17 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   double gam[K], lam[K];
   gamma_rows<false>(R, L, lane, gam);
   double eta;
-  const int npcg = pcg(R, gam, lam, L, N, lane, gbase, a.pcg_eps, a.pcg_max, &eta);
+  const int npcg = pcg<G>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta);
 #pragma unroll
   for (int s = 0; s < K; ++s)
     if (R.ok[s]) L.lamS[(R.t[s] + 1) * NX + lane] = lam[s];
   __syncthreads();
   double xt[K], ut[K];
-  recover_rows(R, L, N, lane, xt, ut);
+  recover_rows<G>(R, L, N, lane, xt, ut);
   // per-row contraction terms -> LDS (reusing the PCG vectors), then one
   // thread per component sums over time in a fixed order
   double* TX = L.YX;  // (NB, NX): 2 (x - ref) x~
-  double* TU = L.YU;  // (NB, NU)
+  double* TU = L.Y;   // (NB, NU)
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     if (R.ok[s]) {
@@ -766,10 +854,11 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
         a.g_xf[b * NX + i] = 2.0 * (wxi + a.wf[b * NX + i]) * xt[s];
       }
       if (i < NU) {
-        const double ur = a.penalize ? a.uref[((size_t)b * NB + t) * NU + i] : 0.0;
-        TU[t * NU + i] = (t < N) ? 2.0 * (L.ub[t * NU + i] - ur) * ut[s] : 0.0;
-        a.g_uref[((size_t)b * NB + t) * NU + i] =
-            (t < N && a.penalize) ? 2.0 * a.wu[b * NU + i] * ut[s] : 0.0;
+        const int tu = control_row(t, N);
+        const double ur = a.penalize ? a.uref[((size_t)b * NB + tu) * NU + i] : 0.0;
+        TU[tu * NU + i] = (tu < N) ? 2.0 * (L.ub[tu * NU + i] - ur) * ut[s] : 0.0;
+        a.g_uref[((size_t)b * NB + tu) * NU + i] =
+            (tu < N && a.penalize) ? 2.0 * a.wu[b * NU + i] * ut[s] : 0.0;
       }
       a.lam_bwd[((size_t)b * NB + t) * NX + i] = lam[s];
       if (t == 0) a.g_x0[b * NX + i] = lam[s];
@@ -806,7 +895,9 @@ Launch plan(int N, int P) {
   constexpr int NX = Model::NX, NU = Model::NU;
   const int NB = N + 1;
   Launch l{0, 0, LdsPlan<NX, NU>::bytes(N, P)};
-  const int cand[3] = {1, 2, 4};
+  // two blocks per thread group first: twice the registers per block group
+  // (measured fastest where both fit), then one, then four
+  const int cand[3] = {2, 1, 4};
   for (int c = 0; c < 3; ++c) {
     const int K = cand[c];
     if (g_forced_K > 0 && K != g_forced_K) continue;
