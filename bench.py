@@ -39,7 +39,22 @@ def main():
     ap.add_argument("--horizon", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=96, help="instances in the CPU baseline sample")
+    ap.add_argument("--fused-only", action="store_true",
+                    help="time only the fused path (for profiler runs); prints the fused figures")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB mebibytes device-to-device once (reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "check": float(dst[-1])}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: instances are independent and this build drives a single GPU "
               "per process; run with --gpus 1", file=sys.stderr)
@@ -108,6 +123,13 @@ def main():
     flops_qp_launch = pcg_flops(N, nx) * pcg_fwd / sp["num_scp_iteration_max"]
     qp_avg_ms = qp_ms / max(qp_n, 1)
     achieved = flops_qp_launch / (qp_avg_ms * 1e-3) / 1e12
+
+    if args.fused_only:
+        print(json.dumps({"fused_ms_fwd": t_fwd * 1e3, "fused_ms_fwd_bwd": t_fb * 1e3,
+                          "qp_iteration_launch_avg_ms": qp_avg_ms, "qp_iteration_launches": qp_n,
+                          "backward_launch_avg_ms": bw_ms / max(bw_n, 1), "pcg_forward": pcg_fwd,
+                          "pcg_backward": pcg_bwd, "achieved_tflops": achieved}), flush=True)
+        return
 
     # ---- per-launch path --------------------------------------------------
     psolver, pparams, pguess = make("plain")

@@ -1,17 +1,21 @@
 // Fused batched SQP solve and adjoint backward for CDNA4 (gfx950).
 //
-// One workgroup owns one problem instance for the whole SQP solve
-// (diffmpc/solvers/sqp.py:436-560): linearization, Schur/preconditioner
-// assembly, the PCG loop (pcg.py:350-445) and primal recovery all run inside
-// one persistent kernel with no host round trip.
+// One SQP iteration of the batch (diffmpc/solvers/sqp.py:262-299) is two
+// launches: linearize_kernel (one thread per instance, step and tangent) and
+// qp_iteration_kernel (one workgroup per instance: Schur/preconditioner
+// assembly, the PCG loop of pcg.py:350-445 and primal recovery).  All
+// num_scp_iteration_max iterations are enqueued back to back with no host
+// round trip; an instance that has met the stop rule of sqp.py:460-466 skips
+// its work in later launches.  fused_backward_kernel is the adjoint solve.
 //
 // Work mapping.  Time block t (0..N) of the Schur system is owned by a group
-// of G lanes (G = next power of two >= nx, so a block never straddles a
-// wavefront); lane i of the group owns row i of the block.  A thread group
-// owns K blocks t = s * NG + group (s < K).  Block-local products (the
-// Cholesky factor of theta_t, y = theta_t^-1 r_t) use cross-lane shuffles;
-// only neighbour-block vectors go through LDS, so one PCG iteration costs
-// three workgroup barriers.
+// of G lanes (G = next power of two >= nx, so a block never straddles a DPP
+// row or a wavefront); lane i of the group owns row i of the block.  A thread
+// group owns K blocks t = s * NG + group (s < K).  Products with a block's
+// matrices keep one matrix row per lane in registers and broadcast the vector
+// entries inside the group with DPP; each lane reads only its own entry of a
+// neighbour block's vector from LDS, so one PCG iteration costs three
+// workgroup barriers and about six LDS accesses per lane.
 //
 // Numerics.  Dynamics are integrated in float32 with forward-mode dual
 // numbers (one tangent per lane).  Everything of the QP is float64: the
@@ -20,9 +24,11 @@
 //   Phi^-1 r |_t = D_t (r_t - phi_t D_{t-1} r_{t-1} - phi_{t+1}^T D_{t+1} r_{t+1}),
 //   D_t = theta_t^-1,
 // which equals the explicit Phiinv rows of pcg.py:213-239, is applied in
-// factored form with D_t rows (float64) in registers.  As_next = -I and the
-// diagonal cost Hessians of optimal_control_problem.py:301-322 are used
-// in closed form.
+// factored form with D_t (float64) in registers (nx <= 8) or LDS (nx > 8).
+// As_next = -I and the diagonal cost Hessians of
+// optimal_control_problem.py:301-322 are used in closed form.
+#include <type_traits>
+
 #include "dmpc_internal.h"
 #include "dmpc_models.h"
 
@@ -163,6 +169,7 @@ __device__ __forceinline__ double cvt_in_loop(float f, int salt) {
   asm("v_cvt_f64_f32 %0, %1" : "=v"(d) : "v"(f), "s"(salt));
   return d;
 }
+__device__ __forceinline__ double cvt_in_loop(double d, int) { return d; }
 
 // ---------------------------------------------------------------------------
 // Broadcast of lane n of each G-lane block group to the whole group with DPP
@@ -200,8 +207,8 @@ __device__ __forceinline__ double gbcast(double v, int lane) {
 }
 
 // sum_j a[j] * v_j where v_j lives in lane j of the group (j < M <= NX).
-template <int G, int M, int j = 0>
-__device__ __forceinline__ double gdot(const float* a, double v, int lane, int salt, double acc = 0.0) {
+template <int G, int M, int j = 0, class AT>
+__device__ __forceinline__ double gdot_chain(const AT* a, double v, int lane, int salt, double acc) {
   if constexpr (j == M) {
     return acc;
   } else {
@@ -209,7 +216,36 @@ __device__ __forceinline__ double gdot(const float* a, double v, int lane, int s
     // keep the scheduler from converting and broadcasting a whole row ahead
     // of its FMAs (each in-flight term holds four VGPRs)
     if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-    return gdot<G, M, j + 1>(a, v, lane, salt, acc);
+    return gdot_chain<G, M, j + 1>(a, v, lane, salt, acc);
+  }
+}
+
+// acc + sum_{j<13} a[j] * v_j for 16-lane groups with a[] in float64
+// registers: each term is one v_fmac_f64 whose first operand is read through
+// DPP row_newbcast:j (lane j of the 16-lane row), so the broadcast costs no
+// instruction.  One asm block keeps the chain free of compiler-inserted
+// writes to v; the leading s_nop 4 covers the DPP read-after-VALU-write wait
+// states (2 for a VGPR, 5 after a VALU write of EXEC).  Terms are summed in
+// order j = 0..12 with one rounding each, as the generic chain does.
+__device__ __forceinline__ double gdot16x13(const double* a, double v, double acc) {
+#define DMPC_FMAC_BCAST(op, j) "v_fmac_f64_dpp %0, %1, %" #op " row_newbcast:" #j " row_mask:0xf bank_mask:0xf\n"
+  asm("s_nop 4\n" DMPC_FMAC_BCAST(2, 0) DMPC_FMAC_BCAST(3, 1) DMPC_FMAC_BCAST(4, 2) DMPC_FMAC_BCAST(5, 3)
+          DMPC_FMAC_BCAST(6, 4) DMPC_FMAC_BCAST(7, 5) DMPC_FMAC_BCAST(8, 6) DMPC_FMAC_BCAST(9, 7)
+              DMPC_FMAC_BCAST(10, 8) DMPC_FMAC_BCAST(11, 9) DMPC_FMAC_BCAST(12, 10) DMPC_FMAC_BCAST(13, 11)
+                  DMPC_FMAC_BCAST(14, 12)
+      : "+v"(acc)
+      : "v"(v), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]),
+        "v"(a[8]), "v"(a[9]), "v"(a[10]), "v"(a[11]), "v"(a[12]));
+#undef DMPC_FMAC_BCAST
+  return acc;
+}
+
+template <int G, int M, class AT>
+__device__ __forceinline__ double gdot(const AT* a, double v, int lane, int salt) {
+  if constexpr (G == 16 && M == 13 && std::is_same<AT, double>::value) {
+    return gdot16x13(a, v, 0.0);
+  } else {
+    return gdot_chain<G, M>(a, v, lane, salt, 0.0);
   }
 }
 template <int G, int M, int j = 0>
@@ -245,9 +281,12 @@ __device__ __forceinline__ double gdot_strided(const double* a, double v, int la
 // Per-thread register state of the K blocks it owns.
 template <int NX, int NU, int K>
 struct Regs {
-  float acol[K][NX];   // column `lane` of A_t            (zero for t = N)
-  float arow[K][NX];   // row `lane` of A_{t-1}           (zero for t = 0)
-  float brow[K][NU];   // row `lane` of B_{t-1}           (zero for t = 0)
+  // A/B entries are float32 values; with one block per thread group they are
+  // kept as float64 (no conversion in the PCG loop), with more as float32
+  using AT = typename std::conditional<K == 1, double, float>::type;
+  AT acol[K][NX];   // column `lane` of A_t            (zero for t = N)
+  AT arow[K][NX];   // row `lane` of A_{t-1}           (zero for t = 0)
+  AT brow[K][NU];   // row `lane` of B_{t-1}           (zero for t = 0)
                        // (B_{t-1} terms of S are formed by block t alone)
   double drow[K][d_in_lds<NX>() ? 1 : NX];  // row `lane` of D_t = theta_t^-1 (registers mode)
   double qi[K];        // Qinv_t[lane, lane]

@@ -2,10 +2,11 @@
  *
  * All data pointers are DEVICE pointers to contiguous, batch-major arrays;
  * every call is asynchronous on the given HIP stream (passed as void*) and
- * returns DMPC_OK or a negative error code.  Nothing here allocates on the
- * solve path except the per-launch (plain) mode's scratch, which it frees
- * before returning.  N is the reference's `horizon`: trajectories have N+1
- * knots and the last control row is zero.
+ * returns DMPC_OK or a negative error code.  The fused mode allocates
+ * nothing: every buffer is the caller's.  The per-launch (plain) mode keeps a
+ * device scratch per device that grows on demand and lives until the process
+ * ends.  N is the reference's `horizon`: trajectories have N+1 knots and the
+ * last control row is zero.
  *
  * Reference interfaces replaced (ToyotaResearchInstitute/diffmpc):
  *   dmpc_sqp_solve      SQPSolver.solve / custom_vjp primal   diffmpc/solvers/sqp.py:572-603 (+ _solve :436-560)
@@ -38,7 +39,7 @@ extern "C" {
 #define DMPC_MODEL_CARTPOLE 4    /* masscart, masspole, length, gravity (4/1)  cartpole_dynamics.py:36-80 */
 #define DMPC_MODEL_CHAIN 5       /* spring k[12], damper c[12]   (48/12) planar chain, no reference counterpart */
 
-#define DMPC_MODE_FUSED 0 /* one persistent workgroup per instance for the whole SQP solve */
+#define DMPC_MODE_FUSED 0 /* one workgroup per instance per SQP iteration, no host synchronisation */
 #define DMPC_MODE_PLAIN 1 /* one kernel per operation, host-driven loops (the reference's op graph) */
 
 typedef struct dmpc_problem {

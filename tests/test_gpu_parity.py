@@ -74,6 +74,37 @@ def test_sqp_solve_matches_oracle(cuda, case, mode):
     assert np.allclose(ce, cr, rtol=1e-3, atol=1e-5 * np.max(np.abs(ref["states"])))
 
 
+@pytest.mark.parametrize("blocks", [1, 2, 4])
+@pytest.mark.parametrize("case", ["lti", "quadrotor", "spacecraft7", "cartpole"])
+def test_every_blocks_per_group_variant_matches_oracle(cuda, case, blocks):
+    """Each fused kernel variant (time blocks per lane group K = 1, 2, 4 --
+    different register layouts and product code) solves and differentiates
+    like the oracle; the automatic choice covers only one of them."""
+    from diffmpc_amd import _lib
+
+    say(f"[blocks] case={case} K={blocks}")
+    lib = _lib.load()
+    name, prob, sp = CASES[case]()
+    _lib.check(lib.dmpc_set_blocks_per_group(blocks), "dmpc_set_blocks_per_group")
+    try:
+        solver, params, guess = W.make_solver(name, prob, sp, device=cuda, mode="fused")
+        wx = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
+        sol = solver.solve(guess, params, {"weights_penalization_reference_state_trajectory": wx})
+        loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+        (g,) = torch.autograd.grad(loss, [wx])
+        torch.cuda.synchronize()
+    finally:
+        lib.dmpc_set_blocks_per_group(0)
+    op = oracle_problem(name, prob)
+    ref = osqp.sqp_solve(op, sp)
+    gref = osqp.sqp_backward(op, sp, ref, ref["states"] - prob["x_ref"], np.zeros_like(ref["controls"]))
+    say("  rel states", rel(_np(sol.states), ref["states"]), "grad", rel(_np(g), gref["w_x"]))
+    assert rel(_np(sol.states), ref["states"]) <= 1e-5
+    assert rel(_np(sol.controls), ref["controls"]) <= 1e-5
+    assert rel(_np(sol.dual), ref["dual"]) <= 1e-5
+    assert rel(_np(g), gref["w_x"]) <= 1e-4
+
+
 @pytest.mark.parametrize("case", ["quadrotor", "spacecraft7", "lti"])
 def test_fused_and_plain_agree(cuda, case):
     say(f"[fused-vs-plain] case={case}")
