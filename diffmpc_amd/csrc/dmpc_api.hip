@@ -148,7 +148,7 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   if (!saved->AB || !saved->fd || !saved->qinv || !saved->rinv) return DMPC_ERR_ARG;
   if (params->num_scp_iteration_max < 1 || params->pcg_max_iter < 1) return DMPC_ERR_ARG;
   if (params->n_alphas < 0 || params->n_alphas > kMaxAlphas) return DMPC_ERR_ARG;
-  if (params->linesearch && !linesearch_is_noop(params)) return DMPC_ERR_UNSUPPORTED;
+  if (params->linesearch && (params->n_alphas < 1 || params->n_alphas > 8)) return DMPC_ERR_ARG;
   SolveArgs a;
   std::memset(&a, 0, sizeof(a));
   a.B = prob->batch;
@@ -157,11 +157,13 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   a.scheme = prob->scheme;
   a.penalize = prob->penalize_control_reference;
   a.dt = (float)prob->dt;
+  a.dt64 = prob->dt;
   a.maxit = params->num_scp_iteration_max;
   a.tol = params->tol_convergence;
   a.pcg_max = params->pcg_max_iter;
   a.pcg_eps = params->pcg_tol_epsilon;
   a.linesearch = params->linesearch;
+  a.ls_eval = params->linesearch && !linesearch_is_noop(params);
   a.ls_eta = params->linesearch_eta;
   a.n_alphas = params->n_alphas;
   for (int k = 0; k < params->n_alphas; ++k) a.alphas[k] = params->linesearch_alphas[k];

@@ -121,18 +121,28 @@ struct LdsPlan {
            NB * NX * NX /*LF*/ + NB * NX /*ID*/;
   }
   static __host__ __device__ size_t floats(int N, int P) { return (size_t)N * NX + P; }
-  static __host__ size_t bytes(int N, int P) { return doubles(N) * 8 + floats(N, P) * 4 + 16; }
+  // doubles(N) plus `ls` doubles of line-search scratch
+  static __host__ size_t bytes(int N, int P, int ls = 0) {
+    return (doubles(N) + ls) * 8 + floats(N, P) * 4 + 16;
+  }
+};
+
+// Line-search scratch: merit parts of (n_alphas + 1) x (N + 1) items (cost,
+// constraint) and their (n_alphas + 1) sums of each.
+__host__ __device__ inline int linesearch_doubles(int linesearch, int n_alphas, int N) {
+  return linesearch ? 2 * (n_alphas + 1) * (N + 1) + 2 * (n_alphas + 1) : 0;
 };
 
 struct Lds {
   double *xb, *ub, *lamS, *qinv, *qv, *rv, *rinv, *x0, *redA, *redB, *redC;
   double *RT, *Y, *YQ, *YX;
   double *LF, *ID;  // Cholesky factor rows of theta_t and 1 / L_kk
+  double* LS;       // line-search scratch (see linesearch_doubles)
   float *fd, *prm;
 };
 
 template <int NX, int NU>
-__device__ Lds carve(void* base, int N) {
+__device__ Lds carve(void* base, int N, int ls = 0) {
   const int NB = N + 1, NBP = N + 3;
   double* d = reinterpret_cast<double*>(base);
   Lds L;
@@ -153,6 +163,7 @@ __device__ Lds carve(void* base, int N) {
   L.YX = d; d += NBP * NX;
   L.LF = d; d += NB * NX * NX;
   L.ID = d; d += NB * NX;
+  L.LS = d; d += ls;
   float* f = reinterpret_cast<float*>(d);
   L.fd = f; f += N * NX;
   L.prm = f;
@@ -174,8 +185,8 @@ __device__ __forceinline__ double cvt_in_loop(double d, int) { return d; }
 // ---------------------------------------------------------------------------
 // Broadcast of lane n of each G-lane block group to the whole group with DPP
 // (no LDS traffic): G = 16 is one DPP row (row_newbcast:n, one 64-bit move on
-// gfx950); G = 4 a quad (quad_perm [n,n,n,n]); G = 8 takes row_newbcast of n
-// and 8 + n and selects by the lane's half of the row.
+// gfx950); G = 4 a quad (quad_perm [n,n,n,n]); G = 8 is half a row (two
+// bank-masked row_newbcast moves).
 template <int n>
 __device__ __forceinline__ double dpp_rowbcast(double v) {
   long long x = __double_as_longlong(v);
@@ -198,11 +209,12 @@ __device__ __forceinline__ double gbcast(double v, int lane) {
     return dpp_quadbcast<n>(v);
   } else {
     static_assert(G == 8, "block groups of 4, 8 or 16 lanes");
-    // groups are 8-aligned in the workgroup: the upper group of a DPP row
-    // is the one with bit 3 of the thread index set (`lane` is below 8)
-    const double lo = dpp_rowbcast<n>(v);
-    const double hi = dpp_rowbcast<n + 8>(v);
-    return (threadIdx.x & 8) ? hi : lo;
+    // groups are 8-aligned in the workgroup, two per DPP row: broadcast lane
+    // n into banks 0-1 (lanes 0-7) and lane 8 + n into banks 2-3 (lanes 8-15)
+    // of the same register; the bank masks do the selection
+    const long long x = __double_as_longlong(v);
+    const long long lo = __builtin_amdgcn_mov_dpp(x, 0x150 + n, 0xf, 0x3, false);
+    return __longlong_as_double(__builtin_amdgcn_update_dpp(lo, x, 0x150 + n + 8, 0xf, 0xc, false));
   }
 }
 
@@ -739,6 +751,117 @@ __global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
 // One SQP iteration of one instance per workgroup: Schur/preconditioner
 // assembly, gamma, the whole PCG loop, primal recovery and the step test
 // (sqp.py:262-299, 506-528).
+// Merit line search of sqp.py:301-411 for instance b (Nocedal & Wright,
+// Alg. 18.3): phi(z; mu) = cost(z) + mu * ||g(z)||_1, mu = max(0.1,
+// (dcost/dz . dz) / (0.5 ||g||_1)) (max propagates NaN, like jnp.maximum),
+// decrease_k = phi(z + alpha_k dz) - (phi(z) + eta alpha_k D phi), then the
+// candidates are scanned from the largest with the reference's clamped
+// indexing (alphas[min(i, n-1)]).  z = (xbar, ubar) in L.xb / L.ub, dz in
+// DX (N+1, nx) and DU (N+1, nu).  Everything is float64, including the
+// dynamics evaluations of the equality constraints
+// g = [x_0 - x_init; f_d(x_t, u_t) - x_{t+1}] (ocp.py:104-151).
+// Items (candidate k, knot t) are spread over the workgroup; the sums run
+// in a fixed order, so every thread returns the same alpha.
+template <class Model>
+__device__ double line_search(const SolveArgs& a, const Lds& L, int N, int b, const double* DX, const double* DU) {
+  constexpr int NX = Model::NX, NU = Model::NU;
+  const int NB = N + 1, nal = a.n_alphas, items = (nal + 1) * NB;
+  double* LC = L.LS;               // cost part of each item
+  double* LG = L.LS + items;       // constraint part of each item
+  double* SUM = L.LS + 2 * items;  // (nal + 1) x {cost, constraint}
+  const double* wx = a.wx + (size_t)b * NX;
+  const double* wu = a.wu + (size_t)b * NU;
+  const double* wf = a.wf + (size_t)b * NX;
+  const double* xr = a.xref + (size_t)b * NB * NX;
+  const double* ur = a.uref + (size_t)b * NB * NU;
+  const double* xf = a.xf + (size_t)b * NX;
+  const double* x0 = a.x0 + (size_t)b * NX;
+  const float* prm = a.dyn + (size_t)b * a.P;
+  // directional derivative of the cost (the last control row has none)
+  double part = 0.0;
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
+    const int t = k / NX, i = k - t * NX;
+    const double gx = (t < N) ? 2.0 * wx[i] * (L.xb[k] - xr[k]) : 2.0 * (wx[i] + wf[i]) * (L.xb[k] - xf[i]);
+    part += gx * DX[k];
+  }
+  for (int k = threadIdx.x; k < N * NU; k += blockDim.x) {
+    const int j = k % NU;
+    const double ref = a.penalize ? ur[k] : 0.0;
+    part += 2.0 * wu[j] * (L.ub[k] - ref) * DU[k];
+  }
+  const double dcdz = block_sum(part, L.redA);
+  // merit parts: item (kc, t) holds the cost of knot t and the constraint
+  // block t at alpha = 0 (kc = 0) or alphas[kc - 1]
+  for (int j = threadIdx.x; j < items; j += blockDim.x) {
+    const int kc = j / NB, t = j - kc * NB;
+    const double al = kc == 0 ? 0.0 : a.alphas[kc - 1];
+    double c = 0.0, g = 0.0;
+    if (t < N) {
+#pragma unroll
+      for (int i = 0; i < NX; ++i) {
+        const double d = L.xb[t * NX + i] + al * DX[t * NX + i] - xr[t * NX + i];
+        c += wx[i] * d * d;
+      }
+#pragma unroll
+      for (int i = 0; i < NU; ++i) {
+        const double d = L.ub[t * NU + i] + al * DU[t * NU + i] - (a.penalize ? ur[t * NU + i] : 0.0);
+        c += wu[i] * d * d;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NX; ++i) {
+        const double d = L.xb[t * NX + i] + al * DX[t * NX + i] - xf[i];
+        c += (wx[i] + wf[i]) * d * d;
+      }
+    }
+    if (t == 0) {
+#pragma unroll
+      for (int i = 0; i < NX; ++i) g += fabs(L.xb[i] + al * DX[i] - x0[i]);
+    } else {
+      double xp[NX], up[NU], fx[NX];
+#pragma unroll
+      for (int i = 0; i < NX; ++i) xp[i] = L.xb[(t - 1) * NX + i] + al * DX[(t - 1) * NX + i];
+#pragma unroll
+      for (int i = 0; i < NU; ++i) up[i] = L.ub[(t - 1) * NU + i] + al * DU[(t - 1) * NU + i];
+      step<Model, double>(xp, up, prm, a.dt64, a.scheme, fx);
+#pragma unroll
+      for (int i = 0; i < NX; ++i) g += fabs(fx[i] - (L.xb[t * NX + i] + al * DX[t * NX + i]));
+    }
+    LC[j] = c;
+    LG[j] = g;
+  }
+  __syncthreads();
+  if (threadIdx.x <= nal) {
+    double c = 0.0, g = 0.0;
+    for (int t = 0; t < NB; ++t) {
+      c += LC[threadIdx.x * NB + t];
+      g += LG[threadIdx.x * NB + t];
+    }
+    SUM[2 * threadIdx.x] = c;
+    SUM[2 * threadIdx.x + 1] = g;
+  }
+  __syncthreads();
+  const double c0 = SUM[0], g0 = SUM[1];
+  const double mu_min = dcdz / (0.5 * g0);
+  const double mu = (mu_min != mu_min) ? mu_min : fmax(0.1, mu_min);
+  const double merit = c0 + mu * g0;
+  const double dmerit = dcdz + -mu * g0;
+  double amax = a.alphas[0];
+  for (int k = 1; k < nal; ++k) amax = (a.alphas[k] != a.alphas[k] || amax != amax) ? NAN : fmax(amax, a.alphas[k]);
+  // decrease of candidate k (0-based in alphas)
+  auto decrease = [&](int k) {
+    const double al = a.alphas[k];
+    return (SUM[2 * (k + 1)] + mu * SUM[2 * (k + 1) + 1]) - (merit + a.ls_eta * al * dmerit);
+  };
+  int i = nal + 1;
+  double alpha = amax;
+  while (!(decrease(i < nal - 1 ? i : nal - 1) < 0.0 || i == 0)) {
+    alpha = a.alphas[(i - 1) < nal - 1 ? (i - 1) : nal - 1];
+    --i;
+  }
+  return alpha;
+}
+
 template <class Model, int G, int K, int MAXT>
 __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
@@ -747,7 +870,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   if (!sqp_active(a, b)) return;
   const int it = a.num_iter[b];
   const int N = a.N, NB = N + 1;
-  const Lds L = carve<NX, NU>(smem_, N);
+  const Lds L = carve<NX, NU>(smem_, N, linesearch_doubles(a.ls_eval, a.n_alphas, N));
   Regs<NX, NU, K> R;
   int lane, gbase;
   init_blocks(R, G, N, &lane, &gbase);
@@ -777,6 +900,29 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   __syncthreads();
   double xn[K], un[K];
   recover_rows<G>(R, L, N, lane, xn, un);
+  double alpha = 1.0;
+  if (a.ls_eval) {  // uniform over the workgroup
+    double* DX = L.YX;  // (N+1, nx): QP step of the states
+    double* DU = L.Y;   // (N+1, nu): QP step of the controls
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      if (R.ok[s]) {
+        const int t = R.t[s], tu = control_row(t, N);
+        DX[t * NX + lane] = xn[s] - L.xb[t * NX + lane];
+        if (lane < NU) DU[tu * NU + lane] = un[s] - L.ub[tu * NU + lane];
+      }
+    }
+    __syncthreads();
+    alpha = line_search<Model>(a, L, N, b, DX, DU);
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      if (R.ok[s]) {
+        const int t = R.t[s], tu = control_row(t, N);
+        xn[s] = L.xb[t * NX + lane] + alpha * DX[t * NX + lane];
+        if (lane < NU) un[s] = L.ub[tu * NU + lane] + alpha * DU[tu * NU + lane];
+      }
+    }
+  }
   double e = 0.0;
 #pragma unroll
   for (int s = 0; s < K; ++s) {
@@ -801,7 +947,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   if (threadIdx.x == 0) {
     a.conv_errs[(size_t)b * a.maxit + it] = err;
     a.pcg_iters[(size_t)b * a.maxit + it] = npcg;
-    if (a.linesearch) a.ls_alphas[(size_t)b * a.maxit + it] = 1.0;
+    if (a.linesearch) a.ls_alphas[(size_t)b * a.maxit + it] = alpha;
     a.conv_err[b] = err;
     a.num_iter[b] = it + 1;
   }
@@ -930,10 +1076,10 @@ struct Launch {
 inline int max_threads(int K) { return K == 1 ? 1024 : (K == 2 ? 512 : 256); }
 
 template <class Model, int G>
-Launch plan(int N, int P) {
+Launch plan(int N, int P, int ls = 0) {
   constexpr int NX = Model::NX, NU = Model::NU;
   const int NB = N + 1;
-  Launch l{0, 0, LdsPlan<NX, NU>::bytes(N, P)};
+  Launch l{0, 0, LdsPlan<NX, NU>::bytes(N, P, ls)};
   // two blocks per thread group first: twice the registers per block group
   // (measured fastest where both fit), then one, then four
   const int cand[3] = {2, 1, 4};
@@ -992,7 +1138,7 @@ int launch_bwd_k(const BackwardArgs& a, const Launch& l, hipStream_t st) {
 
 template <class Model, int G>
 int fwd_model(const SolveArgs& a, hipStream_t st) {
-  Launch l = plan<Model, G>(a.N, a.P);
+  Launch l = plan<Model, G>(a.N, a.P, linesearch_doubles(a.ls_eval, a.n_alphas, a.N));
   if (l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
   switch (l.K) {
     case 1: return launch_fwd_k<Model, G, 1, 1024>(a, l, st);

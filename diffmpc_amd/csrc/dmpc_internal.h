@@ -14,12 +14,14 @@ constexpr int kMaxAlphas = 8;
 // Everything one batched SQP solve needs, flattened for kernel arguments.
 struct SolveArgs {
   int B, N, P, scheme, penalize;
-  float dt;
+  float dt;       // step of the float32 linearization
+  double dt64;    // step of the float64 line-search evaluations
   int maxit;
   double tol;
   int pcg_max;
   double pcg_eps;
-  int linesearch;
+  int linesearch;  // record alphas in the history
+  int ls_eval;     // evaluate the merit line search (off for the alphas = [1] no-op)
   double ls_eta;
   int n_alphas;
   double alphas[kMaxAlphas];

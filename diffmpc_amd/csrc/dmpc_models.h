@@ -2,7 +2,10 @@
 //
 // Every model is a struct with NX, NU and a templated
 //   f(const T* x, const T* u, const float* prm, T* xdot)
-// where T is float or Dual (forward-mode derivative with one tangent).  The
+// where T is float or Dual (forward-mode derivative with one tangent, for
+// the linearization) or double (value only, for the line search's merit
+// evaluations).  Constants derived from the parameters are computed in the
+// scalar type S of T (float for float and Dual, double for double).  The
 // flat parameter layouts are the ones documented in include/dmpc.h.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -34,20 +37,37 @@ __device__ __forceinline__ Dual dsqrt(Dual a) {
   return {s, a.d * (0.5f / s)};
 }
 __device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
+__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ Dual dsin(Dual a) { return {sinf(a.v), a.d * cosf(a.v)}; }
 __device__ __forceinline__ Dual dcos(Dual a) { return {cosf(a.v), -a.d * sinf(a.v)}; }
 __device__ __forceinline__ float dsin(float a) { return sinf(a); }
 __device__ __forceinline__ float dcos(float a) { return cosf(a); }
+__device__ __forceinline__ double dsin(double a) { return sin(a); }
+__device__ __forceinline__ double dcos(double a) { return cos(a); }
 __device__ __forceinline__ float val(float a) { return a; }
 __device__ __forceinline__ float val(Dual a) { return a.v; }
 __device__ __forceinline__ float tan_(Dual a) { return a.d; }
 
+// scalar type of T
 template <class T>
-__device__ __forceinline__ T sc(float v);
+struct ScalarOf {
+  using type = float;
+};
+template <>
+struct ScalarOf<double> {
+  using type = double;
+};
+template <class T>
+using scalar_t = typename ScalarOf<T>::type;
+
+template <class T>
+__device__ __forceinline__ T sc(scalar_t<T> v);
 template <>
 __device__ __forceinline__ float sc<float>(float v) { return v; }
 template <>
 __device__ __forceinline__ Dual sc<Dual>(float v) { return Dual{v, 0.0f}; }
+template <>
+__device__ __forceinline__ double sc<double>(double v) { return v; }
 
 enum ModelId : int {
   MODEL_LINEAR = 0,
@@ -84,30 +104,33 @@ struct QuadrotorModel {
   static constexpr int NX = 13, NU = 4, NPARAM = 10;
   template <class T>
   __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
-    const float m = p[0];
-    const float* J = p + 1;
-    // inverse of J by the adjugate (J is constant: plain floats)
-    float c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
-    float det = J[0] * c00 + J[1] * c01 + J[2] * c02;
-    float id = 1.0f / det;
-    float Ji[9] = {c00 * id, (J[2] * J[7] - J[1] * J[8]) * id, (J[1] * J[5] - J[2] * J[4]) * id,
-                   c01 * id, (J[0] * J[8] - J[2] * J[6]) * id, (J[2] * J[3] - J[0] * J[5]) * id,
-                   c02 * id, (J[1] * J[6] - J[0] * J[7]) * id, (J[0] * J[4] - J[1] * J[3]) * id};
+    using S = scalar_t<T>;
+    const S m = p[0];
+    S J[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) J[i] = p[1 + i];
+    // inverse of J by the adjugate (J is constant: plain scalars)
+    S c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
+    S det = J[0] * c00 + J[1] * c01 + J[2] * c02;
+    S id = S(1) / det;
+    S Ji[9] = {c00 * id, (J[2] * J[7] - J[1] * J[8]) * id, (J[1] * J[5] - J[2] * J[4]) * id,
+               c01 * id, (J[0] * J[8] - J[2] * J[6]) * id, (J[2] * J[3] - J[0] * J[5]) * id,
+               c02 * id, (J[1] * J[6] - J[0] * J[7]) * id, (J[0] * J[4] - J[1] * J[3]) * id};
     T qw = x[6], qx = x[7], qy = x[8], qz = x[9];
-    T inv = sc<T>(1.0f) / dsqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+    T inv = sc<T>(S(1)) / dsqrt(qw * qw + qx * qx + qy * qy + qz * qz);
     qw = qw * inv; qx = qx * inv; qy = qy * inv; qz = qz * inv;
     T wx = x[10], wy = x[11], wz = x[12];
-    T fm = u[0] * (1.0f / m);
+    T fm = u[0] * (S(1) / m);
     xd[0] = x[3]; xd[1] = x[4]; xd[2] = x[5];
     // v' = -g e3 + (f/m) R(q) e3
-    xd[3] = fm * (2.0f * (qx * qz + qw * qy));
-    xd[4] = fm * (2.0f * (qy * qz - qw * qx));
-    xd[5] = fm * (qw * qw - qx * qx - qy * qy + qz * qz) - 9.81f;
+    xd[3] = fm * (S(2) * (qx * qz + qw * qy));
+    xd[4] = fm * (S(2) * (qy * qz - qw * qx));
+    xd[5] = fm * (qw * qw - qx * qx - qy * qy + qz * qz) - S(9.81);
     // q' = 1/2 L(q) [0; w]
-    xd[6] = 0.5f * (-(qx * wx) - qy * wy - qz * wz);
-    xd[7] = 0.5f * (qw * wx - qz * wy + qy * wz);
-    xd[8] = 0.5f * (qz * wx + qw * wy - qx * wz);
-    xd[9] = 0.5f * (qx * wy - qy * wx + qw * wz);
+    xd[6] = S(0.5) * (-(qx * wx) - qy * wy - qz * wz);
+    xd[7] = S(0.5) * (qw * wx - qz * wy + qy * wz);
+    xd[8] = S(0.5) * (qz * wx + qw * wy - qx * wz);
+    xd[9] = S(0.5) * (qx * wy - qy * wx + qw * wz);
     // w' = J^-1 ((J w) x w + tau)
     T hx = J[0] * wx + J[1] * wy + J[2] * wz;
     T hy = J[3] * wx + J[4] * wy + J[5] * wz;
@@ -127,10 +150,12 @@ struct Spacecraft3Model {
   static constexpr int NX = 3, NU = 3, NPARAM = 3;
   template <class T>
   __device__ static void f(const T* w, const T* u, const float* p, T* xd) {
-    T hx = p[0] * w[0], hy = p[1] * w[1], hz = p[2] * w[2];
-    xd[0] = (u[0] - (w[1] * hz - w[2] * hy)) * (1.0f / p[0]);
-    xd[1] = (u[1] - (w[2] * hx - w[0] * hz)) * (1.0f / p[1]);
-    xd[2] = (u[2] - (w[0] * hy - w[1] * hx)) * (1.0f / p[2]);
+    using S = scalar_t<T>;
+    const S J0 = p[0], J1 = p[1], J2 = p[2];
+    T hx = J0 * w[0], hy = J1 * w[1], hz = J2 * w[2];
+    xd[0] = (u[0] - (w[1] * hz - w[2] * hy)) * (S(1) / J0);
+    xd[1] = (u[1] - (w[2] * hx - w[0] * hz)) * (S(1) / J1);
+    xd[2] = (u[2] - (w[0] * hy - w[1] * hx)) * (S(1) / J2);
   }
 };
 
@@ -141,18 +166,20 @@ struct Spacecraft7Model {
   static constexpr int NX = 7, NU = 3, NPARAM = 3;
   template <class T>
   __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+    using S = scalar_t<T>;
     T qw = x[0], qx = x[1], qy = x[2], qz = x[3];
-    T inv = sc<T>(1.0f) / dsqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+    T inv = sc<T>(S(1)) / dsqrt(qw * qw + qx * qx + qy * qy + qz * qz);
     qw = qw * inv; qx = qx * inv; qy = qy * inv; qz = qz * inv;
     T wx = x[4], wy = x[5], wz = x[6];
-    xd[0] = 0.5f * (-(qx * wx) - qy * wy - qz * wz);
-    xd[1] = 0.5f * (qw * wx - qz * wy + qy * wz);
-    xd[2] = 0.5f * (qz * wx + qw * wy - qx * wz);
-    xd[3] = 0.5f * (qx * wy - qy * wx + qw * wz);
-    T hx = p[0] * wx, hy = p[1] * wy, hz = p[2] * wz;
-    xd[4] = (u[0] - (wy * hz - wz * hy)) * (1.0f / p[0]);
-    xd[5] = (u[1] - (wz * hx - wx * hz)) * (1.0f / p[1]);
-    xd[6] = (u[2] - (wx * hy - wy * hx)) * (1.0f / p[2]);
+    xd[0] = S(0.5) * (-(qx * wx) - qy * wy - qz * wz);
+    xd[1] = S(0.5) * (qw * wx - qz * wy + qy * wz);
+    xd[2] = S(0.5) * (qz * wx + qw * wy - qx * wz);
+    xd[3] = S(0.5) * (qx * wy - qy * wx + qw * wz);
+    const S J0 = p[0], J1 = p[1], J2 = p[2];
+    T hx = J0 * wx, hy = J1 * wy, hz = J2 * wz;
+    xd[4] = (u[0] - (wy * hz - wz * hy)) * (S(1) / J0);
+    xd[5] = (u[1] - (wz * hx - wx * hz)) * (S(1) / J1);
+    xd[6] = (u[2] - (wx * hy - wy * hx)) * (S(1) / J2);
   }
 };
 
@@ -162,11 +189,12 @@ struct CartpoleModel {
   static constexpr int NX = 4, NU = 1, NPARAM = 4;
   template <class T>
   __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
-    const float mc = p[0], mp = p[1], len = p[2], g = p[3];
-    const float total = mc + mp, pml = mp * len;
+    using S = scalar_t<T>;
+    const S mc = p[0], mp = p[1], len = p[2], g = p[3];
+    const S total = mc + mp, pml = mp * len;
     T s = dsin(x[2]), c = dcos(x[2]);
-    T temp = (u[0] + pml * (x[3] * x[3]) * s) * (1.0f / total);
-    T th_acc = (g * s - c * temp) / (len * (sc<T>(4.0f / 3.0f) - (mp / total) * (c * c)));
+    T temp = (u[0] + pml * (x[3] * x[3]) * s) * (S(1) / total);
+    T th_acc = (g * s - c * temp) / (len * (sc<T>(S(4) / S(3)) - (mp / total) * (c * c)));
     T x_acc = temp - (pml / total) * (th_acc * c);
     xd[0] = x[1];
     xd[1] = x_acc;
@@ -189,14 +217,15 @@ struct ChainModel {
     for (int j = 0; j < M; ++j) {
 #pragma unroll
       for (int d = 0; d < 2; ++d) {
-        T dp = x[2 * j + d] - (j > 0 ? x[2 * (j - 1) + d] : sc<T>(0.0f));
-        T dv = x[2 * M + 2 * j + d] - (j > 0 ? x[2 * M + 2 * (j - 1) + d] : sc<T>(0.0f));
-        T fj = -(p[j] * dp) - p[M + j] * dv;
+        T dp = x[2 * j + d] - (j > 0 ? x[2 * (j - 1) + d] : sc<T>(0));
+        T dv = x[2 * M + 2 * j + d] - (j > 0 ? x[2 * M + 2 * (j - 1) + d] : sc<T>(0));
+        using S = scalar_t<T>;
+        T fj = -(S(p[j]) * dp) - S(p[M + j]) * dv;
         T acc = fj;
         if (j + 1 < M) {
           T dp1 = x[2 * (j + 1) + d] - x[2 * j + d];
           T dv1 = x[2 * M + 2 * (j + 1) + d] - x[2 * M + 2 * j + d];
-          acc = acc + p[j + 1] * dp1 + p[M + j + 1] * dv1;
+          acc = acc + S(p[j + 1]) * dp1 + S(p[M + j + 1]) * dv1;
         }
         if (d == 0) acc = acc + u[j];
         xd[2 * M + 2 * j + d] = acc;
@@ -209,7 +238,8 @@ struct ChainModel {
 // One integration step (diffmpc/dynamics/integrators.py:20-71):
 // scheme 0 = Euler, 1 = midpoint, 2 = RK4.
 template <class Model, class T>
-__device__ void step(const T* x, const T* u, const float* p, float dt, int scheme, T* out) {
+__device__ void step(const T* x, const T* u, const float* p, scalar_t<T> dt, int scheme, T* out) {
+  using S = scalar_t<T>;
   constexpr int NX = Model::NX;
   T k1[NX], xt[NX];
   Model::f(x, u, p, k1);
@@ -220,7 +250,7 @@ __device__ void step(const T* x, const T* u, const float* p, float dt, int schem
   }
   if (scheme == 1) {
 #pragma unroll
-    for (int i = 0; i < NX; ++i) xt[i] = x[i] + (0.5f * dt) * k1[i];
+    for (int i = 0; i < NX; ++i) xt[i] = x[i] + (S(0.5) * dt) * k1[i];
     T k2[NX];
     Model::f(xt, u, p, k2);
 #pragma unroll
@@ -231,24 +261,24 @@ __device__ void step(const T* x, const T* u, const float* p, float dt, int schem
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
     acc[i] = k1[i];
-    xt[i] = x[i] + (0.5f * dt) * k1[i];
+    xt[i] = x[i] + (S(0.5) * dt) * k1[i];
   }
   T k[NX];
   Model::f(xt, u, p, k);
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
-    acc[i] = acc[i] + 2.0f * k[i];
-    xt[i] = x[i] + (0.5f * dt) * k[i];
+    acc[i] = acc[i] + S(2) * k[i];
+    xt[i] = x[i] + (S(0.5) * dt) * k[i];
   }
   Model::f(xt, u, p, k);
 #pragma unroll
   for (int i = 0; i < NX; ++i) {
-    acc[i] = acc[i] + 2.0f * k[i];
+    acc[i] = acc[i] + S(2) * k[i];
     xt[i] = x[i] + dt * k[i];
   }
   Model::f(xt, u, p, k);
 #pragma unroll
-  for (int i = 0; i < NX; ++i) out[i] = x[i] + (acc[i] + k[i]) * (dt / 6.0f);
+  for (int i = 0; i < NX; ++i) out[i] = x[i] + (acc[i] + k[i]) * (dt / S(6));
 }
 
 // Value and column j of the Jacobian d f_d / d[x; u] at (x, u) in float.

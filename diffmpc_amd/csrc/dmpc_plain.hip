@@ -269,7 +269,6 @@ __global__ void gamma_kernel(int B, int N, int nx, int nu, const double* AQ, con
 __global__ void matvec_kernel(int B, int N, int nx, const double* M, const double* v, double* y, const int* active) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)B * (N + 1) * nx) return;
-  const int i = idx % nx;
   const int t = (idx / nx) % (N + 1);
   const int b = idx / ((long long)nx * (N + 1));
   if (active && !active[b]) return;
@@ -405,10 +404,126 @@ __global__ void recover_kernel(int B, int N, int nx, int nu, const double* lam, 
 
 // per-instance SQP step bookkeeping (sqp.py:506-528): err = max|dx|,|du|;
 // accept the new iterate for active instances
+// Merit line search of sqp.py:301-411, per-launch form.  ls_items_kernel:
+// one thread per (instance, candidate kc, knot t); kc = 0 is the current
+// iterate, kc >= 1 the step alphas[kc - 1].  It writes the cost of knot t and
+// the l1 norm of constraint block t (g_0 = x_0 - x_init,
+// g_t = f_d(x_{t-1}, u_{t-1}) - x_t), dynamics evaluated in float64.
+template <class Model>
+__global__ void ls_items_kernel(SolveArgs a, const double* xn, const double* un, double* lsC, double* lsG,
+                                const int* sqp_active) {
+  constexpr int NX = Model::NX, NU = Model::NU;
+  const int N = a.N, NB = N + 1, nal = a.n_alphas;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)a.B * (nal + 1) * NB) return;
+  const int b = (int)(idx / ((nal + 1) * NB));
+  const int rem = (int)(idx - (long long)b * (nal + 1) * NB);
+  const int kc = rem / NB, t = rem - kc * NB;
+  if (!sqp_active[b]) return;
+  const double al = kc == 0 ? 0.0 : a.alphas[kc - 1];
+  const double* x = a.x + (size_t)b * NB * NX;
+  const double* u = a.u + (size_t)b * NB * NU;
+  const double* xq = xn + (size_t)b * NB * NX;
+  const double* uq = un + (size_t)b * NB * NU;
+  const double* wx = a.wx + (size_t)b * NX;
+  const double* wu = a.wu + (size_t)b * NU;
+  const double* wf = a.wf + (size_t)b * NX;
+  double c = 0.0, g = 0.0;
+  if (t < N) {
+    for (int i = 0; i < NX; ++i) {
+      const double d = x[t * NX + i] + al * (xq[t * NX + i] - x[t * NX + i]) - a.xref[((size_t)b * NB + t) * NX + i];
+      c += wx[i] * d * d;
+    }
+    for (int i = 0; i < NU; ++i) {
+      const double ref = a.penalize ? a.uref[((size_t)b * NB + t) * NU + i] : 0.0;
+      const double d = u[t * NU + i] + al * (uq[t * NU + i] - u[t * NU + i]) - ref;
+      c += wu[i] * d * d;
+    }
+  } else {
+    for (int i = 0; i < NX; ++i) {
+      const double d = x[t * NX + i] + al * (xq[t * NX + i] - x[t * NX + i]) - a.xf[(size_t)b * NX + i];
+      c += (wx[i] + wf[i]) * d * d;
+    }
+  }
+  if (t == 0) {
+    for (int i = 0; i < NX; ++i) g += fabs(x[i] + al * (xq[i] - x[i]) - a.x0[(size_t)b * NX + i]);
+  } else {
+    double xp[NX], up[NU], fx[NX];
+    for (int i = 0; i < NX; ++i) xp[i] = x[(t - 1) * NX + i] + al * (xq[(t - 1) * NX + i] - x[(t - 1) * NX + i]);
+    for (int i = 0; i < NU; ++i) up[i] = u[(t - 1) * NU + i] + al * (uq[(t - 1) * NU + i] - u[(t - 1) * NU + i]);
+    step<Model, double>(xp, up, a.dyn + (size_t)b * a.P, a.dt64, a.scheme, fx);
+    for (int i = 0; i < NX; ++i) g += fabs(fx[i] - (x[t * NX + i] + al * (xq[t * NX + i] - x[t * NX + i])));
+  }
+  lsC[(size_t)b * (kMaxAlphas + 1) * NB + rem] = c;
+  lsG[(size_t)b * (kMaxAlphas + 1) * NB + rem] = g;
+}
+
+// One block per instance: mu, merit, decreases and the reference's clamped
+// scan over the candidates (thread 0, fixed summation order), then the step
+// x + alpha (x_qp - x) is applied to the QP solution in place.
+__global__ void ls_select_kernel(SolveArgs a, int nx, int nu, double* xn, double* un, const double* lsC,
+                                 const double* lsG, double* alpha_out, const int* sqp_active) {
+  __shared__ double alpha_s;
+  const int b = blockIdx.x;
+  if (!sqp_active[b]) return;
+  const int N = a.N, NB = N + 1, nal = a.n_alphas;
+  const double* x = a.x + (size_t)b * NB * nx;
+  const double* u = a.u + (size_t)b * NB * nu;
+  double* xq = xn + (size_t)b * NB * nx;
+  double* uq = un + (size_t)b * NB * nu;
+  if (threadIdx.x == 0) {
+    double dcdz = 0.0;
+    for (int k = 0; k < NB * nx; ++k) {
+      const int t = k / nx, i = k - t * nx;
+      const double wxi = a.wx[(size_t)b * nx + i];
+      const double gx = (t < N) ? 2.0 * wxi * (x[k] - a.xref[(size_t)b * NB * nx + k])
+                                : 2.0 * (wxi + a.wf[(size_t)b * nx + i]) * (x[k] - a.xf[(size_t)b * nx + i]);
+      dcdz += gx * (xq[k] - x[k]);
+    }
+    for (int k = 0; k < N * nu; ++k) {
+      const int j = k % nu;
+      const double ref = a.penalize ? a.uref[(size_t)b * NB * nu + k] : 0.0;
+      dcdz += 2.0 * a.wu[(size_t)b * nu + j] * (u[k] - ref) * (uq[k] - u[k]);
+    }
+    double sc[kMaxAlphas + 1], sg[kMaxAlphas + 1];
+    for (int kc = 0; kc <= nal; ++kc) {
+      double c = 0.0, g = 0.0;
+      for (int t = 0; t < NB; ++t) {
+        c += lsC[(size_t)b * (kMaxAlphas + 1) * NB + kc * NB + t];
+        g += lsG[(size_t)b * (kMaxAlphas + 1) * NB + kc * NB + t];
+      }
+      sc[kc] = c;
+      sg[kc] = g;
+    }
+    const double mu_min = dcdz / (0.5 * sg[0]);
+    const double mu = (mu_min != mu_min) ? mu_min : fmax(0.1, mu_min);
+    const double merit = sc[0] + mu * sg[0];
+    const double dmerit = dcdz + -mu * sg[0];
+    double amax = a.alphas[0];
+    for (int k = 1; k < nal; ++k) amax = (a.alphas[k] != a.alphas[k] || amax != amax) ? NAN : fmax(amax, a.alphas[k]);
+    int i = nal + 1;
+    double alpha = amax;
+    while (true) {
+      const int ic = i < nal - 1 ? i : nal - 1;
+      const double dec = (sc[ic + 1] + mu * sg[ic + 1]) - (merit + a.ls_eta * a.alphas[ic] * dmerit);
+      if (dec < 0.0 || i == 0) break;
+      alpha = a.alphas[(i - 1) < nal - 1 ? (i - 1) : nal - 1];
+      --i;
+    }
+    alpha_s = alpha;
+    alpha_out[b] = alpha;
+  }
+  __syncthreads();
+  const double alpha = alpha_s;
+  for (int k = threadIdx.x; k < NB * nx; k += blockDim.x) xq[k] = x[k] + alpha * (xq[k] - x[k]);
+  for (int k = threadIdx.x; k < NB * nu; k += blockDim.x) uq[k] = u[k] + alpha * (uq[k] - u[k]);
+}
+
 __global__ void sqp_accept_kernel(int B, int N, int nx, int nu, int maxit, int sqp_it, const int* sqp_active,
                                   double* x, double* u, double* lam, const double* xn, const double* un,
                                   const double* lamn, double* conv_err, double* conv_errs, int32_t* pcg_hist,
-                                  const int* pcg_it, double* ls_hist, int linesearch, int32_t* num_iter) {
+                                  const int* pcg_it, double* ls_hist, int linesearch, int ls_eval,
+                                  const double* alpha, int32_t* num_iter) {
   __shared__ double red[256];
   const int b = blockIdx.x;
   if (!sqp_active[b]) return;
@@ -441,7 +556,7 @@ __global__ void sqp_accept_kernel(int B, int N, int nx, int nu, int maxit, int s
     conv_err[b] = err;
     conv_errs[(size_t)b * maxit + sqp_it] = err;
     pcg_hist[(size_t)b * maxit + sqp_it] = pcg_it[b];
-    if (linesearch) ls_hist[(size_t)b * maxit + sqp_it] = 1.0;
+    if (linesearch) ls_hist[(size_t)b * maxit + sqp_it] = ls_eval ? alpha[b] : 1.0;
     num_iter[b] = sqp_it + 1;
   }
 }
@@ -634,6 +749,7 @@ struct Carver {
 struct QpBufs {
   double *As_next, *As, *Bs, *Cs, *fd, *Qinv, *qv, *Rinv, *rv, *AQ, *AnQ, *BR, *theta, *phi, *thinv, *tmp, *M2, *S, *Phi;
   double *gamma, *lam, *r, *rt, *p, *ups, *eta, *etan, *v, *xn, *un;
+  double *lsC, *lsG, *alpha;  // line search: merit parts per (instance, candidate, knot), chosen step
   int *active, *iters, *any, *sqp_active;
 };
 
@@ -651,6 +767,7 @@ size_t qp_bytes(int B, int N, int nx, int nu) {
   d += 2 * (size_t)B * NB * nx * 3 * nx;       // S, Phi
   d += 6 * (size_t)B * NB * nx;                // gamma lam r rt p ups
   d += 3 * (size_t)B + 2 * (size_t)B * NB * nx + (size_t)B * NB * nu;  // eta etan v xn un(+)
+  d += 2 * (size_t)B * (kMaxAlphas + 1) * NB + (size_t)B;  // lsC lsG alpha
   return d * 8 + 4 * (size_t)B * 4 + 64 * 256;
 }
 
@@ -688,6 +805,9 @@ QpBufs carve_qp(void* base, int B, int N, int nx, int nu) {
   q.v = c.take<double>(B);
   q.xn = c.take<double>((size_t)B * NB * nx);
   q.un = c.take<double>((size_t)B * NB * nu);
+  q.lsC = c.take<double>((size_t)B * (kMaxAlphas + 1) * NB);
+  q.lsG = c.take<double>((size_t)B * (kMaxAlphas + 1) * NB);
+  q.alpha = c.take<double>(B);
   q.active = c.take<int>(B);
   q.iters = c.take<int>(B);
   q.sqp_active = c.take<int>(B);
@@ -782,11 +902,18 @@ int plain_forward_t(const SolveArgs& a, hipStream_t st) {
     if (rc) return rc;
     hipLaunchKernelGGL(recover_kernel, dim3(nblk(nst + (long long)B * (N + 1) * nu)), dim3(256), 0, st, B, N, nx, nu,
                        q.lam, q.As_next, q.As, q.Bs, q.Qinv, q.qv, q.Rinv, q.rv, q.xn, q.un);
+    if (a.ls_eval) {
+      const long long nit = (long long)B * (a.n_alphas + 1) * (N + 1);
+      hipLaunchKernelGGL(ls_items_kernel<Model>, dim3(nblk(nit)), dim3(256), 0, st, a, q.xn, q.un, q.lsC, q.lsG,
+                         q.sqp_active);
+      hipLaunchKernelGGL(ls_select_kernel, dim3(B), dim3(256), 0, st, a, nx, nu, q.xn, q.un, q.lsC, q.lsG, q.alpha,
+                         q.sqp_active);
+    }
     hipLaunchKernelGGL(save_kernel, dim3(nblk((long long)B * N * nx * (nx + nu) + nst)), dim3(256), 0, st, B, N, nx,
                        nu, q.As, q.Bs, q.Qinv, q.Rinv, q.fd, q.sqp_active, a.AB, a.fd, a.qinv, a.rinv);
     hipLaunchKernelGGL(sqp_accept_kernel, dim3(B), dim3(256), 0, st, B, N, nx, nu, a.maxit, sqp_it, q.sqp_active, a.x,
                        a.u, a.lam, q.xn, q.un, q.lam, a.conv_err, a.conv_errs, a.pcg_iters, q.iters, a.ls_alphas,
-                       a.linesearch, a.num_iter);
+                       a.linesearch, a.ls_eval, q.alpha, a.num_iter);
     HIPCHK(hipGetLastError());
     ++sqp_it;
   }

@@ -74,6 +74,43 @@ def test_sqp_solve_matches_oracle(cuda, case, mode):
     assert np.allclose(ce, cr, rtol=1e-3, atol=1e-5 * np.max(np.abs(ref["states"])))
 
 
+LINESEARCH_CASES = {
+    # instances and iteration counts whose merit decisions have a margin of at
+    # least 1e-5 relative in the oracle (closer calls are not pinned by a
+    # float32 linearization); candidates 0.01 .. 1, eta 0.4 (sqp.yaml)
+    "quadrotor": (lambda: W.quadrotor(batch=4, seed=1, horizon=19), [0, 1, 3], 4),
+    "cartpole": (lambda: W.cartpole(batch=3, seed=6), [0, 1, 2], 4),
+    "spacecraft7": (lambda: W.spacecraft7(batch=3, seed=2, horizon=30), [0, 1, 2], 3),
+}
+
+
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+@pytest.mark.parametrize("case", list(LINESEARCH_CASES))
+def test_linesearch_matches_oracle(cuda, case, mode):
+    """Merit line search (sqp.py:301-411): same step sizes as the oracle, and
+    the trajectories after the damped steps."""
+    say(f"[linesearch] case={case} mode={mode}")
+    gen, keep, maxit = LINESEARCH_CASES[case]
+    name, prob, sp = gen()
+    prob = W.slice_problem(prob, np.array(keep))
+    sp = dict(sp)
+    sp.update(linesearch=True, num_scp_iteration_max=maxit, tol_convergence=1e-8)
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda, mode=mode)
+    sol = solver.solve(guess, params)
+    torch.cuda.synchronize()
+    ref = osqp.sqp_solve(oracle_problem(name, prob), sp)
+    say("  alphas gpu", _np(sol.linesearch_alphas).tolist())
+    say("  alphas ref", ref["linesearch_alphas"].tolist())
+    say("  rel states", rel(_np(sol.states), ref["states"]), "controls", rel(_np(sol.controls), ref["controls"]))
+    assert np.array_equal(_np(sol.linesearch_alphas), ref["linesearch_alphas"])
+    assert np.min(ref["linesearch_alphas"]) < 1.0  # the search did reject full steps
+    assert rel(_np(sol.states), ref["states"]) <= 1e-5
+    assert rel(_np(sol.controls), ref["controls"]) <= 1e-5
+    assert rel(_np(sol.dual), ref["dual"]) <= 1e-5
+    assert np.array_equal(_np(sol.num_iter), ref["num_iter"])
+    assert np.array_equal(_np(sol.status), ref["status"])
+
+
 @pytest.mark.parametrize("blocks", [1, 2, 4])
 @pytest.mark.parametrize("case", ["lti", "quadrotor", "spacecraft7", "cartpole"])
 def test_every_blocks_per_group_variant_matches_oracle(cuda, case, blocks):
