@@ -269,7 +269,7 @@ int dmpc_pcg_gamma(int32_t B, int32_t N, int32_t nx, int32_t nu, const double* A
 
 int dmpc_pcg_solve(int32_t B, int32_t N, int32_t nx, const double* S, const double* Phiinv, const double* gamma,
                    double* lam, int32_t max_iter, double tol_epsilon, int32_t* iters, double* eta, void* stream) {
-  if (B < 1 || N < 1 || nx < 1 || max_iter < 1) return DMPC_ERR_ARG;
+  if (B < 1 || N < 1 || nx < 1 || nx > 64 || max_iter < 1) return DMPC_ERR_ARG;
   if (!S || !Phiinv || !gamma || !lam || !iters || !eta) return DMPC_ERR_ARG;
   return pcg_layer_solve(B, N, nx, S, Phiinv, gamma, lam, max_iter, tol_epsilon, iters, eta,
                          reinterpret_cast<hipStream_t>(stream));

@@ -265,22 +265,38 @@ __global__ void gamma_kernel(int B, int N, int nx, int nu, const double* AQ, con
   gamma[idx] = g;
 }
 
-// y_t = M_t [v_{t-1}; v_t; v_{t+1}] (pcg.py:370-392, 405-428)
+// y_t = M_t [v_{t-1}; v_t; v_{t+1}] (pcg.py:370-392, 405-428).
+// One workgroup per block row (b, t): the 3nx-entry vector window is staged
+// in LDS, each row of the block is read by 16 consecutive lanes (128-byte
+// coalesced segments) and reduced with shuffles.  Columns outside 0..N are
+// skipped.  Launch with matvec_threads(nx) threads, B*(N+1) workgroups.
+constexpr int kMatvecMaxNx = 64;
+inline int matvec_threads(int nx) { return nx * 16 >= 256 ? 256 : ((nx * 16 + 63) / 64) * 64; }
+
 __global__ void matvec_kernel(int B, int N, int nx, const double* M, const double* v, double* y, const int* active) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)B * (N + 1) * nx) return;
-  const int t = (idx / nx) % (N + 1);
-  const int b = idx / ((long long)nx * (N + 1));
+  __shared__ double vs[3 * kMatvecMaxNx];
+  const long long bt = blockIdx.x;
+  const int b = (int)(bt / (N + 1)), t = (int)(bt - (long long)b * (N + 1));
   if (active && !active[b]) return;
-  const double* row = M + idx * 3 * nx;
+  const int w = 3 * nx;
   const double* vb = v + (size_t)b * (N + 1) * nx;
-  double acc = 0.0;
-  for (int c = 0; c < 3 * nx; ++c) {
+  for (int c = threadIdx.x; c < w; c += blockDim.x) {
     const int tt = t - 1 + c / nx;
-    if (tt < 0 || tt > N) continue;
-    acc += row[c] * vb[tt * nx + c % nx];
+    vs[c] = (tt >= 0 && tt <= N) ? vb[tt * nx + c % nx] : 0.0;
   }
-  y[idx] = acc;
+  __syncthreads();
+  const int c_lo = t == 0 ? nx : 0, c_hi = t == N ? 2 * nx : w;
+  const int g = threadIdx.x >> 4, j = threadIdx.x & 15, ng = blockDim.x >> 4;
+  for (int i = g; i < nx; i += ng) {
+    const double* row = M + ((size_t)bt * nx + i) * w;
+    double acc = 0.0;
+    for (int c = c_lo + j; c < c_hi; c += 16) acc += row[c] * vs[c];
+    acc += __shfl_xor(acc, 8, 16);
+    acc += __shfl_xor(acc, 4, 16);
+    acc += __shfl_xor(acc, 2, 16);
+    acc += __shfl_xor(acc, 1, 16);
+    if (j == 0) y[(size_t)bt * nx + i] = acc;
+  }
 }
 
 // per-instance dot product, one workgroup per instance, fixed order
@@ -838,9 +854,9 @@ int assemble(const QpBufs& q, int B, int N, int nx, int nu, hipStream_t st) {
 int pcg_loop(const QpBufs& q, int B, int N, int nx, int max_iter, double eps, hipStream_t st) {
   const int n = (N + 1) * nx;
   const long long nn = (long long)B * n;
-  hipLaunchKernelGGL(matvec_kernel, dim3(nblk(nn)), dim3(256), 0, st, B, N, nx, q.S, q.lam, q.ups, nullptr);
+  hipLaunchKernelGGL(matvec_kernel, dim3((unsigned)((long long)B * (N + 1))), dim3(matvec_threads(nx)), 0, st, B, N, nx, q.S, q.lam, q.ups, nullptr);
   hipLaunchKernelGGL(sub_kernel, dim3(nblk(nn)), dim3(256), 0, st, nn, q.gamma, q.ups, q.r);
-  hipLaunchKernelGGL(matvec_kernel, dim3(nblk(nn)), dim3(256), 0, st, B, N, nx, q.Phi, q.r, q.rt, nullptr);
+  hipLaunchKernelGGL(matvec_kernel, dim3((unsigned)((long long)B * (N + 1))), dim3(matvec_threads(nx)), 0, st, B, N, nx, q.Phi, q.r, q.rt, nullptr);
   hipLaunchKernelGGL(copy_kernel, dim3(nblk(nn)), dim3(256), 0, st, nn, q.rt, q.p);
   hipLaunchKernelGGL(dot_kernel, dim3(B), dim3(256), 0, st, n, q.r, q.rt, q.eta, nullptr);
   HIPCHK(hipMemsetAsync(q.iters, 0, sizeof(int) * B, st));
@@ -852,11 +868,11 @@ int pcg_loop(const QpBufs& q, int B, int N, int nx, int max_iter, double eps, hi
     HIPCHK(hipMemcpyAsync(&host_any, q.any, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (!host_any) break;
-    hipLaunchKernelGGL(matvec_kernel, dim3(nblk(nn)), dim3(256), 0, st, B, N, nx, q.S, q.p, q.ups, q.active);
+    hipLaunchKernelGGL(matvec_kernel, dim3((unsigned)((long long)B * (N + 1))), dim3(matvec_threads(nx)), 0, st, B, N, nx, q.S, q.p, q.ups, q.active);
     hipLaunchKernelGGL(dot_kernel, dim3(B), dim3(256), 0, st, n, q.p, q.ups, q.v, q.active);
     hipLaunchKernelGGL(pcg_step_kernel, dim3(nblk(nn)), dim3(256), 0, st, B, n, q.eta, q.v, q.p, q.ups, q.lam, q.r,
                        q.active);
-    hipLaunchKernelGGL(matvec_kernel, dim3(nblk(nn)), dim3(256), 0, st, B, N, nx, q.Phi, q.r, q.rt, q.active);
+    hipLaunchKernelGGL(matvec_kernel, dim3((unsigned)((long long)B * (N + 1))), dim3(matvec_threads(nx)), 0, st, B, N, nx, q.Phi, q.r, q.rt, q.active);
     hipLaunchKernelGGL(dot_kernel, dim3(B), dim3(256), 0, st, n, q.r, q.rt, q.etan, q.active);
     hipLaunchKernelGGL(pcg_dir_kernel, dim3(nblk(nn)), dim3(256), 0, st, B, n, q.etan, q.eta, q.rt, q.p, q.active);
     hipLaunchKernelGGL(pcg_scalar_kernel, dim3(nblk(B)), dim3(256), 0, st, B, q.eta, q.etan, q.iters, q.active);

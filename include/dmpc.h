@@ -128,7 +128,7 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
 int dmpc_linearize(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,
                    const double* controls, double* fd, double* A, double* Bm, double* C, void* stream);
 
-/* PCGOptimalControl layer, dense general blocks, float64.
+/* PCGOptimalControl layer, dense general blocks, float64, 1 <= nx <= 64.
  * As_next, As (B,N,nx,nx); Bs (B,N,nx,nu); Qinv (B,N+1,nx,nx); Rinv (B,N,nu,nu);
  * S, Phiinv (B,N+1,nx,3nx). */
 int dmpc_pcg_assemble(int32_t B, int32_t N, int32_t nx, int32_t nu, const double* As_next, const double* As,

@@ -303,3 +303,69 @@ def test_notebook_spacecraft_rollout_and_gradient(cuda):
             say("  w after step 1", wx.tolist(), wu.tolist())
             assert np.allclose(_np(wx), ref["w_x_after_step1"], atol=3e-5, rtol=0)
             assert np.allclose(_np(wu), ref["w_u_after_step1"], atol=3e-5, rtol=0)
+
+
+def _sampled_oracle(name, prob, sp, idx, loss_grad):
+    """Oracle forward + backward on the instances `idx` of a full batch."""
+    sub = oracle_problem(name, W.slice_problem(prob, np.array(idx)))
+    ref = osqp.sqp_solve(sub, sp)
+    dl_dx, dl_du = loss_grad(sub, ref)
+    return sub, ref, osqp.sqp_backward(sub, sp, ref, dl_dx, dl_du)
+
+
+def test_headline_workload_full_batch_matches_oracle_on_sampled_instances(cuda):
+    """Quadrotor nx=13 nu=4 N=50, B=1024, 5 SQP iterations, PCG 1e-6 (the
+    benchmark workload) solved and differentiated as one batch; instances are
+    independent, so sampled instances are checked against the oracle."""
+    say("[full] quadrotor B=1024 N=50")
+    name, prob, sp = W.quadrotor(batch=1024, seed=1)
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    keys = ["weights_penalization_reference_state_trajectory", "weights_penalization_control_squared", "initial_state"]
+    weights = {k: params[k].clone().requires_grad_(True) for k in keys}
+    sol = solver.solve(guess, params, weights)
+    loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+    grads = torch.autograd.grad(loss, [weights[k] for k in keys])
+    torch.cuda.synchronize()
+    idx = [0, 511, 1023]
+    _, ref, g = _sampled_oracle(name, prob, sp, idx,
+                                lambda p, r: (r["states"] - p["x_ref"], np.zeros_like(r["controls"])))
+    say("  rel states", rel(_np(sol.states)[idx], ref["states"]))
+    assert rel(_np(sol.states)[idx], ref["states"]) <= 1e-5
+    assert rel(_np(sol.controls)[idx], ref["controls"]) <= 1e-5
+    for k, short, gr in zip(keys, ["w_x", "w_u", "x0"], grads):
+        e = rel(_np(gr)[idx], g[short])
+        say(f"  {short}: rel {e:.2e}")
+        assert e <= 1e-4, short
+
+
+def test_spacecraft_attitude_full_batch_gradients_match_oracle_on_sampled_instances(cuda):
+    """Spacecraft attitude nx=7 nu=3 N=100, B=4096, 3 SQP iterations: gradients
+    of 0.5 sum ||x* - x_ref||^2 w.r.t. log weights (per instance here, so that
+    sampled instances can be compared), x0 and x_ref (the direct term of the
+    loss plus the one through the solve)."""
+    say("[full] spacecraft7 B=4096 N=100")
+    name, prob, sp = W.spacecraft7(batch=4096, seed=2)
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    B = prob["x0"].shape[0]
+    th_q = torch.tensor(np.broadcast_to(prob["theta_q"], (B, 7)).copy(), device=cuda, requires_grad=True)
+    th_r = torch.tensor(np.broadcast_to(prob["theta_r"], (B, 3)).copy(), device=cuda, requires_grad=True)
+    x0 = params["initial_state"].clone().requires_grad_(True)
+    xref = params["reference_state_trajectory"].clone().requires_grad_(True)
+    weights = {"weights_penalization_reference_state_trajectory": torch.exp(th_q),
+               "weights_penalization_control_squared": torch.exp(th_r),
+               "initial_state": x0, "reference_state_trajectory": xref}
+    sol = solver.solve(guess, params, weights)
+    loss = 0.5 * (sol.states - xref).square().sum()
+    g_tq, g_tr, g_x0, g_xref = torch.autograd.grad(loss, [th_q, th_r, x0, xref])
+    torch.cuda.synchronize()
+    idx = [0, 2048, 4095]
+    sub, ref, g = _sampled_oracle(name, prob, sp, idx,
+                                  lambda p, r: (r["states"] - p["x_ref"], np.zeros_like(r["controls"])))
+    expect = {"theta_q": g["w_x"] * sub["w_x"], "theta_r": g["w_u"] * sub["w_u"], "x0": g["x0"],
+              "x_ref": g["x_ref"] - (ref["states"] - sub["x_ref"])}
+    got = {"theta_q": g_tq, "theta_r": g_tr, "x0": g_x0, "x_ref": g_xref}
+    assert rel(_np(sol.states)[idx], ref["states"]) <= 1e-5
+    for k in expect:
+        e = rel(_np(got[k])[idx], expect[k])
+        say(f"  {k}: rel {e:.2e}")
+        assert e <= 1e-4, k

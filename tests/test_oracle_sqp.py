@@ -133,3 +133,114 @@ def test_notebook_spacecraft_training_trace():
         if step == 0:
             assert np.allclose(wx, ref["w_x_after_step1"], atol=2e-5, rtol=0)
             assert np.allclose(wu, ref["w_u_after_step1"], atol=2e-5, rtol=0)
+
+
+def _attitude_case():
+    name, prob, sp = W.spacecraft7(batch=2, seed=3, horizon=12)
+    prob = oracle_problem(name, prob)
+    rng = np.random.default_rng(1)
+    prob["x_ref"] = prob["x_ref"] + 0.05 * rng.normal(size=prob["x_ref"].shape)
+    sp = dict(sp)
+    sp.update(num_scp_iteration_max=30, tol_convergence=1e-13)
+    sp["pcg"] = dict(sp["pcg"], tol_epsilon=1e-26, max_iter=2000)
+    return prob, sp
+
+
+def _with(prob, tq, tr, x0, xref):
+    p = dict(prob)
+    p["w_x"] = np.broadcast_to(np.exp(tq), p["w_x"].shape).copy()
+    p["w_u"] = np.broadcast_to(np.exp(tr), p["w_u"].shape).copy()
+    p["x0"], p["x_ref"] = x0, xref
+    return p
+
+
+def _attitude_gradients(prob, sp):
+    """Implicit gradients of 0.5 sum ||x* - x_ref||^2 w.r.t. log weights
+    (shared), x0 and x_ref (direct term included)."""
+    th_q, th_r = prob["theta_q"], prob["theta_r"]
+    p = _with(prob, th_q, th_r, prob["x0"], prob["x_ref"])
+    s = sqp.sqp_solve(p, sp)
+    assert np.all(s["status"] == sqp.SUCCESS)
+    g = sqp.sqp_backward(p, sp, s, s["states"] - p["x_ref"], np.zeros_like(s["controls"]))
+    grad = {
+        "theta_q": np.sum(g["w_x"], axis=0) * np.exp(th_q),
+        "theta_r": np.sum(g["w_u"], axis=0) * np.exp(th_r),
+        "x0": g["x0"],
+        "x_ref": g["x_ref"] - (s["states"] - p["x_ref"]),
+    }
+    base = dict(theta_q=th_q, theta_r=th_r, x0=prob["x0"], x_ref=prob["x_ref"])
+    return s, grad, base
+
+
+_PICKS = {
+    "theta_q": [(i,) for i in range(7)],
+    "theta_r": [(i,) for i in range(3)],
+    "x0": [(b, i) for b in range(2) for i in (0, 2, 4, 6)],
+    "x_ref": [(0, 0, 0), (0, 3, 1), (1, 7, 4), (1, 12, 6), (0, 12, 0), (1, 5, 2)],
+}
+
+
+def _central(loss, base, key, idx, h):
+    hi = {k: v.copy() for k, v in base.items()}
+    lo = {k: v.copy() for k, v in base.items()}
+    hi[key][idx] += h
+    lo[key][idx] -= h
+    return (loss(**hi) - loss(**lo)) / (2 * h)
+
+
+def test_attitude_gradients_are_the_sensitivities_of_the_converged_qp():
+    """The reference's backward (sqp.py:605-798) differentiates the KKT system
+    of the last QP: cost Hessian plus linearized dynamics, without the
+    multiplier-weighted curvature of the dynamics.  So on a nonlinear model its
+    gradient is exactly the derivative of that QP's solution at the converged
+    linearization point.  Pinned here by central differences of the QP re-solved
+    with perturbed weights / x0 / references at the fixed linearization."""
+    prob, sp = _attitude_case()
+    s, grad, base = _attitude_gradients(prob, sp)
+    xbar, ubar, lam = s["states"], s["controls"], s["dual"]
+
+    def loss(theta_q, theta_r, x0, x_ref):
+        p = _with(prob, theta_q, theta_r, x0, x_ref)
+        x, _, _, _, _, _ = sqp.solve_qp(sqp.qp_parameters(p, xbar, ubar), lam, sp["pcg"])
+        return 0.5 * np.sum((x - x_ref) ** 2)
+
+    for key, idxs in _PICKS.items():
+        scale = np.max(np.abs(grad[key]))
+        for idx in idxs:
+            num = _central(loss, base, key, idx, 1e-6)
+            assert abs(grad[key][idx] - num) <= 1e-6 * scale, (key, idx, grad[key][idx], num)
+
+
+def test_reference_differentiability_check_on_the_rate_spacecraft():
+    """The reference's own gradient test (tests/test_differentiable.py:21-92):
+    rate-only spacecraft (spacecraft.yaml: inertia [5, 2, 1], dt 0.1, Euler),
+    horizon 10 (N = 9), x0 = -0.1 + 0.2 rand(3) after np.random.seed(0), zero
+    references, w_f = 0, 30 SQP iterations to 1e-12, PCG 1e-24; objective
+    ||states|| + ||controls|| (Frobenius); gradient w.r.t. the state and
+    control weights against central differences, |fd - grad| <= 1e-3 as there."""
+    rs = np.random.RandomState(0)
+    x_init = -0.1 + rs.rand(3) * 0.2
+    N = 9
+    prob = dict(horizon=N, dt=0.1, scheme=0, penalize_control_reference=False, x0=x_init[None],
+                x_final=np.zeros((1, 3)), x_ref=np.zeros((1, N + 1, 3)), u_ref=np.zeros((1, N + 1, 3)),
+                w_x=np.ones((1, 3)), w_u=np.ones((1, 3)), w_f=np.zeros((1, 3)),
+                dyn=np.array([[5.0, 2.0, 1.0]]))
+    prob = oracle_problem("spacecraft3", prob)
+    sp = W.solver_params(30, 1e-12, 1e-24)
+
+    def objective(p):
+        s = sqp.sqp_solve(p, sp)
+        return np.linalg.norm(s["states"][0]) + np.linalg.norm(s["controls"][0]), s
+
+    f0, s = objective(prob)
+    x, u = s["states"], s["controls"]
+    g = sqp.sqp_backward(prob, sp, s, x / np.linalg.norm(x[0]), u / np.linalg.norm(u[0]))
+    h = 1e-6
+    for key in ("w_x", "w_u"):
+        for i in range(3):
+            hi, lo = dict(prob), dict(prob)
+            hi[key], lo[key] = prob[key].copy(), prob[key].copy()
+            hi[key][0, i] += h
+            lo[key][0, i] -= h
+            num = (objective(hi)[0] - objective(lo)[0]) / (2 * h)
+            assert abs(num - g[key][0, i]) <= 1e-3, (key, i, num, g[key][0, i])
