@@ -57,7 +57,7 @@ class ProblemData(ctypes.Structure):
     _fields_ = [
         ("x0", ctypes.c_void_p), ("x_final", ctypes.c_void_p), ("x_ref", ctypes.c_void_p),
         ("u_ref", ctypes.c_void_p), ("w_x", ctypes.c_void_p), ("w_u", ctypes.c_void_p),
-        ("w_f", ctypes.c_void_p), ("dyn_params", ctypes.c_void_p),
+        ("w_f", ctypes.c_void_p), ("dyn_params", ctypes.c_void_p), ("horizons", ctypes.c_void_p),
     ]
 
 
@@ -71,7 +71,8 @@ class Solution(ctypes.Structure):
 
 
 class Saved(ctypes.Structure):
-    _fields_ = [("AB", ctypes.c_void_p), ("fd", ctypes.c_void_p), ("qinv", ctypes.c_void_p), ("rinv", ctypes.c_void_p)]
+    _fields_ = [("AB", ctypes.c_void_p), ("fd", ctypes.c_void_p), ("qinv", ctypes.c_void_p), ("rinv", ctypes.c_void_p),
+                ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_uint64)]
 
 
 class Grads(ctypes.Structure):
@@ -98,6 +99,9 @@ _SIGNATURES = {
                        + [ctypes.c_void_p] * 3),
     "dmpc_pcg_recover": (ctypes.c_int, [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 11),
     "dmpc_set_blocks_per_group": (ctypes.c_int, [ctypes.c_int32]),
+    "dmpc_set_fused_variant": (ctypes.c_int, [ctypes.c_int32]),
+    "dmpc_workspace_bytes": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(Problem), ctypes.POINTER(SolverParams),
+                                            ctypes.POINTER(ctypes.c_uint64)]),
     "dmpc_profile_enable": (ctypes.c_int, [ctypes.c_int32]),
     "dmpc_profile_read": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
 }
@@ -139,6 +143,14 @@ def load():
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+def workspace_bytes(mode, prob, sp):
+    """Device workspace (bytes) dmpc_sqp_solve / dmpc_sqp_backward need."""
+    n = ctypes.c_uint64(0)
+    check(load().dmpc_workspace_bytes(mode, ctypes.byref(prob), ctypes.byref(sp), ctypes.byref(n)),
+          "dmpc_workspace_bytes")
+    return n.value
 
 
 def exported_symbols():

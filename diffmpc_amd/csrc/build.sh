@@ -10,7 +10,8 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
   -Xclang -target-feature -Xclang -packed-fp32-ops \
   -mcode-object-version=5 \
   -Wall -Wno-unused-function \
-  "${here}/dmpc_fused.hip" "${here}/dmpc_plain.hip" "${here}/dmpc_api.hip" \
+  "${here}/dmpc_fused.hip" "${here}/dmpc_stream.hip" "${here}/dmpc_plain.hip" "${here}/dmpc_api.hip" \
+  -Wl,--no-undefined -L"${ROCM_PATH:-/opt/rocm}/lib" -lamdhip64 \
   -o "${out}.tmp"
 mv "${out}.tmp" "${out}"
 echo "built ${out}"

@@ -134,6 +134,25 @@ int dmpc_set_blocks_per_group(int32_t k) {
   return DMPC_OK;
 }
 
+int dmpc_set_fused_variant(int32_t v) {
+  if (v < 0 || v > 2) return DMPC_ERR_ARG;
+  g_fused_variant = v;
+  return DMPC_OK;
+}
+
+int dmpc_workspace_bytes(int32_t mode, const dmpc_problem* prob, const dmpc_solver_params* params, uint64_t* bytes) {
+  int rc = check_problem(prob);
+  if (rc) return rc;
+  if (!params || !bytes) return DMPC_ERR_ARG;
+  *bytes = 0;
+  if (mode == DMPC_MODE_PLAIN) return DMPC_OK;
+  if (mode != DMPC_MODE_FUSED) return DMPC_ERR_ARG;
+  const int ls_eval = params->linesearch && !linesearch_is_noop(params);
+  *bytes = (uint64_t)fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, ls_eval, params->n_alphas) *
+           (uint64_t)prob->batch;
+  return DMPC_OK;
+}
+
 int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
                    dmpc_solution* sol, dmpc_saved* saved, int32_t mode, void* stream) {
   int rc = check_problem(prob);
@@ -175,6 +194,7 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   a.wu = data->w_u;
   a.wf = data->w_f;
   a.dyn = data->dyn_params;
+  a.horizons = data->horizons;
   a.x = sol->states;
   a.u = sol->controls;
   a.lam = sol->dual;
@@ -189,8 +209,18 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   a.qinv = saved->qinv;
   a.rinv = saved->rinv;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  if (mode == DMPC_MODE_FUSED) return fused_forward(prob->model, prob->nx, prob->nu, a, st);
-  if (mode == DMPC_MODE_PLAIN) return plain_forward(prob->model, prob->nx, prob->nu, a, st);
+  if (mode == DMPC_MODE_FUSED) {
+    a.work_stride = fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, a.ls_eval, a.n_alphas);
+    if (a.work_stride) {
+      if (!saved->work || saved->work_bytes < (uint64_t)a.work_stride * (uint64_t)a.B) return DMPC_ERR_ARG;
+      a.work = static_cast<char*>(saved->work);
+    }
+    return fused_forward(prob->model, prob->nx, prob->nu, a, st);
+  }
+  if (mode == DMPC_MODE_PLAIN) {
+    if (data->horizons) return DMPC_ERR_UNSUPPORTED;  // ragged batches: fused mode only
+    return plain_forward(prob->model, prob->nx, prob->nu, a, st);
+  }
   return DMPC_ERR_ARG;
 }
 
@@ -219,6 +249,7 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
   a.wx = data->w_x;
   a.wu = data->w_u;
   a.wf = data->w_f;
+  a.horizons = data->horizons;
   a.x = states;
   a.u = controls;
   a.AB = saved->AB;
@@ -236,8 +267,18 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
   a.g_xf = g->x_final;
   a.pcg_iters = g->pcg_iters;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  if (mode == DMPC_MODE_FUSED) return fused_backward(prob->model, prob->nx, prob->nu, a, st);
-  if (mode == DMPC_MODE_PLAIN) return plain_backward(prob->model, prob->nx, prob->nu, a, st);
+  if (mode == DMPC_MODE_FUSED) {
+    a.work_stride = fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, 0, 0);
+    if (a.work_stride) {
+      if (!saved->work || saved->work_bytes < (uint64_t)a.work_stride * (uint64_t)a.B) return DMPC_ERR_ARG;
+      a.work = static_cast<char*>(saved->work);
+    }
+    return fused_backward(prob->model, prob->nx, prob->nu, a, st);
+  }
+  if (mode == DMPC_MODE_PLAIN) {
+    if (data->horizons) return DMPC_ERR_UNSUPPORTED;
+    return plain_backward(prob->model, prob->nx, prob->nu, a, st);
+  }
   return DMPC_ERR_ARG;
 }
 

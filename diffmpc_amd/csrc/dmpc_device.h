@@ -1,0 +1,208 @@
+// Device helpers shared by the fused kernels: DPP group broadcasts and
+// reductions inside a wavefront, workgroup reductions, float64 dot products
+// of a lane group against a matrix row held per lane.  Every function here
+// must be called by all lanes of the wavefront (DPP and readlane read other
+// lanes), i.e. outside lane-divergent branches.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+namespace dmpc {
+namespace {
+
+// Sum over the G lanes of a block group, result in every lane of the group
+// (DPP butterflies: quad swaps, then half-row and row mirrors).
+template <int ctrl>
+__device__ __forceinline__ double dpp_mov(double v) {
+  long long x = __double_as_longlong(v);
+  x = __builtin_amdgcn_mov_dpp(x, ctrl, 0xf, 0xf, true);
+  return __longlong_as_double(x);
+}
+// Value of lane src (wave-uniform result, read as scalars).
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const long long x = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// G = 64 is a whole wavefront: the four row sums are combined as scalars.
+template <int G>
+__device__ __forceinline__ double gsum(double v) {
+  static_assert(G == 4 || G == 8 || G == 16 || G == 64, "block groups of 4, 8, 16 or 64 lanes");
+  v += dpp_mov<0xB1>(v);                        // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);                        // quad_perm [2,3,0,1]
+  if constexpr (G >= 8) v += dpp_mov<0x141>(v);  // row_half_mirror
+  if constexpr (G >= 16) v += dpp_mov<0x140>(v); // row_mirror
+  if constexpr (G == 64) v = (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+  return v;
+}
+
+
+// Sum over a full wave: DPP sums inside each 16-lane row, then the four row
+// sums are read as scalars.  The result is wave-uniform.
+__device__ __forceinline__ double wave_sum(double v) {
+  v = gsum<16>(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
+__device__ __forceinline__ double nanmax(double a, double b) {
+  if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000ULL);
+  return a > b ? a : b;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Sum over the workgroup through one barrier; every thread gets the same,
+// bit-identical result (same summation order everywhere).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  double s = 0.0;
+  for (int k = 0; k < nw; ++k) s += red[k];
+  return s;
+}
+
+__device__ __forceinline__ double block_max(double v, double* red) {
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  double s = red[0];
+  for (int k = 1; k < nw; ++k) s = nanmax(s, red[k]);
+  return s;
+}
+
+__device__ __forceinline__ double nan_to_num(double v) {
+  if (v != v) return 0.0;
+  if (v == __longlong_as_double(0x7ff0000000000000ULL)) return 1.7976931348623157e308;
+  if (v == __longlong_as_double(0xfff0000000000000ULL)) return -1.7976931348623157e308;
+  return v;
+}
+
+
+// float -> double conversion that the optimizer cannot move out of the PCG
+// loop: the asm takes the (uniform) iteration counter as an operand, so it is
+// not loop-invariant, but it has no side effects and schedules freely.
+// Without it the compiler hoists the conversions of the A/B registers and
+// keeps the double copies, which do not fit in the register file.
+__device__ __forceinline__ double cvt_in_loop(float f, int salt) {
+  double d;
+  asm("v_cvt_f64_f32 %0, %1" : "=v"(d) : "v"(f), "s"(salt));
+  return d;
+}
+__device__ __forceinline__ double cvt_in_loop(double d, int) { return d; }
+
+// ---------------------------------------------------------------------------
+// Broadcast of lane n of each G-lane block group to the whole group with DPP
+// (no LDS traffic): G = 16 is one DPP row (row_newbcast:n, one 64-bit move on
+// gfx950); G = 4 a quad (quad_perm [n,n,n,n]); G = 8 is half a row (two
+// bank-masked row_newbcast moves).
+template <int n>
+__device__ __forceinline__ double dpp_rowbcast(double v) {
+  long long x = __double_as_longlong(v);
+  x = __builtin_amdgcn_mov_dpp(x, 0x150 + n, 0xf, 0xf, true);
+  return __longlong_as_double(x);
+}
+template <int n>
+__device__ __forceinline__ double dpp_quadbcast(double v) {
+  long long x = __double_as_longlong(v);
+  x = __builtin_amdgcn_mov_dpp(x, n | (n << 2) | (n << 4) | (n << 6), 0xf, 0xf, true);
+  return __longlong_as_double(x);
+}
+
+template <int G, int n>
+__device__ __forceinline__ double gbcast(double v, int lane) {
+  static_assert(n < G, "source lane outside the group");
+  if constexpr (G == 64) {
+    return lane_value(v, n);  // the group is the wavefront
+  } else if constexpr (G == 16) {
+    return dpp_rowbcast<n>(v);
+  } else if constexpr (G == 4) {
+    return dpp_quadbcast<n>(v);
+  } else {
+    static_assert(G == 8, "block groups of 4, 8, 16 or 64 lanes");
+    // groups are 8-aligned in the workgroup, two per DPP row: broadcast lane
+    // n into banks 0-1 (lanes 0-7) and lane 8 + n into banks 2-3 (lanes 8-15)
+    // of the same register; the bank masks do the selection
+    const long long x = __double_as_longlong(v);
+    const long long lo = __builtin_amdgcn_mov_dpp(x, 0x150 + n, 0xf, 0x3, false);
+    return __longlong_as_double(__builtin_amdgcn_update_dpp(lo, x, 0x150 + n + 8, 0xf, 0xc, false));
+  }
+}
+
+// sum_j a[j] * v_j where v_j lives in lane j of the group (j < M <= NX).
+template <int G, int M, int j = 0, class AT>
+__device__ __forceinline__ double gdot_chain(const AT* a, double v, int lane, int salt, double acc) {
+  if constexpr (j == M) {
+    return acc;
+  } else {
+    acc += cvt_in_loop(a[j], salt) * gbcast<G, j>(v, lane);
+    // keep the scheduler from converting and broadcasting a whole row ahead
+    // of its FMAs (each in-flight term holds four VGPRs)
+    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    return gdot_chain<G, M, j + 1>(a, v, lane, salt, acc);
+  }
+}
+
+// acc + sum_{j<13} a[j] * v_j for 16-lane groups with a[] in float64
+// registers: each term is one v_fmac_f64 whose first operand is read through
+// DPP row_newbcast:j (lane j of the 16-lane row), so the broadcast costs no
+// instruction.  One asm block keeps the chain free of compiler-inserted
+// writes to v; the leading s_nop 4 covers the DPP read-after-VALU-write wait
+// states (2 for a VGPR, 5 after a VALU write of EXEC).  Terms are summed in
+// order j = 0..12 with one rounding each, as the generic chain does.
+__device__ __forceinline__ double gdot16x13(const double* a, double v, double acc) {
+#define DMPC_FMAC_BCAST(op, j) "v_fmac_f64_dpp %0, %1, %" #op " row_newbcast:" #j " row_mask:0xf bank_mask:0xf\n"
+  asm("s_nop 4\n" DMPC_FMAC_BCAST(2, 0) DMPC_FMAC_BCAST(3, 1) DMPC_FMAC_BCAST(4, 2) DMPC_FMAC_BCAST(5, 3)
+          DMPC_FMAC_BCAST(6, 4) DMPC_FMAC_BCAST(7, 5) DMPC_FMAC_BCAST(8, 6) DMPC_FMAC_BCAST(9, 7)
+              DMPC_FMAC_BCAST(10, 8) DMPC_FMAC_BCAST(11, 9) DMPC_FMAC_BCAST(12, 10) DMPC_FMAC_BCAST(13, 11)
+                  DMPC_FMAC_BCAST(14, 12)
+      : "+v"(acc)
+      : "v"(v), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]),
+        "v"(a[8]), "v"(a[9]), "v"(a[10]), "v"(a[11]), "v"(a[12]));
+#undef DMPC_FMAC_BCAST
+  return acc;
+}
+
+template <int G, int M, class AT>
+__device__ __forceinline__ double gdot(const AT* a, double v, int lane, int salt) {
+  if constexpr (G == 16 && M == 13 && std::is_same<AT, double>::value) {
+    return gdot16x13(a, v, 0.0);
+  } else {
+    return gdot_chain<G, M>(a, v, lane, salt, 0.0);
+  }
+}
+template <int G, int M, int j = 0>
+__device__ __forceinline__ double gdotd(const double* a, double v, int lane, double acc = 0.0) {
+  if constexpr (j == M) {
+    return acc;
+  } else {
+    acc += a[j] * gbcast<G, j>(v, lane);
+    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    return gdotd<G, M, j + 1>(a, v, lane, acc);
+  }
+}
+
+
+// sum_j a[j * STRIDE] * v_j (a in LDS), v_j in lane j of the group.
+template <int G, int M, int STRIDE, int j = 0>
+__device__ __forceinline__ double gdot_strided(const double* a, double v, int lane, double acc = 0.0) {
+  if constexpr (j == M) {
+    return acc;
+  } else {
+    acc += a[j * STRIDE] * gbcast<G, j>(v, lane);
+    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+    return gdot_strided<G, M, STRIDE, j + 1>(a, v, lane, acc);
+  }
+}
+
+}  // namespace
+}  // namespace dmpc

@@ -29,85 +29,22 @@
 // optimal_control_problem.py:301-322 are used in closed form.
 #include <type_traits>
 
+#include "dmpc_device.h"
 #include "dmpc_internal.h"
+#include "dmpc_linesearch.h"
 #include "dmpc_models.h"
 
 namespace dmpc {
 
 int g_forced_K = 0;
 
+// streamed variant (dmpc_stream.hip)
+bool stream_supported(int model, int nx, int nu);
+size_t stream_work_stride(int nx, int nu, int N, int ls_eval, int n_alphas);
+int stream_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st);
+int stream_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t st);
+
 namespace {
-
-// Sum over the G lanes of a block group, result in every lane of the group
-// (DPP butterflies: quad swaps, then half-row and row mirrors).
-template <int ctrl>
-__device__ __forceinline__ double dpp_mov(double v) {
-  long long x = __double_as_longlong(v);
-  x = __builtin_amdgcn_mov_dpp(x, ctrl, 0xf, 0xf, true);
-  return __longlong_as_double(x);
-}
-template <int G>
-__device__ __forceinline__ double gsum(double v) {
-  v += dpp_mov<0xB1>(v);                        // quad_perm [1,0,3,2]
-  v += dpp_mov<0x4E>(v);                        // quad_perm [2,3,0,1]
-  if constexpr (G >= 8) v += dpp_mov<0x141>(v);  // row_half_mirror
-  if constexpr (G >= 16) v += dpp_mov<0x140>(v); // row_mirror
-  return v;
-}
-
-__device__ __forceinline__ double lane_value(double v, int src) {
-  const long long x = __double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)x, src);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(x >> 32), src);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Sum over a full wave: DPP sums inside each 16-lane row, then the four row
-// sums are read as scalars.  The result is wave-uniform.
-__device__ __forceinline__ double wave_sum(double v) {
-  v = gsum<16>(v);
-  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
-}
-
-__device__ __forceinline__ double nanmax(double a, double b) {
-  if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000ULL);
-  return a > b ? a : b;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-// Sum over the workgroup through one barrier; every thread gets the same,
-// bit-identical result (same summation order everywhere).
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  const int nw = blockDim.x >> 6;
-  double s = 0.0;
-  for (int k = 0; k < nw; ++k) s += red[k];
-  return s;
-}
-
-__device__ __forceinline__ double block_max(double v, double* red) {
-  v = wave_max(v);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  const int nw = blockDim.x >> 6;
-  double s = red[0];
-  for (int k = 1; k < nw; ++k) s = nanmax(s, red[k]);
-  return s;
-}
-
-__device__ __forceinline__ double nan_to_num(double v) {
-  if (v != v) return 0.0;
-  if (v == __longlong_as_double(0x7ff0000000000000ULL)) return 1.7976931348623157e308;
-  if (v == __longlong_as_double(0xfff0000000000000ULL)) return -1.7976931348623157e308;
-  return v;
-}
 
 // LDS carve-up.  Vectors over time blocks use stride NX per block; arrays
 // marked "padded" have a zero block before t = 0 and after t = N, so
@@ -125,12 +62,6 @@ struct LdsPlan {
   static __host__ size_t bytes(int N, int P, int ls = 0) {
     return (doubles(N) + ls) * 8 + floats(N, P) * 4 + 16;
   }
-};
-
-// Line-search scratch: merit parts of (n_alphas + 1) x (N + 1) items (cost,
-// constraint) and their (n_alphas + 1) sums of each.
-__host__ __device__ inline int linesearch_doubles(int linesearch, int n_alphas, int N) {
-  return linesearch ? 2 * (n_alphas + 1) * (N + 1) + 2 * (n_alphas + 1) : 0;
 };
 
 struct Lds {
@@ -170,125 +101,12 @@ __device__ Lds carve(void* base, int N, int ls = 0) {
   return L;
 }
 
-// float -> double conversion that the optimizer cannot move out of the PCG
-// loop: the asm takes the (uniform) iteration counter as an operand, so it is
-// not loop-invariant, but it has no side effects and schedules freely.
-// Without it the compiler hoists the conversions of the A/B registers and
-// keeps the double copies, which do not fit in the register file.
-__device__ __forceinline__ double cvt_in_loop(float f, int salt) {
-  double d;
-  asm("v_cvt_f64_f32 %0, %1" : "=v"(d) : "v"(f), "s"(salt));
-  return d;
-}
-__device__ __forceinline__ double cvt_in_loop(double d, int) { return d; }
-
-// ---------------------------------------------------------------------------
-// Broadcast of lane n of each G-lane block group to the whole group with DPP
-// (no LDS traffic): G = 16 is one DPP row (row_newbcast:n, one 64-bit move on
-// gfx950); G = 4 a quad (quad_perm [n,n,n,n]); G = 8 is half a row (two
-// bank-masked row_newbcast moves).
-template <int n>
-__device__ __forceinline__ double dpp_rowbcast(double v) {
-  long long x = __double_as_longlong(v);
-  x = __builtin_amdgcn_mov_dpp(x, 0x150 + n, 0xf, 0xf, true);
-  return __longlong_as_double(x);
-}
-template <int n>
-__device__ __forceinline__ double dpp_quadbcast(double v) {
-  long long x = __double_as_longlong(v);
-  x = __builtin_amdgcn_mov_dpp(x, n | (n << 2) | (n << 4) | (n << 6), 0xf, 0xf, true);
-  return __longlong_as_double(x);
-}
-
-template <int G, int n>
-__device__ __forceinline__ double gbcast(double v, int lane) {
-  static_assert(n < G, "source lane outside the group");
-  if constexpr (G == 16) {
-    return dpp_rowbcast<n>(v);
-  } else if constexpr (G == 4) {
-    return dpp_quadbcast<n>(v);
-  } else {
-    static_assert(G == 8, "block groups of 4, 8 or 16 lanes");
-    // groups are 8-aligned in the workgroup, two per DPP row: broadcast lane
-    // n into banks 0-1 (lanes 0-7) and lane 8 + n into banks 2-3 (lanes 8-15)
-    // of the same register; the bank masks do the selection
-    const long long x = __double_as_longlong(v);
-    const long long lo = __builtin_amdgcn_mov_dpp(x, 0x150 + n, 0xf, 0x3, false);
-    return __longlong_as_double(__builtin_amdgcn_update_dpp(lo, x, 0x150 + n + 8, 0xf, 0xc, false));
-  }
-}
-
-// sum_j a[j] * v_j where v_j lives in lane j of the group (j < M <= NX).
-template <int G, int M, int j = 0, class AT>
-__device__ __forceinline__ double gdot_chain(const AT* a, double v, int lane, int salt, double acc) {
-  if constexpr (j == M) {
-    return acc;
-  } else {
-    acc += cvt_in_loop(a[j], salt) * gbcast<G, j>(v, lane);
-    // keep the scheduler from converting and broadcasting a whole row ahead
-    // of its FMAs (each in-flight term holds four VGPRs)
-    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-    return gdot_chain<G, M, j + 1>(a, v, lane, salt, acc);
-  }
-}
-
-// acc + sum_{j<13} a[j] * v_j for 16-lane groups with a[] in float64
-// registers: each term is one v_fmac_f64 whose first operand is read through
-// DPP row_newbcast:j (lane j of the 16-lane row), so the broadcast costs no
-// instruction.  One asm block keeps the chain free of compiler-inserted
-// writes to v; the leading s_nop 4 covers the DPP read-after-VALU-write wait
-// states (2 for a VGPR, 5 after a VALU write of EXEC).  Terms are summed in
-// order j = 0..12 with one rounding each, as the generic chain does.
-__device__ __forceinline__ double gdot16x13(const double* a, double v, double acc) {
-#define DMPC_FMAC_BCAST(op, j) "v_fmac_f64_dpp %0, %1, %" #op " row_newbcast:" #j " row_mask:0xf bank_mask:0xf\n"
-  asm("s_nop 4\n" DMPC_FMAC_BCAST(2, 0) DMPC_FMAC_BCAST(3, 1) DMPC_FMAC_BCAST(4, 2) DMPC_FMAC_BCAST(5, 3)
-          DMPC_FMAC_BCAST(6, 4) DMPC_FMAC_BCAST(7, 5) DMPC_FMAC_BCAST(8, 6) DMPC_FMAC_BCAST(9, 7)
-              DMPC_FMAC_BCAST(10, 8) DMPC_FMAC_BCAST(11, 9) DMPC_FMAC_BCAST(12, 10) DMPC_FMAC_BCAST(13, 11)
-                  DMPC_FMAC_BCAST(14, 12)
-      : "+v"(acc)
-      : "v"(v), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]),
-        "v"(a[8]), "v"(a[9]), "v"(a[10]), "v"(a[11]), "v"(a[12]));
-#undef DMPC_FMAC_BCAST
-  return acc;
-}
-
-template <int G, int M, class AT>
-__device__ __forceinline__ double gdot(const AT* a, double v, int lane, int salt) {
-  if constexpr (G == 16 && M == 13 && std::is_same<AT, double>::value) {
-    return gdot16x13(a, v, 0.0);
-  } else {
-    return gdot_chain<G, M>(a, v, lane, salt, 0.0);
-  }
-}
-template <int G, int M, int j = 0>
-__device__ __forceinline__ double gdotd(const double* a, double v, int lane, double acc = 0.0) {
-  if constexpr (j == M) {
-    return acc;
-  } else {
-    acc += a[j] * gbcast<G, j>(v, lane);
-    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-    return gdotd<G, M, j + 1>(a, v, lane, acc);
-  }
-}
-
 // Where D_t = theta_t^-1 lives during PCG: in registers (row `lane` per lane)
 // for small blocks, in LDS for nx > 8, where the registers are needed for the
 // A/B rows.  The LDS copy is stored transposed, DT[t][j][i] = D_t[i][j] as
 // computed by lane i, so the lanes of a group read consecutive words.
 template <int NX>
 __host__ __device__ constexpr bool d_in_lds() { return NX > 8; }
-
-// sum_j a[j * STRIDE] * v_j (a in LDS), v_j in lane j of the group.
-template <int G, int M, int STRIDE, int j = 0>
-__device__ __forceinline__ double gdot_strided(const double* a, double v, int lane, double acc = 0.0) {
-  if constexpr (j == M) {
-    return acc;
-  } else {
-    acc += a[j * STRIDE] * gbcast<G, j>(v, lane);
-    if constexpr (j % 4 == 3) __builtin_amdgcn_sched_barrier(0);
-    return gdot_strided<G, M, STRIDE, j + 1>(a, v, lane, acc);
-  }
-}
 
 // Per-thread register state of the K blocks it owns.
 template <int NX, int NU, int K>
@@ -678,15 +496,16 @@ __device__ void zero_lds_vectors(const Lds& L, int N) {
 // optimal_control_problem.py:249-323): Qinv = 1 / max(2 w, 1e-9),
 // q = -2 w ref (terminal: w_x + w_f and final_state), r = -2 w_u u_ref.
 template <int NX, int NU>
-__device__ void cost_terms(const Lds& L, int N, int b, const double* wx, const double* wu, const double* wf,
+__device__ void cost_terms(const Lds& L, int N, int NBs, int b, const double* wx, const double* wu, const double* wf,
                            const double* xref, const double* uref, const double* xf, int penalize) {
+  // N: this instance's horizon; NBs: knot stride of the global arrays
   const int NB = N + 1;
   for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
     const int t = k / NX, i = k % NX;
     const double w = wx[b * NX + i];
     if (t < N) {
       L.qinv[k] = 1.0 / fmax(2.0 * w, kPsdMinEig);
-      L.qv[k] = -2.0 * w * xref[((size_t)b * NB + t) * NX + i];
+      L.qv[k] = -2.0 * w * xref[((size_t)b * NBs + t) * NX + i];
     } else {
       const double wt = w + wf[b * NX + i];
       L.qinv[k] = 1.0 / fmax(2.0 * wt, kPsdMinEig);
@@ -695,7 +514,7 @@ __device__ void cost_terms(const Lds& L, int N, int b, const double* wx, const d
   }
   for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) {
     const int t = k / NU, i = k % NU;
-    L.rv[k] = (t < N && penalize) ? -2.0 * wu[b * NU + i] * uref[((size_t)b * NB + t) * NU + i] : 0.0;
+    L.rv[k] = (t < N && penalize) ? -2.0 * wu[b * NU + i] * uref[((size_t)b * NBs + t) * NU + i] : 0.0;
   }
   for (int k = threadIdx.x; k < NU; k += blockDim.x) L.rinv[k] = 1.0 / fmax(2.0 * wu[b * NU + k], kPsdMinEig);
 }
@@ -708,10 +527,6 @@ __device__ void cost_terms(const Lds& L, int N, int b, const double* wx, const d
 // (num_iter < maxit && err > tol) || num_iter < 1; finished instances return
 // at once and keep their last linearization, like the reference's vmapped
 // while_loop keeps frozen instances.
-__device__ __forceinline__ bool sqp_active(const SolveArgs& a, int b) {
-  const int it = a.num_iter[b];
-  return (it < a.maxit && a.conv_err[b] > a.tol) || it < 1;
-}
 
 __global__ void sqp_reset_kernel(int B, int32_t* num_iter) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -729,7 +544,7 @@ __global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
   const int j = idx % NZ;
   const int t = (idx / NZ) % N;
   const int b = idx / ((long long)NZ * N);
-  if (!sqp_active(a, b)) return;
+  if (!sqp_active(a, b) || t >= horizon_of(a.horizons, N, b)) return;
   float xs[NX], us[NU], prm[Model::NPARAM];
 #pragma unroll
   for (int i = 0; i < NX; ++i) xs[i] = (float)a.x[((size_t)b * (N + 1) + t) * NX + i];
@@ -751,117 +566,6 @@ __global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
 // One SQP iteration of one instance per workgroup: Schur/preconditioner
 // assembly, gamma, the whole PCG loop, primal recovery and the step test
 // (sqp.py:262-299, 506-528).
-// Merit line search of sqp.py:301-411 for instance b (Nocedal & Wright,
-// Alg. 18.3): phi(z; mu) = cost(z) + mu * ||g(z)||_1, mu = max(0.1,
-// (dcost/dz . dz) / (0.5 ||g||_1)) (max propagates NaN, like jnp.maximum),
-// decrease_k = phi(z + alpha_k dz) - (phi(z) + eta alpha_k D phi), then the
-// candidates are scanned from the largest with the reference's clamped
-// indexing (alphas[min(i, n-1)]).  z = (xbar, ubar) in L.xb / L.ub, dz in
-// DX (N+1, nx) and DU (N+1, nu).  Everything is float64, including the
-// dynamics evaluations of the equality constraints
-// g = [x_0 - x_init; f_d(x_t, u_t) - x_{t+1}] (ocp.py:104-151).
-// Items (candidate k, knot t) are spread over the workgroup; the sums run
-// in a fixed order, so every thread returns the same alpha.
-template <class Model>
-__device__ double line_search(const SolveArgs& a, const Lds& L, int N, int b, const double* DX, const double* DU) {
-  constexpr int NX = Model::NX, NU = Model::NU;
-  const int NB = N + 1, nal = a.n_alphas, items = (nal + 1) * NB;
-  double* LC = L.LS;               // cost part of each item
-  double* LG = L.LS + items;       // constraint part of each item
-  double* SUM = L.LS + 2 * items;  // (nal + 1) x {cost, constraint}
-  const double* wx = a.wx + (size_t)b * NX;
-  const double* wu = a.wu + (size_t)b * NU;
-  const double* wf = a.wf + (size_t)b * NX;
-  const double* xr = a.xref + (size_t)b * NB * NX;
-  const double* ur = a.uref + (size_t)b * NB * NU;
-  const double* xf = a.xf + (size_t)b * NX;
-  const double* x0 = a.x0 + (size_t)b * NX;
-  const float* prm = a.dyn + (size_t)b * a.P;
-  // directional derivative of the cost (the last control row has none)
-  double part = 0.0;
-  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
-    const int t = k / NX, i = k - t * NX;
-    const double gx = (t < N) ? 2.0 * wx[i] * (L.xb[k] - xr[k]) : 2.0 * (wx[i] + wf[i]) * (L.xb[k] - xf[i]);
-    part += gx * DX[k];
-  }
-  for (int k = threadIdx.x; k < N * NU; k += blockDim.x) {
-    const int j = k % NU;
-    const double ref = a.penalize ? ur[k] : 0.0;
-    part += 2.0 * wu[j] * (L.ub[k] - ref) * DU[k];
-  }
-  const double dcdz = block_sum(part, L.redA);
-  // merit parts: item (kc, t) holds the cost of knot t and the constraint
-  // block t at alpha = 0 (kc = 0) or alphas[kc - 1]
-  for (int j = threadIdx.x; j < items; j += blockDim.x) {
-    const int kc = j / NB, t = j - kc * NB;
-    const double al = kc == 0 ? 0.0 : a.alphas[kc - 1];
-    double c = 0.0, g = 0.0;
-    if (t < N) {
-#pragma unroll
-      for (int i = 0; i < NX; ++i) {
-        const double d = L.xb[t * NX + i] + al * DX[t * NX + i] - xr[t * NX + i];
-        c += wx[i] * d * d;
-      }
-#pragma unroll
-      for (int i = 0; i < NU; ++i) {
-        const double d = L.ub[t * NU + i] + al * DU[t * NU + i] - (a.penalize ? ur[t * NU + i] : 0.0);
-        c += wu[i] * d * d;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < NX; ++i) {
-        const double d = L.xb[t * NX + i] + al * DX[t * NX + i] - xf[i];
-        c += (wx[i] + wf[i]) * d * d;
-      }
-    }
-    if (t == 0) {
-#pragma unroll
-      for (int i = 0; i < NX; ++i) g += fabs(L.xb[i] + al * DX[i] - x0[i]);
-    } else {
-      double xp[NX], up[NU], fx[NX];
-#pragma unroll
-      for (int i = 0; i < NX; ++i) xp[i] = L.xb[(t - 1) * NX + i] + al * DX[(t - 1) * NX + i];
-#pragma unroll
-      for (int i = 0; i < NU; ++i) up[i] = L.ub[(t - 1) * NU + i] + al * DU[(t - 1) * NU + i];
-      step<Model, double>(xp, up, prm, a.dt64, a.scheme, fx);
-#pragma unroll
-      for (int i = 0; i < NX; ++i) g += fabs(fx[i] - (L.xb[t * NX + i] + al * DX[t * NX + i]));
-    }
-    LC[j] = c;
-    LG[j] = g;
-  }
-  __syncthreads();
-  if (threadIdx.x <= nal) {
-    double c = 0.0, g = 0.0;
-    for (int t = 0; t < NB; ++t) {
-      c += LC[threadIdx.x * NB + t];
-      g += LG[threadIdx.x * NB + t];
-    }
-    SUM[2 * threadIdx.x] = c;
-    SUM[2 * threadIdx.x + 1] = g;
-  }
-  __syncthreads();
-  const double c0 = SUM[0], g0 = SUM[1];
-  const double mu_min = dcdz / (0.5 * g0);
-  const double mu = (mu_min != mu_min) ? mu_min : fmax(0.1, mu_min);
-  const double merit = c0 + mu * g0;
-  const double dmerit = dcdz + -mu * g0;
-  double amax = a.alphas[0];
-  for (int k = 1; k < nal; ++k) amax = (a.alphas[k] != a.alphas[k] || amax != amax) ? NAN : fmax(amax, a.alphas[k]);
-  // decrease of candidate k (0-based in alphas)
-  auto decrease = [&](int k) {
-    const double al = a.alphas[k];
-    return (SUM[2 * (k + 1)] + mu * SUM[2 * (k + 1) + 1]) - (merit + a.ls_eta * al * dmerit);
-  };
-  int i = nal + 1;
-  double alpha = amax;
-  while (!(decrease(i < nal - 1 ? i : nal - 1) < 0.0 || i == 0)) {
-    alpha = a.alphas[(i - 1) < nal - 1 ? (i - 1) : nal - 1];
-    --i;
-  }
-  return alpha;
-}
-
 template <class Model, int G, int K, int MAXT>
 __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
@@ -869,25 +573,26 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   const int b = blockIdx.x;
   if (!sqp_active(a, b)) return;
   const int it = a.num_iter[b];
-  const int N = a.N, NB = N + 1;
-  const Lds L = carve<NX, NU>(smem_, N, linesearch_doubles(a.ls_eval, a.n_alphas, N));
+  const int Np = a.N, NBs = Np + 1;                          // padded horizon, knot stride
+  const int N = horizon_of(a.horizons, Np, b), NB = N + 1;  // this instance
+  const Lds L = carve<NX, NU>(smem_, Np, linesearch_doubles(a.ls_eval, a.n_alphas, Np));
   Regs<NX, NU, K> R;
   int lane, gbase;
   init_blocks(R, G, N, &lane, &gbase);
 
-  zero_lds_vectors<NX, NU>(L, N);
+  zero_lds_vectors<NX, NU>(L, Np);
   __syncthreads();
   for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
-    L.xb[k] = a.x[(size_t)b * NB * NX + k];
-    L.lamS[NX + k] = a.lam[(size_t)b * NB * NX + k];
+    L.xb[k] = a.x[(size_t)b * NBs * NX + k];
+    L.lamS[NX + k] = a.lam[(size_t)b * NBs * NX + k];
   }
-  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) L.ub[k] = a.u[(size_t)b * NB * NU + k];
-  for (int k = threadIdx.x; k < N * NX; k += blockDim.x) L.fd[k] = a.fd[(size_t)b * N * NX + k];
+  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) L.ub[k] = a.u[(size_t)b * NBs * NU + k];
+  for (int k = threadIdx.x; k < N * NX; k += blockDim.x) L.fd[k] = a.fd[(size_t)b * Np * NX + k];
   for (int k = threadIdx.x; k < NX; k += blockDim.x) L.x0[k] = a.x0[b * NX + k];
-  cost_terms<NX, NU>(L, N, b, a.wx, a.wu, a.wf, a.xref, a.uref, a.xf, a.penalize);
+  cost_terms<NX, NU>(L, N, NBs, b, a.wx, a.wu, a.wf, a.xref, a.uref, a.xf, a.penalize);
   __syncthreads();
 
-  const float* ABb = a.AB + (size_t)b * N * NX * NZ;
+  const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
   assemble_dinv(R, ABb, L, lane, gbase);
   load_ab(R, ABb, L, N, lane);
   double gam[K], lam[K];
@@ -913,7 +618,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
       }
     }
     __syncthreads();
-    alpha = line_search<Model>(a, L, N, b, DX, DU);
+    alpha = line_search<Model>(a, b, N, L.xb, L.ub, DX, DU, L.LS, L.redA);
 #pragma unroll
     for (int s = 0; s < K; ++s) {
       if (R.ok[s]) {
@@ -937,12 +642,18 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   for (int s = 0; s < K; ++s) {
     if (R.ok[s]) {
       const int t = R.t[s];
-      a.x[((size_t)b * NB + t) * NX + lane] = xn[s];
-      a.lam[((size_t)b * NB + t) * NX + lane] = lam[s];
-      if (lane < NU) a.u[((size_t)b * NB + control_row(t, N)) * NU + lane] = un[s];
+      a.x[((size_t)b * NBs + t) * NX + lane] = xn[s];
+      a.lam[((size_t)b * NBs + t) * NX + lane] = lam[s];
+      if (lane < NU) a.u[((size_t)b * NBs + control_row(t, N)) * NU + lane] = un[s];
     }
   }
-  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) a.qinv[(size_t)b * NB * NX + k] = L.qinv[k];
+  // knots past this instance's horizon (ragged batches) are zero
+  for (int k = NB * NX + threadIdx.x; k < NBs * NX; k += blockDim.x) {
+    a.x[(size_t)b * NBs * NX + k] = 0.0;
+    a.lam[(size_t)b * NBs * NX + k] = 0.0;
+  }
+  for (int k = NB * NU + threadIdx.x; k < NBs * NU; k += blockDim.x) a.u[(size_t)b * NBs * NU + k] = 0.0;
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) a.qinv[(size_t)b * NBs * NX + k] = L.qinv[k];
   for (int k = threadIdx.x; k < NU; k += blockDim.x) a.rinv[(size_t)b * NU + k] = L.rinv[k];
   if (threadIdx.x == 0) {
     a.conv_errs[(size_t)b * a.maxit + it] = err;
@@ -986,28 +697,29 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   extern __shared__ double smem_[];
   const int b = blockIdx.x;
-  const int N = a.N, NB = N + 1;
-  const Lds L = carve<NX, NU>(smem_, N);
+  const int Np = a.N, NBs = Np + 1;                          // padded horizon, knot stride
+  const int N = horizon_of(a.horizons, Np, b), NB = N + 1;  // this instance
+  const Lds L = carve<NX, NU>(smem_, Np);
   Regs<NX, NU, K> R;
   int lane, gbase;
   init_blocks(R, G, N, &lane, &gbase);
 
-  zero_lds_vectors<NX, NU>(L, N);
+  zero_lds_vectors<NX, NU>(L, Np);
   __syncthreads();
   for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
-    L.qinv[k] = a.qinv[(size_t)b * NB * NX + k];
-    L.qv[k] = -a.dl_dx[(size_t)b * NB * NX + k];
-    L.lamS[NX + k] = a.lam_bwd[(size_t)b * NB * NX + k];
-    L.xb[k] = a.x[(size_t)b * NB * NX + k];
+    L.qinv[k] = a.qinv[(size_t)b * NBs * NX + k];
+    L.qv[k] = -a.dl_dx[(size_t)b * NBs * NX + k];
+    L.lamS[NX + k] = a.lam_bwd[(size_t)b * NBs * NX + k];
+    L.xb[k] = a.x[(size_t)b * NBs * NX + k];
   }
   for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) {
     const int t = k / NU;
-    L.rv[k] = t < N ? -a.dl_du[(size_t)b * NB * NU + k] : 0.0;
-    L.ub[k] = a.u[(size_t)b * NB * NU + k];
+    L.rv[k] = t < N ? -a.dl_du[(size_t)b * NBs * NU + k] : 0.0;
+    L.ub[k] = a.u[(size_t)b * NBs * NU + k];
   }
   for (int k = threadIdx.x; k < NU; k += blockDim.x) L.rinv[k] = a.rinv[(size_t)b * NU + k];
   __syncthreads();
-  const float* ABb = a.AB + (size_t)b * N * NX * NZ;
+  const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
   assemble_dinv(R, ABb, L, lane, gbase);
   load_ab(R, ABb, L, N, lane);
   double gam[K], lam[K];
@@ -1029,26 +741,32 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
     if (R.ok[s]) {
       const int t = R.t[s];
       const int i = lane;
-      const double ref = (t < N) ? a.xref[((size_t)b * NB + t) * NX + i] : a.xf[b * NX + i];
+      const double ref = (t < N) ? a.xref[((size_t)b * NBs + t) * NX + i] : a.xf[b * NX + i];
       const double ex = L.xb[t * NX + i] - ref;
       TX[t * NX + i] = 2.0 * ex * xt[s];
       const double wxi = a.wx[b * NX + i];
-      a.g_xref[((size_t)b * NB + t) * NX + i] = (t < N) ? 2.0 * wxi * xt[s] : 0.0;
+      a.g_xref[((size_t)b * NBs + t) * NX + i] = (t < N) ? 2.0 * wxi * xt[s] : 0.0;
       if (t == N) {
         a.g_wf[b * NX + i] = -2.0 * ex * xt[s];
         a.g_xf[b * NX + i] = 2.0 * (wxi + a.wf[b * NX + i]) * xt[s];
       }
       if (i < NU) {
         const int tu = control_row(t, N);
-        const double ur = a.penalize ? a.uref[((size_t)b * NB + tu) * NU + i] : 0.0;
+        const double ur = a.penalize ? a.uref[((size_t)b * NBs + tu) * NU + i] : 0.0;
         TU[tu * NU + i] = (tu < N) ? 2.0 * (L.ub[tu * NU + i] - ur) * ut[s] : 0.0;
-        a.g_uref[((size_t)b * NB + tu) * NU + i] =
+        a.g_uref[((size_t)b * NBs + tu) * NU + i] =
             (tu < N && a.penalize) ? 2.0 * a.wu[b * NU + i] * ut[s] : 0.0;
       }
-      a.lam_bwd[((size_t)b * NB + t) * NX + i] = lam[s];
+      a.lam_bwd[((size_t)b * NBs + t) * NX + i] = lam[s];
       if (t == 0) a.g_x0[b * NX + i] = lam[s];
     }
   }
+  // knots past this instance's horizon (ragged batches) are zero
+  for (int k = NB * NX + threadIdx.x; k < NBs * NX; k += blockDim.x) {
+    a.g_xref[(size_t)b * NBs * NX + k] = 0.0;
+    a.lam_bwd[(size_t)b * NBs * NX + k] = 0.0;
+  }
+  for (int k = NB * NU + threadIdx.x; k < NBs * NU; k += blockDim.x) a.g_uref[(size_t)b * NBs * NU + k] = 0.0;
   __syncthreads();
   for (int i = threadIdx.x; i < NX + NU; i += blockDim.x) {
     double acc = 0.0;
@@ -1161,27 +879,93 @@ int bwd_model(const BackwardArgs& a, hipStream_t st) {
 }
 
 template <class Model, int G>
-bool fits(int N) {
-  Launch l = plan<Model, G>(N, 64);
+bool fits(int N, int lsd) {
+  Launch l = plan<Model, G>(N, Model::NPARAM, lsd);
   return l.K > 0 && l.lds <= kMaxLds;
 }
 
-}  // namespace
-
-bool fused_supported(int model, int nx, int nu, int N) {
+bool register_fits(int model, int nx, int nu, int N, int lsd) {
   if (N < 1) return false;
   switch (model) {
-    case MODEL_LINEAR: return nx == 4 && nu == 2 && fits<LinearModel<4, 2>, 4>(N);
-    case MODEL_QUADROTOR: return nx == 13 && nu == 4 && fits<QuadrotorModel, 16>(N);
-    case MODEL_SPACECRAFT3: return nx == 3 && nu == 3 && fits<Spacecraft3Model, 4>(N);
-    case MODEL_SPACECRAFT7: return nx == 7 && nu == 3 && fits<Spacecraft7Model, 8>(N);
-    case MODEL_CARTPOLE: return nx == 4 && nu == 1 && fits<CartpoleModel, 4>(N);
+    case MODEL_LINEAR: return nx == 4 && nu == 2 && fits<LinearModel<4, 2>, 4>(N, lsd);
+    case MODEL_QUADROTOR: return nx == 13 && nu == 4 && fits<QuadrotorModel, 16>(N, lsd);
+    case MODEL_SPACECRAFT3: return nx == 3 && nu == 3 && fits<Spacecraft3Model, 4>(N, lsd);
+    case MODEL_SPACECRAFT7: return nx == 7 && nu == 3 && fits<Spacecraft7Model, 8>(N, lsd);
+    case MODEL_CARTPOLE: return nx == 4 && nu == 1 && fits<CartpoleModel, 4>(N, lsd);
     default: return false;
   }
 }
 
+// register-resident kernels when the instance fits on chip (or when forced),
+// else the streamed ones
+bool use_stream(int model, int nx, int nu, int N, int lsd) {
+  if (g_fused_variant == 1) return false;
+  if (g_fused_variant == 2) return stream_supported(model, nx, nu);
+  return !register_fits(model, nx, nu, N, lsd) && stream_supported(model, nx, nu);
+}
+
+}  // namespace
+
+int g_fused_variant = 0;
+
+bool fused_supported(int model, int nx, int nu, int N) {
+  if (N < 1) return false;
+  return register_fits(model, nx, nu, N, 0) || stream_supported(model, nx, nu);
+}
+
+size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_alphas) {
+  const int lsd = linesearch_doubles(ls_eval, n_alphas, N);
+  if (N < 1 || !use_stream(model, nx, nu, N, lsd)) return 0;
+  return stream_work_stride(nx, nu, N, ls_eval, n_alphas);
+}
+
+// launches the streamed variant shares with the register-resident one
+int stream_launch_reset(const SolveArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter);
+  return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
+}
+
+template <class Model>
+int launch_linearize(const SolveArgs& a, hipStream_t st) {
+  constexpr int NZ = Model::NX + Model::NU;
+  const long long nlin = (long long)a.B * a.N * NZ;
+  prof_begin(PROF_LINEARIZE, st);
+  hipLaunchKernelGGL(linearize_kernel<Model>, dim3((unsigned)((nlin + 255) / 256)), dim3(256), 0, st, a);
+  prof_end(PROF_LINEARIZE, st);
+  return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
+}
+
+int stream_launch_linearize(int model, const SolveArgs& a, hipStream_t st) {
+  switch (model) {
+    case MODEL_LINEAR: return launch_linearize<LinearModel<4, 2>>(a, st);
+    case MODEL_QUADROTOR: return launch_linearize<QuadrotorModel>(a, st);
+    case MODEL_SPACECRAFT3: return launch_linearize<Spacecraft3Model>(a, st);
+    case MODEL_SPACECRAFT7: return launch_linearize<Spacecraft7Model>(a, st);
+    case MODEL_CARTPOLE: return launch_linearize<CartpoleModel>(a, st);
+    case MODEL_CHAIN: return launch_linearize<ChainModel>(a, st);
+    default: return DMPC_ERR_UNSUPPORTED;
+  }
+}
+
+int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st) {
+  int nx, nu;
+  switch (model) {
+    case MODEL_LINEAR: nx = 4; nu = 2; break;
+    case MODEL_QUADROTOR: nx = QuadrotorModel::NX; nu = QuadrotorModel::NU; break;
+    case MODEL_SPACECRAFT3: nx = Spacecraft3Model::NX; nu = Spacecraft3Model::NU; break;
+    case MODEL_SPACECRAFT7: nx = Spacecraft7Model::NX; nu = Spacecraft7Model::NU; break;
+    case MODEL_CARTPOLE: nx = CartpoleModel::NX; nu = CartpoleModel::NU; break;
+    case MODEL_CHAIN: nx = ChainModel::NX; nu = ChainModel::NU; break;
+    default: return DMPC_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(sqp_finish_kernel, dim3(a.B), dim3(256), 0, st, a, nx, nu);
+  return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
+}
+
 int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st) {
-  if (!fused_supported(model, nx, nu, a.N)) return DMPC_ERR_UNSUPPORTED;
+  const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
+  if (use_stream(model, nx, nu, a.N, lsd)) return stream_forward(model, nx, nu, a, st);
+  if (!register_fits(model, nx, nu, a.N, lsd)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {
     case MODEL_LINEAR: return fwd_model<LinearModel<4, 2>, 4>(a, st);
     case MODEL_QUADROTOR: return fwd_model<QuadrotorModel, 16>(a, st);
@@ -1193,7 +977,8 @@ int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st)
 }
 
 int fused_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t st) {
-  if (!fused_supported(model, nx, nu, a.N)) return DMPC_ERR_UNSUPPORTED;
+  if (use_stream(model, nx, nu, a.N, 0)) return stream_backward(model, nx, nu, a, st);
+  if (!register_fits(model, nx, nu, a.N, 0)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {
     case MODEL_LINEAR: return bwd_model<LinearModel<4, 2>, 4>(a, st);
     case MODEL_QUADROTOR: return bwd_model<QuadrotorModel, 16>(a, st);

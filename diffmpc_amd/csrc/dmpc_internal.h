@@ -28,6 +28,7 @@ struct SolveArgs {
   // problem data (device)
   const double *x0, *xf, *xref, *uref, *wx, *wu, *wf;
   const float* dyn;
+  const int32_t* horizons;  // (B,) per-instance horizon, or null (all N)
   // solution (device, in: guess and guess histories, out: solution)
   double *x, *u, *lam;
   int32_t *status, *num_iter;
@@ -38,12 +39,16 @@ struct SolveArgs {
   float* fd;      // (B, N, nx): f_d(x_t, u_t)
   double* qinv;   // (B, N+1, nx) diagonal of Qinv
   double* rinv;   // (B, nu) diagonal of Rinv (the same for every t)
+  // workspace of the streamed variant: work_stride bytes per instance
+  char* work;
+  size_t work_stride;
 };
 
 struct BackwardArgs {
   int B, N, penalize, pcg_max;
   double pcg_eps;
   const double *x0, *xf, *xref, *uref, *wx, *wu, *wf;
+  const int32_t* horizons;             // (B,) or null
   const double *x, *u;                 // solution
   const float* AB;                     // saved linearization
   const double *qinv, *rinv;
@@ -51,7 +56,24 @@ struct BackwardArgs {
   double* lam_bwd;                     // in: adjoint warm start, out: lam~
   double *g_wx, *g_wu, *g_wf, *g_x0, *g_xref, *g_uref, *g_xf;
   int32_t* pcg_iters;                  // (B,)
+  char* work;                          // streamed variant workspace
+  size_t work_stride;
 };
+
+// SQP continue rule of sqp.py:460-466 for instance b:
+// (num_iter < maxit && err > tol) || num_iter < 1.
+__device__ __forceinline__ bool sqp_active(const SolveArgs& a, int b) {
+  const int it = a.num_iter[b];
+  return (it < a.maxit && a.conv_err[b] > a.tol) || it < 1;
+}
+
+// Instance b's horizon, clamped to [1, N] so a bad entry cannot index out of
+// the padded arrays.
+__device__ __forceinline__ int horizon_of(const int32_t* h, int N, int b) {
+  if (!h) return N;
+  const int v = h[b];
+  return v < 1 ? 1 : (v > N ? N : v);
+}
 
 // fused path (dmpc_fused.hip)
 int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t s);
@@ -62,8 +84,14 @@ bool fused_supported(int model, int nx, int nu, int N);
 int plain_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t s);
 int plain_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t s);
 
-// tuning knob: thread groups handle K time blocks each (0 = automatic)
+// tuning knobs: thread groups handle K time blocks each (0 = automatic);
+// fused variant (0 automatic, 1 register-resident, 2 streamed)
 extern int g_forced_K;
+extern int g_fused_variant;
+
+// bytes of workspace per instance the fused forward / backward need (0 when
+// the register-resident variant runs)
+size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_alphas);
 
 // optional HIP-event timing of individual kernel launches (dmpc_profile_*)
 enum ProfKernel { PROF_QP_ITERATION = 0, PROF_BACKWARD = 1, PROF_LINEARIZE = 2, PROF_NKERNELS = 3 };

@@ -1,0 +1,809 @@
+// Streamed fused variant: the batched SQP solve and adjoint for any horizon
+// and nx <= 64, with the per-instance QP data in a caller-provided device
+// workspace instead of registers and LDS.
+//
+// The register-resident kernels of dmpc_fused.hip keep each time block's
+// A/B rows in registers and D_t in registers or LDS, which bounds the
+// horizon (about 64 blocks of 13 states, 128 of 7) and excludes nx > 16.
+// Here a workgroup still owns one instance and lane i of a G-lane group
+// still owns row i of a block, but the groups walk over the blocks in
+// rounds (block t = round * NG + group) and every per-block quantity lives
+// in global memory (L2 / MALL resident in practice):
+//   * [A_t | B_t] from the linearization (row-major) and its transpose AT,
+//     so that both A_t^T v (columns) and A_{t-1} v (rows) read coalesced;
+//   * D_t = theta_t^-1 in float64, transposed (DT[t][j][i] = D_t[i][j]);
+//   * the PCG vectors (each lane owns its entries; neighbours are read after
+//     a workgroup barrier).
+// The arithmetic and its order are those of the register-resident kernels
+// (pcg.py:350-445 with the factored block-Jacobi-stair preconditioner), so
+// both variants agree to rounding.  Assembly (theta_t, its Cholesky factor
+// and D_t) runs in its own kernel with one workgroup per (instance, block).
+#include <type_traits>
+
+#include "dmpc_device.h"
+#include "dmpc_internal.h"
+#include "dmpc_linesearch.h"
+#include "dmpc_models.h"
+
+namespace dmpc {
+namespace {
+
+constexpr int kStreamVectors = 11;
+enum { V_LAM = 0, V_P, V_R, V_RT, V_Y, V_YQ, V_YX, V_AP, V_YXO, V_GB, V_GAM };
+
+__host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Per-instance workspace: 11 padded vectors (N+3 blocks, zero blocks at
+// both ends), DT (N+1, nx, nx), 1/L_kk (N+1, nx), the line-search step and
+// scratch, and AT (N, nx+nu, nx) float.  Each array starts on 256 bytes.
+struct Work {
+  double* v[kStreamVectors];
+  double *DT, *ID, *DX, *DU, *LS;
+  float* AT;
+};
+
+__host__ __device__ inline size_t carve_work(char* base, int nx, int nu, int N, int ls_doubles, Work* w) {
+  size_t off = 0;
+  const size_t vbytes = align256((size_t)(N + 3) * nx * 8);
+  for (int k = 0; k < kStreamVectors; ++k) {
+    if (w) w->v[k] = reinterpret_cast<double*>(base + off);
+    off += vbytes;
+  }
+  if (w) w->DT = reinterpret_cast<double*>(base + off);
+  off += align256((size_t)(N + 1) * nx * nx * 8);
+  if (w) w->ID = reinterpret_cast<double*>(base + off);
+  off += align256((size_t)(N + 1) * nx * 8);
+  if (w) w->DX = reinterpret_cast<double*>(base + off);
+  off += align256((size_t)(N + 1) * nx * 8);
+  if (w) w->DU = reinterpret_cast<double*>(base + off);
+  off += align256((size_t)(N + 1) * nu * 8);
+  if (w) w->LS = reinterpret_cast<double*>(base + off);
+  off += align256((size_t)ls_doubles * 8);
+  if (w) w->AT = reinterpret_cast<float*>(base + off);
+  off += align256((size_t)N * (nx + nu) * nx * 4);
+  return off;
+}
+
+// Entry `lane` of block tb of a padded vector (slot tb + 1), 0 past nx.
+template <int NX>
+__device__ __forceinline__ double vget(const double* V, int tb, int lane) {
+  return lane < NX ? V[(tb + 1) * NX + lane] : 0.0;
+}
+
+// Diagonal cost model in closed form (sqp.py:160-197,
+// optimal_control_problem.py:249-323) for one instance.
+template <int NX, int NU>
+struct CostView {
+  const double *wx, *wu, *wf, *xref, *uref, *xf;  // instance rows / bases
+  int penalize, N;                                 // N: the instance's horizon
+  __device__ double qinv(int t, int i) const {
+    const double w = t < N ? wx[i] : wx[i] + wf[i];
+    return 1.0 / fmax(2.0 * w, kPsdMinEig);
+  }
+  __device__ double q(int t, int i) const {
+    return t < N ? -2.0 * wx[i] * xref[t * NX + i] : -2.0 * (wx[i] + wf[i]) * xf[i];
+  }
+  __device__ double rinv(int k) const { return 1.0 / fmax(2.0 * wu[k], kPsdMinEig); }
+  __device__ double r(int t, int k) const {
+    return (t < N && penalize) ? -2.0 * wu[k] * uref[t * NU + k] : 0.0;
+  }
+};
+
+// sum_j a[j * stride] * v_j (a in global memory), v_j in lane j of the group.
+template <int G, int M, class AT>
+__device__ __forceinline__ double gdot_mem(const AT* a, int stride, double v, int lane) {
+  AT r[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) r[j] = a[j * stride];
+  return gdot_chain<G, M>(r, v, lane, 0, 0.0);
+}
+
+// Group sum of b_k * v for the NU columns k of B_{t-1} (rows in AT), and the
+// B_{t-1} Rinv B_{t-1}^T v product (see bgram in the register variant).
+template <int G, int NX, int NU>
+__device__ __forceinline__ double bgram_mem(const float* brow, const CostView<NX, NU>& c, double v, double* quad) {
+  double g = 0.0, q = 0.0;
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const double bk = brow[k * NX];
+    const double sk = gsum<G>(bk * v);
+    const double yk = c.rinv(k) * sk;
+    g += bk * yk;
+    q += sk * yk;
+  }
+  *quad = q;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// Assembly: one workgroup of 64 threads per (instance, block t).  Thread i
+// owns row i.  theta_t (pcg.py:182-196 with As_next = -I), its Cholesky
+// factor (right-looking, the operation order of the register variant's
+// shuffle version) and D_t = theta_t^-1 by one column per thread, written
+// transposed to DT.  It also writes AT[t-1] = [A_{t-1} | B_{t-1}]^T.
+struct AssembleArgs {
+  int B, N;  // padded horizon
+  const int32_t* horizons;
+  const float* AB;
+  const double *wx, *wu, *wf;
+  // forward: skip instances that stopped iterating (null in the backward)
+  const int32_t* num_iter;
+  const double* conv_err;
+  int maxit;
+  double tol;
+  char* work;
+  size_t work_stride;
+  int ls_doubles;
+};
+
+template <int NX, int NU>
+__global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
+  constexpr int NZ = NX + NU;
+  constexpr bool kColsInLds = NX > 16;     // large blocks: substitution columns in LDS
+  __shared__ double AP[NX * NZ];          // [A_{t-1} | B_{t-1}] as float64
+  __shared__ double TH[NX * (NX + 1)];    // theta_t, then its Cholesky factor
+  __shared__ double IDs[NX];
+  __shared__ double COL[kColsInLds ? NX * 64 : 1];  // COL[j * 64 + thread]
+  const int Np = a.N, NB1 = Np + 1;
+  const long long bt = blockIdx.x;
+  const int b = (int)(bt / NB1), t = (int)(bt - (long long)b * NB1);
+  if (a.num_iter) {
+    const int it = a.num_iter[b];
+    if (!((it < a.maxit && a.conv_err[b] > a.tol) || it < 1)) return;
+  }
+  const int N = horizon_of(a.horizons, Np, b);
+  if (t > N) return;
+  Work w;
+  carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, a.ls_doubles, &w);
+  const int i = threadIdx.x;
+  const double* wx = a.wx + (size_t)b * NX;
+  const double* wu = a.wu + (size_t)b * NU;
+  const double* wf = a.wf + (size_t)b * NX;
+  auto qinv = [&](int tt, int k) {
+    const double ww = tt < N ? wx[k] : wx[k] + wf[k];
+    return 1.0 / fmax(2.0 * ww, kPsdMinEig);
+  };
+  if (t >= 1) {
+    const float* Ap = a.AB + ((size_t)b * Np + (t - 1)) * NX * NZ;
+    float* ATp = w.AT + (size_t)(t - 1) * NZ * NX;
+    for (int e = i; e < NX * NZ; e += 64) {
+      const float v = Ap[e];
+      AP[e] = v;
+      const int r = e / NZ, k = e - r * NZ;
+      ATp[k * NX + r] = v;
+    }
+  }
+  __syncthreads();
+  if (i < NX) {
+    // row i of theta_t; A_{t-1} Qinv_{t-1} row and B_{t-1} Rinv row first
+    double aq[NX], bq[NU];
+    if (t >= 1) {
+#pragma unroll
+      for (int k = 0; k < NX; ++k) aq[k] = AP[i * NZ + k] * qinv(t - 1, k);
+#pragma unroll
+      for (int k = 0; k < NU; ++k) bq[k] = AP[i * NZ + NX + k] * (1.0 / fmax(2.0 * wu[k], kPsdMinEig));
+    }
+    const double qi = qinv(t, i);
+#pragma unroll 1
+    for (int j = 0; j < NX; ++j) {
+      double acc = (j == i) ? qi : 0.0;
+      if (t >= 1) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) acc += aq[k] * AP[j * NZ + k];
+#pragma unroll
+        for (int k = 0; k < NU; ++k) acc += bq[k] * AP[j * NZ + NX + k];
+      }
+      TH[i * (NX + 1) + j] = acc;
+    }
+  }
+  __syncthreads();
+  // right-looking Cholesky: row i ends with L[i][k], k <= i
+  for (int k = 0; k < NX; ++k) {
+    const double lkk = sqrt(TH[k * (NX + 1) + k]);
+    const double il = 1.0 / lkk;
+    __syncthreads();  // every thread has read the pivot
+    if (i == k) {
+      TH[k * (NX + 1) + k] = lkk;
+      IDs[k] = il;
+    } else if (i > k && i < NX) {
+      TH[i * (NX + 1) + k] *= il;
+    }
+    __syncthreads();
+    if (i > k && i < NX) {
+      const double lik = TH[i * (NX + 1) + k];
+      for (int j = k + 1; j <= i; ++j) TH[i * (NX + 1) + j] -= lik * TH[j * (NX + 1) + k];
+    }
+    __syncthreads();
+  }
+  // column i of theta^-1 = L^-T L^-1 e_i (forward then back substitution;
+  // the back substitution overwrites the forward result in place)
+  if (i < NX) {
+    double* DTt = w.DT + (size_t)t * NX * NX;
+    if constexpr (kColsInLds) {
+      double* v = COL + i;  // v[j * 64]
+      for (int j = 0; j < NX; ++j) {
+        double acc = (j == i) ? 1.0 : 0.0;
+        for (int k = 0; k < j; ++k) acc -= TH[j * (NX + 1) + k] * v[k * 64];
+        v[j * 64] = acc * IDs[j];
+      }
+      for (int j = NX - 1; j >= 0; --j) {
+        double acc = v[j * 64];
+        for (int k = j + 1; k < NX; ++k) acc -= TH[k * (NX + 1) + j] * v[k * 64];
+        v[j * 64] = acc * IDs[j];
+      }
+      for (int j = 0; j < NX; ++j) DTt[j * NX + i] = v[j * 64];
+    } else {
+      double v[NX], d[NX];
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        double acc = (j == i) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < j; ++k) acc -= TH[j * (NX + 1) + k] * v[k];
+        v[j] = acc * IDs[j];
+      }
+#pragma unroll
+      for (int j = NX - 1; j >= 0; --j) {
+        double acc = v[j];
+#pragma unroll
+        for (int k = j + 1; k < NX; ++k) acc -= TH[k * (NX + 1) + j] * d[k];
+        d[j] = acc * IDs[j];
+      }
+#pragma unroll
+      for (int j = 0; j < NX; ++j) DTt[j * NX + i] = d[j];
+    }
+  }
+}
+
+// D_t v: sum_j D_t[lane][j] v_j (v_j in lane j of the group), 0 when !ok.
+template <int G, int NX>
+__device__ __forceinline__ double dmul_mem(const Work& w, int t, double v, int lane, bool ok) {
+  const double* col = w.DT + (size_t)t * NX * NX + (lane < NX ? lane : 0);
+  const double acc = gdot_mem<G, NX>(col, NX, v, lane);
+  return ok ? acc : 0.0;
+}
+
+// Per-instance setup shared by the forward and the backward kernels.
+template <int G, int NX, int NU>
+struct Blocks {
+  int NG, grp, lane, rounds, N;
+  __device__ void init(int Nb) {
+    NG = blockDim.x / G;
+    grp = threadIdx.x / G;
+    lane = threadIdx.x % G;
+    N = Nb;
+    rounds = (Nb + 1 + NG - 1) / NG;
+  }
+  // block of round r (clamped to N for masked lanes) and whether lane owns a row
+  __device__ int t(int r) const {
+    const int v = r * NG + grp;
+    return v > N ? N : v;
+  }
+  __device__ bool ok(int r) const { return r * NG + grp <= N && lane < NX; }
+};
+
+// PCG on S lam = gamma (pcg.py:350-445) with the vectors in the workspace.
+// gamma in V_GAM, the guess in V_LAM; the solution is left in V_LAM.  Same
+// phases, products and reductions as pcg() of the register variant.
+template <int G, int NX, int NU>
+__device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostView<NX, NU>& c, const float* ABb,
+                          double* red3, double eps, int max_iter, double* eta_out) {
+  constexpr int NZ = NX + NU;
+  double* LAM = w.v[V_LAM];
+  double* P = w.v[V_P];
+  double* R = w.v[V_R];
+  double* RT = w.v[V_RT];
+  double* Y = w.v[V_Y];
+  double* YQ = w.v[V_YQ];
+  double* YX = w.v[V_YX];
+  double* AP = w.v[V_AP];
+  double* YXO = w.v[V_YXO];
+  double* GB = w.v[V_GB];
+  const double* GAM = w.v[V_GAM];
+  const int lane = K.lane, N = K.N;
+  const int li = lane < NX ? lane : 0;
+  auto acol = [&](int t) { return ABb + (size_t)(t < N ? t : 0) * NX * NZ + li; };        // stride NZ
+  auto arow = [&](int t) { return w.AT + (size_t)(t >= 1 ? t - 1 : 0) * NZ * NX + li; };  // stride NX
+  auto has_a = [&](int t) { return t < N; };
+  auto has_p = [&](int t) { return t >= 1; };
+  // ---- r = gamma - S lam0 ----------------------------------------------
+  for (int r = 0; r < K.rounds; ++r) {
+    const int t = K.t(r);
+    const bool ok = K.ok(r);
+    const double qi = ok ? c.qinv(t, lane) : 0.0;
+    const double pv = vget<NX>(LAM, t, lane);
+    const double pn = vget<NX>(LAM, t + 1, lane);
+    const double ax = gdot_mem<G, NX>(acol(t), NZ, pn, lane);
+    double q;
+    const double gb = bgram_mem<G, NX, NU>(arow(t) + NX * NX, c, pv, &q);
+    const double wx = ((t == 0) ? pv : -pv) + ((ok && has_a(t)) ? ax : 0.0);
+    const double yxo = qi * wx;
+    if (ok) {
+      YX[(t + 1) * NX + lane] = yxo;
+      YXO[(t + 1) * NX + lane] = yxo;
+      GB[(t + 1) * NX + lane] = has_p(t) ? gb : 0.0;
+      AP[(t + 1) * NX + lane] = 0.0;
+      P[(t + 1) * NX + lane] = 0.0;
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r < K.rounds; ++r) {
+    const int t = K.t(r);
+    const bool ok = K.ok(r);
+    const double u = vget<NX>(YX, t - 1, lane);
+    const double au = gdot_mem<G, NX>(arow(t), NX, u, lane);
+    const double g = (t == 0 ? 1.0 : -1.0) * vget<NX>(YXO, t, lane) + ((ok && has_p(t)) ? au : 0.0) +
+                     vget<NX>(GB, t, lane);
+    const double rv = ok ? vget<NX>(GAM, t, lane) + g : 0.0;
+    const double y = dmul_mem<G, NX>(w, t, rv, lane, ok);
+    if (ok) {
+      R[(t + 1) * NX + lane] = rv;
+      Y[(t + 1) * NX + lane] = y;
+      YQ[(t + 1) * NX + lane] = c.qinv(t, lane) * y;
+    }
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (int r = 0; r < K.rounds; ++r) {
+    const int t = K.t(r);
+    const bool ok = K.ok(r);
+    const double a1v = gdot_mem<G, NX>(arow(t), NX, vget<NX>(YQ, t - 1, lane), lane);
+    const double a2v = gdot_mem<G, NX>(acol(t), NZ, vget<NX>(Y, t + 1, lane), lane);
+    const double a1 = (ok && has_p(t)) ? a1v : 0.0, a2 = (ok && has_a(t)) ? a2v : 0.0;
+    const double rv = vget<NX>(R, t, lane);
+    const double qi = ok ? c.qinv(t, lane) : 0.0;
+    const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
+    const double rt = dmul_mem<G, NX>(w, t, z, lane, ok);
+    if (ok) {
+      part += rv * rt;
+      RT[(t + 1) * NX + lane] = rt;
+    }
+  }
+  double eta = block_sum(part, red3 + 32);
+  double beta = 0.0;
+  int it = 0;
+  while (true) {
+    const bool cont = (fabs(eta) > eps && it <= max_iter - 1) || it < 1;
+    if (!cont) break;
+    // ---- phase A: p = rt + beta p; w = G^T p; y = H^-1 w; v = -w.y ------
+    double vpart = 0.0;
+    for (int r = 0; r < K.rounds; ++r) {
+      const int t = K.t(r);
+      const bool ok = K.ok(r);
+      const double pv = ok ? vget<NX>(RT, t, lane) + beta * vget<NX>(P, t, lane) : 0.0;
+      const double rn = vget<NX>(RT, t + 1, lane);
+      const double arn = gdot_mem<G, NX>(acol(t), NZ, rn, lane);
+      const double apv = ok ? ((has_a(t) ? arn : 0.0) + beta * vget<NX>(AP, t, lane)) : 0.0;
+      double q;
+      const double gb = bgram_mem<G, NX, NU>(arow(t) + NX * NX, c, pv, &q);
+      const double qi = ok ? c.qinv(t, lane) : 0.0;
+      const double wx = ((t == 0) ? pv : -pv) + apv;
+      const double yxo = qi * wx;
+      if (ok) {
+        P[(t + 1) * NX + lane] = pv;
+        AP[(t + 1) * NX + lane] = apv;
+        YX[(t + 1) * NX + lane] = yxo;
+        YXO[(t + 1) * NX + lane] = yxo;
+        GB[(t + 1) * NX + lane] = has_p(t) ? gb : 0.0;
+        vpart += wx * yxo;
+        if (lane == 0 && has_p(t)) vpart += q;
+      }
+    }
+    const double v = -block_sum(vpart, red3);
+    const double alpha = eta / v;
+    // ---- phase B: ups = S p; lam += alpha p; r -= alpha ups; y = D r -----
+    for (int r = 0; r < K.rounds; ++r) {
+      const int t = K.t(r);
+      const bool ok = K.ok(r);
+      const double u = vget<NX>(YX, t - 1, lane);
+      const double au = gdot_mem<G, NX>(arow(t), NX, u, lane);
+      const double g = (t == 0 ? 1.0 : -1.0) * vget<NX>(YXO, t, lane) + ((ok && has_p(t)) ? au : 0.0) +
+                       vget<NX>(GB, t, lane);
+      double rv = vget<NX>(R, t, lane);
+      if (ok) {
+        LAM[(t + 1) * NX + lane] = vget<NX>(LAM, t, lane) + alpha * vget<NX>(P, t, lane);
+        rv += alpha * g;
+      }
+      const double y = dmul_mem<G, NX>(w, t, rv, lane, ok);
+      if (ok) {
+        R[(t + 1) * NX + lane] = rv;
+        Y[(t + 1) * NX + lane] = y;
+        YQ[(t + 1) * NX + lane] = c.qinv(t, lane) * y;
+      }
+    }
+    __syncthreads();
+    // ---- phase C: rt = Phi^-1 r; eta' = r.rt --------------------------------
+    double epart = 0.0;
+    for (int r = 0; r < K.rounds; ++r) {
+      const int t = K.t(r);
+      const bool ok = K.ok(r);
+      const double a1v = gdot_mem<G, NX>(arow(t), NX, vget<NX>(YQ, t - 1, lane), lane);
+      const double a2v = gdot_mem<G, NX>(acol(t), NZ, vget<NX>(Y, t + 1, lane), lane);
+      const double a1 = (ok && has_p(t)) ? a1v : 0.0, a2 = (ok && has_a(t)) ? a2v : 0.0;
+      const double rv = vget<NX>(R, t, lane);
+      const double qi = ok ? c.qinv(t, lane) : 0.0;
+      const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
+      const double rt = dmul_mem<G, NX>(w, t, z, lane, ok);
+      if (ok) {
+        epart += rv * rt;
+        RT[(t + 1) * NX + lane] = rt;
+      }
+    }
+    const double eta_new = block_sum(epart, red3 + 16);
+    beta = eta_new / eta;
+    eta = eta_new;
+    ++it;
+  }
+  __syncthreads();
+  *eta_out = eta;
+  return it;
+}
+
+// x_t and u_{t-1} of block t (pcg.py:316-348) from V_LAM, see recover_rows.
+template <int G, int NX, int NU>
+__device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r, const Work& w,
+                                               const CostView<NX, NU>& c, const float* ABb, double* xn,
+                                               double* un) {
+  constexpr int NZ = NX + NU;
+  const int t = K.t(r), lane = K.lane, N = K.N;
+  const bool ok = K.ok(r);
+  const int li = lane < NX ? lane : 0;
+  const double* LAM = w.v[V_LAM];
+  const double lt = vget<NX>(LAM, t, lane);
+  const double ln = vget<NX>(LAM, t + 1, lane);
+  const double ax = gdot_mem<G, NX>(ABb + (size_t)(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
+  const float* brow = w.AT + (size_t)(t >= 1 ? t - 1 : 0) * NZ * NX + NX * NX + li;
+  double bl = 0.0;
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const double sk = gsum<G>((double)brow[k * NX] * lt);
+    bl = (lane == k) ? sk : bl;
+  }
+  *xn = 0.0;
+  *un = 0.0;
+  if (ok) {
+    *xn = -c.qinv(t, lane) * (c.q(t, lane) + (t == 0 ? lt : -lt) + (t < N ? ax : 0.0));
+    if (lane < NU && t >= 1) *un = -c.rinv(lane) * (c.r(t - 1, lane) + bl);
+  }
+}
+
+__device__ __forceinline__ int ctl_row(int t, int N) { return t == 0 ? N : t - 1; }
+
+// Clears the padded vectors and loads the multiplier guess.
+template <int NX>
+__device__ void stream_init_vectors(const Work& w, int Np, int N, const double* lam_guess) {
+  const int n = (Np + 3) * NX;
+  for (int k = 0; k < kStreamVectors; ++k)
+    for (int e = threadIdx.x; e < n; e += blockDim.x) w.v[k][e] = 0.0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < (N + 1) * NX; e += blockDim.x) w.v[V_LAM][NX + e] = lam_guess[e];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// One SQP iteration of one instance (streamed): gamma, PCG, recovery, line
+// search, step test and outputs.  The block matrices were assembled by
+// stream_assemble_kernel for this linearization.
+template <class Model, int G>
+__global__ void __launch_bounds__(512) stream_qp_kernel(SolveArgs a) {
+  constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
+  __shared__ double red3[48];
+  const int b = blockIdx.x;
+  if (!sqp_active(a, b)) return;
+  const int it = a.num_iter[b];
+  const int Np = a.N, NBs = Np + 1;
+  const int N = horizon_of(a.horizons, Np, b), NB = N + 1;
+  const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, Np);
+  Work w;
+  carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, lsd, &w);
+  Blocks<G, NX, NU> K;
+  K.init(N);
+  CostView<NX, NU> c{a.wx + (size_t)b * NX, a.wu + (size_t)b * NU, a.wf + (size_t)b * NX,
+                     a.xref + (size_t)b * NBs * NX, a.uref + (size_t)b * NBs * NU, a.xf + (size_t)b * NX,
+                     a.penalize, N};
+  const double* xb = a.x + (size_t)b * NBs * NX;
+  const double* ub = a.u + (size_t)b * NBs * NU;
+  const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
+  const float* fdb = a.fd + (size_t)b * Np * NX;
+  const double* x0 = a.x0 + (size_t)b * NX;
+  stream_init_vectors<NX>(w, Np, N, a.lam + (size_t)b * NBs * NX);
+  // gamma = C + G H^-1 q (pcg.py:245-288), C = [x0; -f_d + A xbar + B ubar]
+  for (int r = 0; r < K.rounds; ++r) {
+    const int t = K.t(r), lane = K.lane;
+    if (!K.ok(r)) continue;
+    double g;
+    if (t == 0) {
+      g = c.qinv(0, lane) * c.q(0, lane) + x0[lane];
+    } else {
+      const int tp = t - 1;
+      g = -c.qinv(t, lane) * c.q(t, lane) - (double)fdb[tp * NX + lane];
+      const float* Ar = ABb + (size_t)tp * NX * NZ + lane * NZ;
+      for (int k = 0; k < NX; ++k) g += (double)Ar[k] * (c.qinv(tp, k) * c.q(tp, k) + xb[tp * NX + k]);
+      for (int k = 0; k < NU; ++k) g += (double)Ar[NX + k] * (c.rinv(k) * c.r(tp, k) + ub[tp * NU + k]);
+    }
+    w.v[V_GAM][(t + 1) * NX + lane] = g;
+  }
+  __syncthreads();
+  double eta;
+  const int npcg = stream_pcg<G, NX, NU>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta);
+  // recovery: the QP solution into XN / UN (free PCG vectors) and the step
+  // to it into DX / DU; with a line search the new iterate is
+  // xbar + alpha * step, otherwise the QP solution itself
+  double* XN = w.v[V_P];
+  double* UN = w.v[V_AP];  // (N+3) nx >= (N+1) nu doubles
+  for (int r = 0; r < K.rounds; ++r) {
+    double xn, un;
+    stream_recover<G, NX, NU>(K, r, w, c, ABb, &xn, &un);
+    if (K.ok(r)) {
+      const int t = K.t(r), lane = K.lane, tu = ctl_row(t, N);
+      XN[t * NX + lane] = xn;
+      w.DX[t * NX + lane] = xn - xb[t * NX + lane];
+      if (lane < NU) {
+        UN[tu * NU + lane] = un;
+        w.DU[tu * NU + lane] = un - ub[tu * NU + lane];
+      }
+    }
+  }
+  __syncthreads();
+  double alpha = 1.0;
+  if (a.ls_eval) {
+    alpha = line_search<Model>(a, b, N, xb, ub, w.DX, w.DU, w.LS, red3);
+    for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) XN[k] = xb[k] + alpha * w.DX[k];
+    for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) UN[k] = ub[k] + alpha * w.DU[k];
+  }
+  double e = 0.0;
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) e = nanmax(e, fabs(XN[k] - xb[k]));
+  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) e = nanmax(e, fabs(UN[k] - ub[k]));
+  const double err = block_max(e, red3);  // every read of the old iterate precedes this barrier
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
+    a.x[(size_t)b * NBs * NX + k] = XN[k];
+    a.lam[(size_t)b * NBs * NX + k] = w.v[V_LAM][NX + k];
+  }
+  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) a.u[(size_t)b * NBs * NU + k] = UN[k];
+  for (int k = NB * NX + threadIdx.x; k < NBs * NX; k += blockDim.x) {
+    a.x[(size_t)b * NBs * NX + k] = 0.0;
+    a.lam[(size_t)b * NBs * NX + k] = 0.0;
+  }
+  for (int k = NB * NU + threadIdx.x; k < NBs * NU; k += blockDim.x) a.u[(size_t)b * NBs * NU + k] = 0.0;
+  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) a.qinv[(size_t)b * NBs * NX + k] = c.qinv(k / NX, k % NX);
+  for (int k = threadIdx.x; k < NU; k += blockDim.x) a.rinv[(size_t)b * NU + k] = c.rinv(k);
+  if (threadIdx.x == 0) {
+    a.conv_errs[(size_t)b * a.maxit + it] = err;
+    a.pcg_iters[(size_t)b * a.maxit + it] = npcg;
+    if (a.linesearch) a.ls_alphas[(size_t)b * a.maxit + it] = alpha;
+    a.conv_err[b] = err;
+    a.num_iter[b] = it + 1;
+  }
+}
+
+// Implicit-differentiation backward (sqp.py:605-798), streamed: adjoint PCG
+// with q = -dl/dx, r = -dl/du[:-1], C = 0 on the saved linearization, then
+// the closed-form contraction of the register variant's backward.
+template <class Model, int G>
+__global__ void __launch_bounds__(512) stream_bwd_kernel(BackwardArgs a) {
+  constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
+  __shared__ double red3[48];
+  const int b = blockIdx.x;
+  const int Np = a.N, NBs = Np + 1;
+  const int N = horizon_of(a.horizons, Np, b), NB = N + 1;
+  Work w;
+  carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, 0, &w);
+  Blocks<G, NX, NU> K;
+  K.init(N);
+  // cost view of the adjoint system: q~ = -dl/dx and r~ = -dl/du replace
+  // q and r; Qinv and Rinv are those of the solve (weights unchanged)
+  CostView<NX, NU> c{a.wx + (size_t)b * NX, a.wu + (size_t)b * NU, a.wf + (size_t)b * NX,
+                     a.xref + (size_t)b * NBs * NX, a.uref + (size_t)b * NBs * NU, a.xf + (size_t)b * NX,
+                     a.penalize, N};
+  const double* dlx = a.dl_dx + (size_t)b * NBs * NX;
+  const double* dlu = a.dl_du + (size_t)b * NBs * NU;
+  const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
+  stream_init_vectors<NX>(w, Np, N, a.lam_bwd + (size_t)b * NBs * NX);
+  // gamma = G H^-1 q~ (C = 0)
+  for (int r = 0; r < K.rounds; ++r) {
+    const int t = K.t(r), lane = K.lane;
+    if (!K.ok(r)) continue;
+    double g;
+    if (t == 0) {
+      g = c.qinv(0, lane) * -dlx[lane];
+    } else {
+      const int tp = t - 1;
+      g = -c.qinv(t, lane) * -dlx[t * NX + lane];
+      const float* Ar = ABb + (size_t)tp * NX * NZ + lane * NZ;
+      for (int k = 0; k < NX; ++k) g += (double)Ar[k] * (c.qinv(tp, k) * -dlx[tp * NX + k]);
+      for (int k = 0; k < NU; ++k) g += (double)Ar[NX + k] * (c.rinv(k) * -dlu[tp * NU + k]);
+    }
+    w.v[V_GAM][(t + 1) * NX + lane] = g;
+  }
+  __syncthreads();
+  double eta;
+  const int npcg = stream_pcg<G, NX, NU>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta);
+  // recovery of x~, u~ with q~, r~; per-row contraction terms into DX / DU
+  const double* xs = a.x + (size_t)b * NBs * NX;
+  const double* us = a.u + (size_t)b * NBs * NU;
+  for (int r = 0; r < K.rounds; ++r) {
+    const int t = K.t(r), lane = K.lane;
+    const bool ok = K.ok(r);
+    const int li = lane < NX ? lane : 0;
+    const double* LAM = w.v[V_LAM];
+    const double lt = vget<NX>(LAM, t, lane);
+    const double ln = vget<NX>(LAM, t + 1, lane);
+    const double ax = gdot_mem<G, NX>(ABb + (size_t)(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
+    const float* brow = w.AT + (size_t)(t >= 1 ? t - 1 : 0) * NZ * NX + NX * NX + li;
+    double bl = 0.0;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) {
+      const double sk = gsum<G>((double)brow[k * NX] * lt);
+      bl = (lane == k) ? sk : bl;
+    }
+    if (!ok) continue;
+    const double xt = -c.qinv(t, lane) * (-dlx[t * NX + lane] + (t == 0 ? lt : -lt) + (t < N ? ax : 0.0));
+    const int i = lane;
+    const double ref = (t < N) ? c.xref[t * NX + i] : c.xf[i];
+    const double ex = xs[t * NX + i] - ref;
+    w.DX[t * NX + i] = 2.0 * ex * xt;
+    const double wxi = c.wx[i];
+    a.g_xref[((size_t)b * NBs + t) * NX + i] = (t < N) ? 2.0 * wxi * xt : 0.0;
+    if (t == N) {
+      a.g_wf[b * NX + i] = -2.0 * ex * xt;
+      a.g_xf[b * NX + i] = 2.0 * (wxi + c.wf[i]) * xt;
+    }
+    if (i < NU) {
+      const int tu = ctl_row(t, N);
+      const double ut = (t >= 1) ? -c.rinv(i) * (-dlu[(t - 1) * NU + i] + bl) : 0.0;
+      const double ur = a.penalize ? c.uref[tu * NU + i] : 0.0;
+      w.DU[tu * NU + i] = (tu < N) ? 2.0 * (us[tu * NU + i] - ur) * ut : 0.0;
+      a.g_uref[((size_t)b * NBs + tu) * NU + i] = (tu < N && a.penalize) ? 2.0 * c.wu[i] * ut : 0.0;
+    }
+    a.lam_bwd[((size_t)b * NBs + t) * NX + i] = lt;
+    if (t == 0) a.g_x0[b * NX + i] = lt;
+  }
+  for (int k = NB * NX + threadIdx.x; k < NBs * NX; k += blockDim.x) {
+    a.g_xref[(size_t)b * NBs * NX + k] = 0.0;
+    a.lam_bwd[(size_t)b * NBs * NX + k] = 0.0;
+  }
+  for (int k = NB * NU + threadIdx.x; k < NBs * NU; k += blockDim.x) a.g_uref[(size_t)b * NBs * NU + k] = 0.0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < NX + NU; i += blockDim.x) {
+    double acc = 0.0;
+    if (i < NX) {
+      for (int t = 0; t <= N; ++t) acc += w.DX[t * NX + i];
+      a.g_wx[b * NX + i] = -acc;
+    } else {
+      const int k = i - NX;
+      for (int t = 0; t < N; ++t) acc += w.DU[t * NU + k];
+      a.g_wu[b * NU + k] = -acc;
+    }
+  }
+  if (threadIdx.x == 0) a.pcg_iters[b] = npcg;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+inline int stream_threads(int N, int G) {
+  const int want = (N + 1) * G;
+  const int t = ((want + 63) / 64) * 64;
+  return t < 64 ? 64 : (t > 512 ? 512 : t);
+}
+
+template <class Model>
+AssembleArgs assemble_args(const SolveArgs& a, int lsd) {
+  AssembleArgs s;
+  s.B = a.B;
+  s.N = a.N;
+  s.horizons = a.horizons;
+  s.AB = a.AB;
+  s.wx = a.wx;
+  s.wu = a.wu;
+  s.wf = a.wf;
+  s.num_iter = a.num_iter;
+  s.conv_err = a.conv_err;
+  s.maxit = a.maxit;
+  s.tol = a.tol;
+  s.work = a.work;
+  s.work_stride = a.work_stride;
+  s.ls_doubles = lsd;
+  return s;
+}
+
+}  // namespace
+
+size_t stream_work_stride(int nx, int nu, int N, int ls_eval, int n_alphas) {
+  return carve_work(nullptr, nx, nu, N, linesearch_doubles(ls_eval, n_alphas, N), nullptr);
+}
+
+// the linearization and reset/finish kernels are shared with the register
+// variant (dmpc_fused.hip)
+int stream_launch_linearize(int model, const SolveArgs& a, hipStream_t st);
+int stream_launch_reset(const SolveArgs& a, hipStream_t st);
+int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st);
+
+template <class Model, int G>
+int stream_forward_t(int model, const SolveArgs& a, hipStream_t st) {
+  constexpr int NX = Model::NX, NU = Model::NU;
+  const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
+  const AssembleArgs aa = assemble_args<Model>(a, lsd);
+  const int threads = stream_threads(a.N, G);
+  int rc = stream_launch_reset(a, st);
+  if (rc) return rc;
+  for (int it = 0; it < a.maxit; ++it) {
+    rc = stream_launch_linearize(model, a, st);
+    if (rc) return rc;
+    prof_begin(PROF_QP_ITERATION, st);
+    hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0,
+                       st, aa);
+    hipLaunchKernelGGL((stream_qp_kernel<Model, G>), dim3(a.B), dim3(threads), 0, st, a);
+    prof_end(PROF_QP_ITERATION, st);
+  }
+  rc = stream_launch_finish(model, a, st);
+  if (rc) return rc;
+  return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
+}
+
+template <class Model, int G>
+int stream_backward_t(const BackwardArgs& a, hipStream_t st) {
+  constexpr int NX = Model::NX, NU = Model::NU;
+  AssembleArgs aa;
+  aa.B = a.B;
+  aa.N = a.N;
+  aa.horizons = a.horizons;
+  aa.AB = a.AB;
+  aa.wx = a.wx;
+  aa.wu = a.wu;
+  aa.wf = a.wf;
+  aa.num_iter = nullptr;
+  aa.conv_err = nullptr;
+  aa.maxit = 0;
+  aa.tol = 0.0;
+  aa.work = a.work;
+  aa.work_stride = a.work_stride;
+  aa.ls_doubles = 0;
+  prof_begin(PROF_BACKWARD, st);
+  hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0, st,
+                     aa);
+  hipLaunchKernelGGL((stream_bwd_kernel<Model, G>), dim3(a.B), dim3(stream_threads(a.N, G)), 0, st, a);
+  prof_end(PROF_BACKWARD, st);
+  return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
+}
+
+bool stream_supported(int model, int nx, int nu) {
+  switch (model) {
+    case MODEL_LINEAR: return nx == 4 && nu == 2;
+    case MODEL_QUADROTOR: return nx == 13 && nu == 4;
+    case MODEL_SPACECRAFT3: return nx == 3 && nu == 3;
+    case MODEL_SPACECRAFT7: return nx == 7 && nu == 3;
+    case MODEL_CARTPOLE: return nx == 4 && nu == 1;
+    case MODEL_CHAIN: return nx == 48 && nu == 12;
+    default: return false;
+  }
+}
+
+int stream_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st) {
+  if (!stream_supported(model, nx, nu)) return DMPC_ERR_UNSUPPORTED;
+  if (!a.work) return DMPC_ERR_ARG;
+  switch (model) {
+    case MODEL_LINEAR: return stream_forward_t<LinearModel<4, 2>, 4>(model, a, st);
+    case MODEL_QUADROTOR: return stream_forward_t<QuadrotorModel, 16>(model, a, st);
+    case MODEL_SPACECRAFT3: return stream_forward_t<Spacecraft3Model, 4>(model, a, st);
+    case MODEL_SPACECRAFT7: return stream_forward_t<Spacecraft7Model, 8>(model, a, st);
+    case MODEL_CARTPOLE: return stream_forward_t<CartpoleModel, 4>(model, a, st);
+    case MODEL_CHAIN: return stream_forward_t<ChainModel, 64>(model, a, st);
+    default: return DMPC_ERR_UNSUPPORTED;
+  }
+}
+
+int stream_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t st) {
+  if (!stream_supported(model, nx, nu)) return DMPC_ERR_UNSUPPORTED;
+  if (!a.work) return DMPC_ERR_ARG;
+  switch (model) {
+    case MODEL_LINEAR: return stream_backward_t<LinearModel<4, 2>, 4>(a, st);
+    case MODEL_QUADROTOR: return stream_backward_t<QuadrotorModel, 16>(a, st);
+    case MODEL_SPACECRAFT3: return stream_backward_t<Spacecraft3Model, 4>(a, st);
+    case MODEL_SPACECRAFT7: return stream_backward_t<Spacecraft7Model, 8>(a, st);
+    case MODEL_CARTPOLE: return stream_backward_t<CartpoleModel, 4>(a, st);
+    case MODEL_CHAIN: return stream_backward_t<ChainModel, 64>(a, st);
+    default: return DMPC_ERR_UNSUPPORTED;
+  }
+}
+
+}  // namespace dmpc

@@ -81,6 +81,8 @@ class OptimalControlProblem:
         x0 = _as_tensor(params["initial_state"], device)
         xf = _as_tensor(params["final_state"], device)
         N = self.horizon
+        if params.get("horizons") is not None:
+            return self._ragged_initial_guess(x0, xf, params["horizons"])
         t = torch.arange(N + 1, dtype=torch.float64, device=x0.device).unsqueeze(-1)
         a1 = (N - t) / N
         a2 = t / N
@@ -93,6 +95,21 @@ class OptimalControlProblem:
         else:
             states = x0 * a1 + xf * a2 + 1e-6
             controls = torch.zeros(N + 1, self.num_control_variables, dtype=torch.float64, device=x0.device) + 1e-6
+        return states, controls
+
+    def _ragged_initial_guess(self, x0, xf, horizons):
+        """The straight-line guess over each instance's own horizon N_b (knots
+        0..N_b); knots past N_b of the padded (B, N+1, .) arrays are zero."""
+        N = self.horizon
+        x0 = x0 if x0.dim() == 2 else x0.unsqueeze(0)
+        xf = xf if xf.dim() == 2 else xf.unsqueeze(0)
+        B = max(x0.shape[0], xf.shape[0])
+        Nb = torch.as_tensor(horizons, device=x0.device).reshape(B, 1, 1).to(torch.float64)
+        t = torch.arange(N + 1, dtype=torch.float64, device=x0.device).reshape(1, N + 1, 1)
+        inside = (t <= Nb).to(torch.float64)
+        states = (x0.unsqueeze(1) * ((Nb - t) / Nb) + xf.unsqueeze(1) * (t / Nb) + 1e-6) * inside
+        controls = (torch.zeros(B, N + 1, self.num_control_variables, dtype=torch.float64, device=x0.device)
+                    + 1e-6) * inside
         return states, controls
 
     def equality_constraints(self, states, controls, params: Dict[str, Any]) -> torch.Tensor:

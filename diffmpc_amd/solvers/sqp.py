@@ -93,6 +93,21 @@ def _c_solver_params(params: Dict[str, Any]) -> "_lib.SolverParams":
     return sp
 
 
+def _ragged_horizons(h, B, N, device):
+    """Per-instance horizons of a ragged batch (problem_params["horizons"],
+    an extension with no reference counterpart): (B,) ints in [1, N], where
+    N (the problem's horizon) is the padded length of every per-knot array."""
+    if h is None:
+        return None
+    h = torch.as_tensor(h).reshape(-1)
+    if h.numel() != B:
+        raise ValueError(f"horizons must have one entry per instance ({B}), got {h.numel()}")
+    hc = h.detach().cpu()
+    if int(hc.min()) < 1 or int(hc.max()) > N:
+        raise ValueError(f"horizons must lie in [1, {N}]")
+    return h.to(device=device, dtype=torch.int32).contiguous()
+
+
 def _mode(params) -> int:
     m = params.get("mode", "fused")
     if m not in ("fused", "plain"):
@@ -104,14 +119,14 @@ class _SQPSolveFn(torch.autograd.Function):
     """custom_vjp of sqp.py:562-802 as a torch.autograd.Function."""
 
     @staticmethod
-    def forward(ctx, solver, guess, x0, x_final, x_ref, u_ref, w_x, w_u, w_f, dyn):
+    def forward(ctx, solver, guess, x0, x_final, x_ref, u_ref, w_x, w_u, w_f, dyn, horizons):
         lib = _lib.load()
         prog = solver.program
         B = guess["states"].shape[0]
         dev = guess["states"].device
         N, nx, nu = prog.horizon, prog.num_state_variables, prog.num_control_variables
-        maxit = solver.maxiter
         data_t = [t.contiguous() for t in (x0, x_final, x_ref, u_ref, w_x, w_u, w_f)] + [dyn.contiguous()]
+        data_t.append(horizons)
         data = _lib.ProblemData(*[_lib.ptr(t) for t in data_t])
         x = guess["states"].to(torch.float64).clone().contiguous()
         u = guess["controls"].to(torch.float64).clone().contiguous()
@@ -128,10 +143,13 @@ class _SQPSolveFn(torch.autograd.Function):
         rinv = torch.zeros(B, nu, dtype=torch.float64, device=dev)
         sol = _lib.Solution(_lib.ptr(x), _lib.ptr(u), _lib.ptr(lam), _lib.ptr(status), _lib.ptr(num_iter),
                             _lib.ptr(conv_err), _lib.ptr(conv_errs), _lib.ptr(ls_alphas), _lib.ptr(pcg_iters))
-        saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv))
         prob = prog.c_problem(B)
         sp = _c_solver_params(solver.params)
         mode = _mode(solver.params)
+        nwork = _lib.workspace_bytes(mode, prob, sp)
+        work = torch.empty(max(nwork, 1), dtype=torch.uint8, device=dev)
+        saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv),
+                           _lib.ptr(work) if nwork else None, nwork)
         rc = lib.dmpc_sqp_solve(prob, sp, data, sol, saved, mode, _lib.stream_ptr(dev))
         _lib.check(rc, "dmpc_sqp_solve")
         dual_bwd_in = guess.get("dual_backward_guess")
@@ -140,7 +158,7 @@ class _SQPSolveFn(torch.autograd.Function):
         ctx.solver = solver
         ctx.prob_args = (prob, sp, mode)
         ctx.data_t = data_t
-        ctx.saved_lin = (AB, fd, qinv, rinv)
+        ctx.saved_lin = (AB, fd, qinv, rinv, work, nwork)
         ctx.dual_bwd_in = dual_bwd_in
         ctx.dual_bwd_out = dual_bwd_out
         ctx.save_for_backward(x, u)
@@ -174,8 +192,9 @@ class _SQPSolveFn(torch.autograd.Function):
                            _lib.ptr(g["x_ref"]), _lib.ptr(g["u_ref"]), _lib.ptr(g["x_final"]), _lib.ptr(lam_b),
                            _lib.ptr(iters))
         data = _lib.ProblemData(*[_lib.ptr(t) for t in ctx.data_t])
-        AB, fd, qinv, rinv = ctx.saved_lin
-        saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv))
+        AB, fd, qinv, rinv, work, nwork = ctx.saved_lin
+        saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv),
+                           _lib.ptr(work) if nwork else None, nwork)
         rc = lib.dmpc_sqp_backward(prob, sp, data, _lib.ptr(x), _lib.ptr(u), saved, _lib.ptr(dl_dx), _lib.ptr(dl_du),
                                    grads, mode, _lib.stream_ptr(dev))
         _lib.check(rc, "dmpc_sqp_backward")
@@ -183,7 +202,7 @@ class _SQPSolveFn(torch.autograd.Function):
             with torch.no_grad():
                 ctx.dual_bwd_in.copy_(lam_b.to(ctx.dual_bwd_in.dtype))
         ctx.solver.last_backward_pcg_iters = iters
-        return (None, None, g["x0"], g["x_final"], g["x_ref"], g["u_ref"], g["w_x"], g["w_u"], g["w_f"], None)
+        return (None, None, g["x0"], g["x_final"], g["x_ref"], g["u_ref"], g["w_x"], g["w_u"], g["w_f"], None, None)
 
 
 class SQPSolver:
@@ -329,7 +348,8 @@ class SQPSolver:
         wf = b("weights_penalization_final_state", 1)
         dyn = prog.dynamics.flat_params({k: (v.detach() if isinstance(v, torch.Tensor) else v)
                                          for k, v in params.items()}, B, device)
-        out = _SQPSolveFn.apply(self, g, x0, xf, xref, uref, wx, wu, wf, dyn)
+        horizons = _ragged_horizons(params.get("horizons"), B, N, device)
+        out = _SQPSolveFn.apply(self, g, x0, xf, xref, uref, wx, wu, wf, dyn, horizons)
         (x, u, lam, status, num_iter, conv_err, conv_errs, ls_alphas, pcg_iters, AB, fd, qinv, rinv,
          dual_bwd_out) = out
         saved = SavedLinearization(AB, fd, qinv, rinv)

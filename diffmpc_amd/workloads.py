@@ -146,6 +146,39 @@ def chain(batch=512, seed=4, horizon=256, pcg_eps=1e-8):
     return "chain", prob, solver_params(1, 0.0, pcg_eps)
 
 
+def ragged(batch=8192, seed=3, tmin=10, tmax=200, sqp_iters=3):
+    """Ragged mixed batch: horizons N ~ U{tmin..tmax} and systems drawn
+    uniformly from {LTI nx=4/nu=2, spacecraft attitude nx=7/nu=3, quadrotor
+    nx=13/nu=4}, each with the seeded per-instance x0 and weights of its own
+    workload generator (lti, spacecraft7, quadrotor), `sqp_iters` SQP
+    iterations.  Returns {name: (prob, solver_params, horizons)} with one
+    group per system, padded to the group's longest horizon."""
+    rng = np.random.default_rng(seed)
+    kinds = rng.integers(0, 3, size=batch)
+    hor = rng.integers(tmin, tmax + 1, size=batch)
+    out = {}
+    for k, gen in enumerate((lti, spacecraft7, quadrotor)):
+        sel = np.nonzero(kinds == k)[0]
+        if sel.size == 0:
+            continue
+        h = hor[sel].astype(np.int32)
+        name, prob, sp = gen(batch=int(sel.size), seed=seed * 10 + k, horizon=int(h.max()))
+        sp = dict(sp)
+        sp["num_scp_iteration_max"] = sqp_iters
+        sp["tol_convergence"] = 0.0
+        out[name] = (prob, sp, h)
+    return out
+
+
+def instance_problem(prob, b, n):
+    """Instance b of a padded (ragged) problem dict, cut to horizon n."""
+    one = slice_problem(prob, np.array([b]))
+    one["horizon"] = int(n)
+    for k in ("x_ref", "u_ref"):
+        one[k] = one[k][:, : n + 1]
+    return one
+
+
 def slice_problem(prob, idx):
     """Sub-batch of a problem dict (array entries with a batch axis)."""
     out = {}
@@ -176,10 +209,11 @@ def dynamics_params(name, dyn, nx=4, nu=2):
     raise ValueError(name)
 
 
-def make_solver(name, prob, sp, device="cuda", mode="fused"):
+def make_solver(name, prob, sp, device="cuda", mode="fused", horizons=None):
     """(solver, problem_params, guess) for a workload dict, on `device`.
 
-    problem_params holds batched torch tensors under the reference's keys.
+    problem_params holds batched torch tensors under the reference's keys;
+    `horizons` (B,) makes it a ragged batch padded to prob["horizon"].
     """
     import torch
 
@@ -208,6 +242,8 @@ def make_solver(name, prob, sp, device="cuda", mode="fused"):
     )
     for k, v in dynamics_params(name, prob["dyn"], nx, nu).items():
         params[k] = v.to(device)
+    if horizons is not None:
+        params["horizons"] = torch.as_tensor(np.asarray(horizons), dtype=torch.int32, device=device)
     sp = dict(sp)
     sp["pcg"] = dict(sp["pcg"])
     sp["mode"] = mode

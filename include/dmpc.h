@@ -74,6 +74,13 @@ typedef struct dmpc_problem_data {
   const double* w_u;       /* (B, nu)      weights_penalization_control_squared */
   const double* w_f;       /* (B, nx)      weights_penalization_final_state */
   const float* dyn_params; /* (B, n_params) */
+  /* Optional (NULL = every instance has `horizon`).  Ragged batches:
+   * horizons[b] in [1, horizon] is instance b's own horizon N_b; every
+   * per-knot array keeps the padded stride horizon + 1 and instance b uses
+   * its first N_b + 1 knots (N_b for the saved linearization).  Outputs past
+   * N_b are zero.  No counterpart in the reference, which solves one horizon
+   * per compiled solve. */
+  const int32_t* horizons; /* (B,) */
 } dmpc_problem_data;
 
 typedef struct dmpc_solution { /* SQPSolution fields, sqp.py:38-57 */
@@ -93,6 +100,11 @@ typedef struct dmpc_saved { /* QP data of the last linearization point (sqp.py:4
   float* fd;    /* (B, N, nx): f_d(x_t, u_t) */
   double* qinv; /* (B, N+1, nx): diagonal of Qinv_t */
   double* rinv; /* (B, nu): diagonal of Rinv_t (the same for every t) */
+  /* Device workspace of dmpc_workspace_bytes() bytes (NULL when that is 0):
+   * the streamed fused variant keeps its vectors and preconditioner blocks
+   * there.  Contents need not survive between calls. */
+  void* work;
+  uint64_t work_bytes;
 } dmpc_saved;
 
 typedef struct dmpc_grads { /* d loss / d input, per instance */
@@ -147,8 +159,17 @@ int dmpc_pcg_recover(int32_t B, int32_t N, int32_t nx, int32_t nu, const double*
                      const double* As, const double* Bs, const double* Qinv, const double* qs, const double* Rinv,
                      const double* rs, double* states, double* controls, void* stream);
 
+/* Bytes of dmpc_saved.work that dmpc_sqp_solve / dmpc_sqp_backward need for
+ * this problem and mode (0 when the fused kernels keep everything on chip).
+ * No reference counterpart (the reference allocates inside XLA). */
+int dmpc_workspace_bytes(int32_t mode, const dmpc_problem* prob, const dmpc_solver_params* params, uint64_t* bytes);
+
 /* Tuning: time blocks per thread group of the fused kernels (0 = automatic). */
 int dmpc_set_blocks_per_group(int32_t k);
+
+/* Tuning: fused variant, 0 = automatic (register-resident blocks when they fit
+ * on chip, else streamed), 1 = register-resident only, 2 = streamed only. */
+int dmpc_set_fused_variant(int32_t v);
 
 /* Instrumentation (no reference counterpart): when enabled, the library
  * records a HIP event pair on the launch stream around every launch of the

@@ -369,3 +369,122 @@ def test_spacecraft_attitude_full_batch_gradients_match_oracle_on_sampled_instan
         e = rel(_np(got[k])[idx], expect[k])
         say(f"  {k}: rel {e:.2e}")
         assert e <= 1e-4, k
+
+
+class _variant:
+    """Forces a fused kernel variant (1 register-resident, 2 streamed)."""
+
+    def __init__(self, v):
+        from diffmpc_amd import _lib
+
+        self.lib, self.v, self._lib = _lib.load(), v, _lib
+
+    def __enter__(self):
+        self._lib.check(self.lib.dmpc_set_fused_variant(self.v), "dmpc_set_fused_variant")
+
+    def __exit__(self, *exc):
+        self.lib.dmpc_set_fused_variant(0)
+
+
+def _solve_and_grad(name, prob, sp, cuda, horizons=None):
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda, mode="fused", horizons=horizons)
+    wx = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
+    x0 = params["initial_state"].clone().requires_grad_(True)
+    sol = solver.solve(guess, params, {"weights_penalization_reference_state_trajectory": wx, "initial_state": x0})
+    loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+    gwx, gx0 = torch.autograd.grad(loss, [wx, x0])
+    torch.cuda.synchronize()
+    return sol, gwx, gx0
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_streamed_variant_matches_oracle(cuda, case):
+    """The streamed fused variant (vectors and blocks in the workspace) on the
+    parity cases, forward and backward."""
+    say(f"[streamed] case={case}")
+    name, prob, sp = CASES[case]()
+    with _variant(2):
+        sol, gwx, gx0 = _solve_and_grad(name, prob, sp, cuda)
+    op = oracle_problem(name, prob)
+    ref = osqp.sqp_solve(op, sp)
+    g = osqp.sqp_backward(op, sp, ref, ref["states"] - prob["x_ref"], np.zeros_like(ref["controls"]))
+    say("  rel states", rel(_np(sol.states), ref["states"]), "w_x", rel(_np(gwx), g["w_x"]))
+    assert rel(_np(sol.states), ref["states"]) <= 1e-5
+    assert rel(_np(sol.controls), ref["controls"]) <= 1e-5
+    assert rel(_np(sol.dual), ref["dual"]) <= 1e-5
+    assert np.max(np.abs(_np(sol.num_pcg_iters) - ref["num_pcg_iters"])) <= 1
+    assert rel(_np(gwx), g["w_x"]) <= 1e-4
+    assert rel(_np(gx0), g["x0"]) <= 1e-4
+
+
+def test_streamed_and_register_variants_agree(cuda):
+    say("[streamed-vs-register] quadrotor")
+    name, prob, sp = CASES["quadrotor"]()
+    with _variant(1):
+        s1, g1, _ = _solve_and_grad(name, prob, sp, cuda)
+    with _variant(2):
+        s2, g2, _ = _solve_and_grad(name, prob, sp, cuda)
+    e = rel(_np(s2.states), _np(s1.states))
+    say("  rel states", e, "grad", rel(_np(g2), _np(g1)))
+    assert e <= 1e-8
+    assert rel(_np(g2), _np(g1)) <= 1e-7
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_ragged_batch_matches_oracle_per_instance(cuda, variant):
+    """A ragged mixed batch (per-instance horizons, three systems): each
+    instance equals the oracle run at its own horizon; knots past it are 0."""
+    say(f"[ragged] variant={variant}")
+    groups = W.ragged(batch=18, seed=5, tmin=4, tmax=30)
+    for name, (prob, sp, h) in groups.items():
+        with _variant(variant):
+            sol, gwx, gx0 = _solve_and_grad(name, prob, sp, cuda, horizons=h)
+        xs, us, gw, g0 = _np(sol.states), _np(sol.controls), _np(gwx), _np(gx0)
+        for b, n in enumerate(h):
+            one = oracle_problem(name, W.instance_problem(prob, b, n))
+            ref = osqp.sqp_solve(one, sp)
+            g = osqp.sqp_backward(one, sp, ref, ref["states"] - one["x_ref"], np.zeros_like(ref["controls"]))
+            e = rel(xs[b : b + 1, : n + 1], ref["states"])
+            say(f"  {name} b={b} N={n}: rel states {e:.2e}")
+            assert e <= 1e-5
+            assert rel(us[b : b + 1, : n + 1], ref["controls"]) <= 1e-5
+            assert np.all(xs[b, n + 1 :] == 0.0) and np.all(us[b, n + 1 :] == 0.0)
+            assert rel(gw[b : b + 1], g["w_x"]) <= 1e-4
+            assert rel(g0[b : b + 1], g["x0"]) <= 1e-4
+
+
+def test_long_horizon_runs_streamed_and_matches_oracle(cuda):
+    """Quadrotor with N = 150: too long for the register-resident kernels, so
+    the automatic choice is the streamed variant."""
+    say("[long] quadrotor N=150")
+    from diffmpc_amd import _lib
+
+    assert _lib.load().dmpc_supported(0, 1, 13, 4, 150) == 1
+    name, prob, sp = W.quadrotor(batch=2, seed=9, horizon=150, sqp_iters=3)
+    sol, gwx, _ = _solve_and_grad(name, prob, sp, cuda)
+    op = oracle_problem(name, prob)
+    ref = osqp.sqp_solve(op, sp)
+    g = osqp.sqp_backward(op, sp, ref, ref["states"] - prob["x_ref"], np.zeros_like(ref["controls"]))
+    say("  rel states", rel(_np(sol.states), ref["states"]))
+    assert rel(_np(sol.states), ref["states"]) <= 1e-5
+    assert rel(_np(gwx), g["w_x"]) <= 1e-4
+
+
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+def test_chain_matches_oracle(cuda, mode):
+    """48-state spring-damper chain (linear, one SQP iteration is the QP
+    solution): fused runs the streamed variant (64-lane groups)."""
+    say(f"[chain] mode={mode}")
+    name, prob, sp = W.chain(batch=2, seed=4, horizon=24)
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda, mode=mode)
+    wx = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
+    sol = solver.solve(guess, params, {"weights_penalization_reference_state_trajectory": wx})
+    (gwx,) = torch.autograd.grad(0.5 * sol.states.square().sum(), [wx])
+    torch.cuda.synchronize()
+    op = oracle_problem(name, prob)
+    ref = osqp.sqp_solve(op, sp)
+    g = osqp.sqp_backward(op, sp, ref, ref["states"], np.zeros_like(ref["controls"]))
+    say("  rel states", rel(_np(sol.states), ref["states"]), "w_x", rel(_np(gwx), g["w_x"]))
+    assert rel(_np(sol.states), ref["states"]) <= 1e-5
+    assert rel(_np(sol.controls), ref["controls"]) <= 1e-5
+    assert rel(_np(gwx), g["w_x"]) <= 1e-4
