@@ -49,7 +49,8 @@ class SolverParams(ctypes.Structure):
         ("linesearch", ctypes.c_int32), ("linesearch_eta", ctypes.c_double),
         ("n_alphas", ctypes.c_int32), ("linesearch_alphas", ctypes.c_double * 8),
         ("pcg_max_iter", ctypes.c_int32), ("pcg_tol_epsilon", ctypes.c_double),
-        ("warm_start_backward", ctypes.c_int32),
+        ("warm_start_backward", ctypes.c_int32), ("blocks_per_group", ctypes.c_int32),
+        ("fused_variant", ctypes.c_int32),
     ]
 
 

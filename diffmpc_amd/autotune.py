@@ -1,0 +1,79 @@
+"""Autotuner for the fused kernels' launch shape.
+
+A fused solve can run as the register-resident kernels with K = 1, 2 or 4
+time blocks per lane group (K trades threads per instance against registers
+per thread: the on-chip tiling of the block-tridiagonal system), or as the
+streamed variant that keeps the blocks in a device workspace.  ``tune`` times
+every candidate the library accepts on the caller's own problem (forward
+plus backward, HIP events on the current stream), writes the fastest into the
+solver's params (``fused_variant``, ``blocks_per_group``) and caches it per
+(model, nx, nu, horizon, batch, line search) key.  No reference counterpart:
+XLA picks its own launch shapes.
+"""
+from typing import Dict, List, Tuple
+
+import torch
+
+from . import _lib
+
+CANDIDATES: List[Tuple[str, int]] = [("register", 1), ("register", 2), ("register", 4), ("streamed", 0)]
+
+_cache: Dict[tuple, Tuple[str, int]] = {}
+
+
+def _key(solver, B):
+    prog, p = solver.program, solver.params
+    return (prog.dynamics.model_id, prog.num_state_variables, prog.num_control_variables, prog.horizon, B,
+            bool(p["linesearch"]), tuple(p["linesearch_alphas"]), p["num_scp_iteration_max"])
+
+
+def _apply(solver, choice):
+    solver.params["fused_variant"], solver.params["blocks_per_group"] = choice
+
+
+def tune(solver, params, guess, reps: int = 2, backward: bool = True, candidates=CANDIDATES):
+    """Pick the fastest fused launch shape for (solver, params); returns
+    (choice, {choice: ms per fwd(+bwd)}).  Cached choices are applied without
+    timing (the timings dict is then empty)."""
+    B = guess.states.shape[0] if guess.states.dim() == 3 else 1
+    key = _key(solver, B)
+    if key in _cache:
+        _apply(solver, _cache[key])
+        return _cache[key], {}
+    wkey = "weights_penalization_reference_state_trajectory"
+
+    def run():
+        if guess.dual_backward_guess is not None:
+            guess.dual_backward_guess.zero_()
+        if not backward:
+            with torch.no_grad():
+                return solver.solve(guess, params)
+        w = params[wkey].detach().clone().requires_grad_(True)
+        sol = solver.solve(guess, params, {wkey: w})
+        torch.autograd.grad(sol.states.square().sum(), [w])
+        return sol
+
+    timings = {}
+    saved = (solver.params.get("fused_variant", "auto"), solver.params.get("blocks_per_group", 0))
+    for choice in candidates:
+        _apply(solver, choice)
+        try:
+            run()
+        except _lib.DmpcError as e:
+            if "unsupported" in str(e):
+                continue
+            raise
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            run()
+        end.record()
+        end.synchronize()
+        timings[choice] = start.elapsed_time(end) / reps
+    if not timings:
+        _apply(solver, saved)
+        raise _lib.DmpcError("no fused launch shape supports this problem")
+    best = min(timings, key=timings.get)
+    _apply(solver, best)
+    _cache[key] = best
+    return best, timings

@@ -84,6 +84,11 @@ int check_problem(const dmpc_problem* p) {
   return DMPC_OK;
 }
 
+bool tuning_ok(const dmpc_solver_params* s) {
+  const int k = s->blocks_per_group, v = s->fused_variant;
+  return (k == 0 || k == 1 || k == 2 || k == 4) && v >= 0 && v <= 2;
+}
+
 bool linesearch_is_noop(const dmpc_solver_params* s) {
   // with a single candidate alpha = 1 the merit test cannot change the step
   // (sqp.py:389-403 always returns alpha = 1)
@@ -148,7 +153,8 @@ int dmpc_workspace_bytes(int32_t mode, const dmpc_problem* prob, const dmpc_solv
   if (mode == DMPC_MODE_PLAIN) return DMPC_OK;
   if (mode != DMPC_MODE_FUSED) return DMPC_ERR_ARG;
   const int ls_eval = params->linesearch && !linesearch_is_noop(params);
-  *bytes = (uint64_t)fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, ls_eval, params->n_alphas) *
+  *bytes = (uint64_t)fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, ls_eval, params->n_alphas,
+                                       params->fused_variant) *
            (uint64_t)prob->batch;
   return DMPC_OK;
 }
@@ -168,6 +174,7 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   if (params->num_scp_iteration_max < 1 || params->pcg_max_iter < 1) return DMPC_ERR_ARG;
   if (params->n_alphas < 0 || params->n_alphas > kMaxAlphas) return DMPC_ERR_ARG;
   if (params->linesearch && (params->n_alphas < 1 || params->n_alphas > 8)) return DMPC_ERR_ARG;
+  if (!tuning_ok(params)) return DMPC_ERR_ARG;
   SolveArgs a;
   std::memset(&a, 0, sizeof(a));
   a.B = prob->batch;
@@ -175,6 +182,8 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   a.P = prob->n_params;
   a.scheme = prob->scheme;
   a.penalize = prob->penalize_control_reference;
+  a.forced_K = params->blocks_per_group;
+  a.variant = params->fused_variant;
   a.dt = (float)prob->dt;
   a.dt64 = prob->dt;
   a.maxit = params->num_scp_iteration_max;
@@ -210,7 +219,7 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   a.rinv = saved->rinv;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (mode == DMPC_MODE_FUSED) {
-    a.work_stride = fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, a.ls_eval, a.n_alphas);
+    a.work_stride = fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, a.ls_eval, a.n_alphas, a.variant);
     if (a.work_stride) {
       if (!saved->work || saved->work_bytes < (uint64_t)a.work_stride * (uint64_t)a.B) return DMPC_ERR_ARG;
       a.work = static_cast<char*>(saved->work);
@@ -231,6 +240,7 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
   if (rc) return rc;
   if (!params || !data || !states || !controls || !saved || !dl_dstates || !dl_dcontrols || !g) return DMPC_ERR_ARG;
   if (!saved->AB || !saved->qinv || !saved->rinv) return DMPC_ERR_ARG;
+  if (!tuning_ok(params)) return DMPC_ERR_ARG;
   if (!g->w_x || !g->w_u || !g->w_f || !g->x0 || !g->x_ref || !g->u_ref || !g->x_final || !g->dual_backward ||
       !g->pcg_iters)
     return DMPC_ERR_ARG;
@@ -242,6 +252,8 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
   a.penalize = prob->penalize_control_reference;
   a.pcg_max = params->pcg_max_iter;
   a.pcg_eps = params->pcg_tol_epsilon;
+  a.forced_K = params->blocks_per_group;
+  a.variant = params->fused_variant;
   a.x0 = data->x0;
   a.xf = data->x_final;
   a.xref = data->x_ref;
@@ -268,7 +280,7 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
   a.pcg_iters = g->pcg_iters;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (mode == DMPC_MODE_FUSED) {
-    a.work_stride = fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, 0, 0);
+    a.work_stride = fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, 0, 0, a.variant);
     if (a.work_stride) {
       if (!saved->work || saved->work_bytes < (uint64_t)a.work_stride * (uint64_t)a.B) return DMPC_ERR_ARG;
       a.work = static_cast<char*>(saved->work);

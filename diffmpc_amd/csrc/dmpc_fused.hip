@@ -794,7 +794,7 @@ struct Launch {
 inline int max_threads(int K) { return K == 1 ? 1024 : (K == 2 ? 512 : 256); }
 
 template <class Model, int G>
-Launch plan(int N, int P, int ls = 0) {
+Launch plan(int N, int P, int ls = 0, int forced_K = 0) {
   constexpr int NX = Model::NX, NU = Model::NU;
   const int NB = N + 1;
   Launch l{0, 0, LdsPlan<NX, NU>::bytes(N, P, ls)};
@@ -803,7 +803,8 @@ Launch plan(int N, int P, int ls = 0) {
   const int cand[3] = {2, 1, 4};
   for (int c = 0; c < 3; ++c) {
     const int K = cand[c];
-    if (g_forced_K > 0 && K != g_forced_K) continue;
+    const int want = forced_K > 0 ? forced_K : g_forced_K;
+    if (want > 0 && K != want) continue;
     const int groups = (NB + K - 1) / K;
     const int threads = ((groups * G + 63) / 64) * 64;
     if (threads <= max_threads(K)) {
@@ -856,7 +857,7 @@ int launch_bwd_k(const BackwardArgs& a, const Launch& l, hipStream_t st) {
 
 template <class Model, int G>
 int fwd_model(const SolveArgs& a, hipStream_t st) {
-  Launch l = plan<Model, G>(a.N, a.P, linesearch_doubles(a.ls_eval, a.n_alphas, a.N));
+  Launch l = plan<Model, G>(a.N, a.P, linesearch_doubles(a.ls_eval, a.n_alphas, a.N), a.forced_K);
   if (l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
   switch (l.K) {
     case 1: return launch_fwd_k<Model, G, 1, 1024>(a, l, st);
@@ -868,7 +869,7 @@ int fwd_model(const SolveArgs& a, hipStream_t st) {
 
 template <class Model, int G>
 int bwd_model(const BackwardArgs& a, hipStream_t st) {
-  Launch l = plan<Model, G>(a.N, 0);
+  Launch l = plan<Model, G>(a.N, 0, 0, a.forced_K);
   if (l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
   switch (l.K) {
     case 1: return launch_bwd_k<Model, G, 1, 1024>(a, l, st);
@@ -880,7 +881,7 @@ int bwd_model(const BackwardArgs& a, hipStream_t st) {
 
 template <class Model, int G>
 bool fits(int N, int lsd) {
-  Launch l = plan<Model, G>(N, Model::NPARAM, lsd);
+  Launch l = plan<Model, G>(N, Model::NPARAM, lsd, 0);
   return l.K > 0 && l.lds <= kMaxLds;
 }
 
@@ -898,9 +899,10 @@ bool register_fits(int model, int nx, int nu, int N, int lsd) {
 
 // register-resident kernels when the instance fits on chip (or when forced),
 // else the streamed ones
-bool use_stream(int model, int nx, int nu, int N, int lsd) {
-  if (g_fused_variant == 1) return false;
-  if (g_fused_variant == 2) return stream_supported(model, nx, nu);
+bool use_stream(int model, int nx, int nu, int N, int lsd, int variant) {
+  const int v = variant > 0 ? variant : g_fused_variant;
+  if (v == 1) return false;
+  if (v == 2) return stream_supported(model, nx, nu);
   return !register_fits(model, nx, nu, N, lsd) && stream_supported(model, nx, nu);
 }
 
@@ -913,9 +915,9 @@ bool fused_supported(int model, int nx, int nu, int N) {
   return register_fits(model, nx, nu, N, 0) || stream_supported(model, nx, nu);
 }
 
-size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_alphas) {
+size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_alphas, int variant) {
   const int lsd = linesearch_doubles(ls_eval, n_alphas, N);
-  if (N < 1 || !use_stream(model, nx, nu, N, lsd)) return 0;
+  if (N < 1 || !use_stream(model, nx, nu, N, lsd, variant)) return 0;
   return stream_work_stride(nx, nu, N, ls_eval, n_alphas);
 }
 
@@ -964,7 +966,7 @@ int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st) {
 
 int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st) {
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
-  if (use_stream(model, nx, nu, a.N, lsd)) return stream_forward(model, nx, nu, a, st);
+  if (use_stream(model, nx, nu, a.N, lsd, a.variant)) return stream_forward(model, nx, nu, a, st);
   if (!register_fits(model, nx, nu, a.N, lsd)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {
     case MODEL_LINEAR: return fwd_model<LinearModel<4, 2>, 4>(a, st);
@@ -977,7 +979,7 @@ int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st)
 }
 
 int fused_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t st) {
-  if (use_stream(model, nx, nu, a.N, 0)) return stream_backward(model, nx, nu, a, st);
+  if (use_stream(model, nx, nu, a.N, 0, a.variant)) return stream_backward(model, nx, nu, a, st);
   if (!register_fits(model, nx, nu, a.N, 0)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {
     case MODEL_LINEAR: return bwd_model<LinearModel<4, 2>, 4>(a, st);

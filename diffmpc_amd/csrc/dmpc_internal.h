@@ -14,6 +14,7 @@ constexpr int kMaxAlphas = 8;
 // Everything one batched SQP solve needs, flattened for kernel arguments.
 struct SolveArgs {
   int B, N, P, scheme, penalize;
+  int forced_K, variant;  // per-call tuning (0 = library default)
   float dt;       // step of the float32 linearization
   double dt64;    // step of the float64 line-search evaluations
   int maxit;
@@ -46,6 +47,7 @@ struct SolveArgs {
 
 struct BackwardArgs {
   int B, N, penalize, pcg_max;
+  int forced_K, variant;  // per-call tuning (0 = library default)
   double pcg_eps;
   const double *x0, *xf, *xref, *uref, *wx, *wu, *wf;
   const int32_t* horizons;             // (B,) or null
@@ -91,7 +93,7 @@ extern int g_fused_variant;
 
 // bytes of workspace per instance the fused forward / backward need (0 when
 // the register-resident variant runs)
-size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_alphas);
+size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_alphas, int variant);
 
 // optional HIP-event timing of individual kernel launches (dmpc_profile_*)
 enum ProfKernel { PROF_QP_ITERATION = 0, PROF_BACKWARD = 1, PROF_LINEARIZE = 2, PROF_NKERNELS = 3 };

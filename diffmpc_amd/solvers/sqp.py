@@ -13,10 +13,12 @@ that follow from the batch axis being explicit here instead of a caller's
   only (problem_params are stop-gradient, sqp.py:456-458); shared weights get
   the batch-summed gradient, like ``vmap(in_axes=None)``.
 
-The solve runs in libdmpc.so: mode "fused" (default) is one persistent
-workgroup per instance per SQP iteration, "plain" is the per-launch op graph
-of the reference.  Solver parameters use the reference's keys plus the
-optional ``"mode"``.
+The solve runs in libdmpc.so: mode "fused" (default) is one workgroup per
+instance per SQP iteration, "plain" is the per-launch op graph of the
+reference.  Solver parameters use the reference's keys plus the optional
+``"mode"``, ``"fused_variant"`` ("auto", "register", "streamed") and
+``"blocks_per_group"`` (0 = automatic, 1, 2, 4), which diffmpc_amd.autotune
+sets.  ``problem_params["horizons"]`` (B,) makes a ragged batch.
 """
 import copy
 import enum
@@ -90,6 +92,9 @@ def _c_solver_params(params: Dict[str, Any]) -> "_lib.SolverParams":
     sp.pcg_max_iter = int(params["pcg"]["max_iter"])
     sp.pcg_tol_epsilon = float(params["pcg"]["tol_epsilon"])
     sp.warm_start_backward = int(bool(params["warm_start_backward"]))
+    # fused-mode tuning (keys of this build; 0 = library default)
+    sp.blocks_per_group = int(params.get("blocks_per_group", 0))
+    sp.fused_variant = {"auto": 0, "register": 1, "streamed": 2}[params.get("fused_variant", "auto")]
     return sp
 
 
@@ -102,9 +107,12 @@ def _ragged_horizons(h, B, N, device):
     h = torch.as_tensor(h).reshape(-1)
     if h.numel() != B:
         raise ValueError(f"horizons must have one entry per instance ({B}), got {h.numel()}")
-    hc = h.detach().cpu()
-    if int(hc.min()) < 1 or int(hc.max()) > N:
-        raise ValueError(f"horizons must lie in [1, {N}]")
+    if not (h.is_cuda and torch.cuda.is_current_stream_capturing()):
+        # (while a graph is being captured the values cannot be read back; the
+        # kernels clamp them to [1, N])
+        hc = h.detach().cpu()
+        if int(hc.min()) < 1 or int(hc.max()) > N:
+            raise ValueError(f"horizons must lie in [1, {N}]")
     return h.to(device=device, dtype=torch.int32).contiguous()
 
 
@@ -321,7 +329,14 @@ class SQPSolver:
             t = t.reshape(-1, t.shape[-1]) if t.dim() >= 1 else t
             return t.expand(B, self.maxiter).to(dtype) if t.shape[0] == 1 else t.to(dtype)
 
-        ce = torch.as_tensor(initial_guess.convergence_error, dtype=torch.float64, device=device).reshape(-1)
+        cev = initial_guess.convergence_error
+        if isinstance(cev, torch.Tensor) and cev.is_cuda:
+            ce = cev.to(torch.float64).reshape(-1)
+        elif isinstance(cev, (int, float)):
+            # a device fill, not a host-to-device copy (graph-capture safe)
+            ce = torch.full((B,), float(cev), dtype=torch.float64, device=device)
+        else:
+            ce = torch.as_tensor(cev, dtype=torch.float64, device=device).reshape(-1)
         g["convergence_error"] = ce.expand(B) if ce.numel() == 1 else ce
         g["convergence_errors"] = hist(initial_guess.convergence_errors, torch.float64)
         g["linesearch_alphas"] = hist(initial_guess.linesearch_alphas, torch.float64)

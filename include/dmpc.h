@@ -63,6 +63,12 @@ typedef struct dmpc_solver_params { /* keys of diffmpc/solvers/params/sqp.yaml *
   int32_t pcg_max_iter;
   double pcg_tol_epsilon;
   int32_t warm_start_backward;
+  /* Fused-mode tuning for this call (no reference counterpart; 0 = the
+   * library default, see dmpc_set_blocks_per_group / dmpc_set_fused_variant):
+   * time blocks per lane group of the register-resident kernels (1, 2, 4) and
+   * the kernel variant (1 register-resident, 2 streamed). */
+  int32_t blocks_per_group;
+  int32_t fused_variant;
 } dmpc_solver_params;
 
 typedef struct dmpc_problem_data {

@@ -488,3 +488,66 @@ def test_chain_matches_oracle(cuda, mode):
     assert rel(_np(sol.states), ref["states"]) <= 1e-5
     assert rel(_np(sol.controls), ref["controls"]) <= 1e-5
     assert rel(_np(gwx), g["w_x"]) <= 1e-4
+
+
+def test_autotune_picks_a_supported_shape_and_keeps_results(cuda):
+    from diffmpc_amd import autotune
+
+    say("[autotune] quadrotor B=64")
+    name, prob, sp = W.quadrotor(batch=64, seed=3, horizon=30, sqp_iters=3)
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    best, timings = autotune.tune(solver, params, guess, reps=1)
+    say("  timings", timings, "best", best)
+    assert best in timings and len(timings) >= 2
+    with torch.no_grad():
+        sol = solver.solve(guess, params)
+    ref = osqp.sqp_solve(oracle_problem(name, W.slice_problem(prob, np.arange(4))), sp)
+    assert rel(_np(sol.states)[:4], ref["states"]) <= 1e-5
+    assert autotune.tune(solver, params, guess)[0] == best  # cached
+
+
+def test_graph_captured_ragged_step_matches_eager_on_every_replay(cuda):
+    """A ragged mixed batch (three systems, per-instance horizons), forward and
+    backward, captured as one hipGraph; every replay equals the eager step,
+    with unrelated eager work and synchronisations between replays, and a
+    replay after an in-place change of the initial states equals the eager
+    step on the changed inputs."""
+    from diffmpc_amd.graph import GraphedStep
+
+    say("[graph] ragged fwd+bwd")
+    groups = []
+    for name, (prob, sp, h) in W.ragged(batch=24, seed=8, tmin=5, tmax=40).items():
+        solver, params, guess = W.make_solver(name, prob, sp, device=cuda, horizons=h)
+        w = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
+        groups.append((solver, params, guess, w))
+
+    def step():
+        outs = []
+        for solver, params, guess, w in groups:
+            guess.dual_backward_guess.zero_()
+            sol = solver.solve(guess, params, {"weights_penalization_reference_state_trajectory": w})
+            loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+            (g,) = torch.autograd.grad(loss, [w])
+            outs.append((sol.states, g))
+        return outs
+
+    def snap(outs):
+        return [(s.detach().clone(), g.detach().clone()) for s, g in outs]
+
+    eager = snap(step())
+    graphed = GraphedStep(step)
+    junk = torch.randn(512, 512, device=cuda)
+    for k in range(3):
+        got = snap(graphed.replay())
+        torch.cuda.synchronize()
+        for (se, ge), (sg, gg) in zip(eager, got):
+            assert torch.equal(se, sg) and torch.equal(ge, gg), f"replay {k}"
+        junk = junk @ junk.T / 512.0  # eager work between replays
+        torch.cuda.synchronize()
+    for _, params, _, _ in groups:
+        params["initial_state"].mul_(0.5)
+    got = snap(graphed.replay())
+    eager2 = snap(step())
+    for (se, ge), (sg, gg) in zip(eager2, got):
+        assert torch.equal(se, sg) and torch.equal(ge, gg)
+    assert not all(torch.equal(a[0], b[0]) for a, b in zip(eager, eager2))
