@@ -43,6 +43,9 @@ def main():
                     help="time only the fused path (for profiler runs); prints the fused figures")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device-to-device once (reference kernel for HBM counters)")
+    ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain"], default="quadrotor",
+                    help="quadrotor: the headline line (default); ragged: mixed batch of 8192 with horizons "
+                         "U{10..200} in one graph; chain: 48-state chain B=512 N=256")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -65,6 +68,11 @@ def main():
 
     from diffmpc_amd import _lib
     from diffmpc_amd import workloads as W
+
+    if args.workload == "ragged":
+        return bench_ragged(args)
+    if args.workload == "chain":
+        return bench_chain(args)
 
     dev = torch.device("cuda:0")
     name, prob, sp = W.quadrotor(batch=args.batch, seed=1, horizon=args.horizon)
@@ -196,6 +204,128 @@ def main():
             "solves_per_s": m / (t1 - t0),
         }
     print(json.dumps(out), flush=True)
+
+
+def _fwd_bwd_step(groups, wkey="weights_penalization_reference_state_trajectory"):
+    """One step over (solver, params, guess, weight) groups: solve, tracking
+    loss 0.5 ||x* - x_ref||^2, gradient w.r.t. the state weights."""
+    import torch
+
+    def step():
+        outs = []
+        for solver, params, guess, w in groups:
+            guess.dual_backward_guess.zero_()
+            sol = solver.solve(guess, params, {wkey: w})
+            loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+            (g,) = torch.autograd.grad(loss, [w])
+            outs.append((sol.states, g))
+        return outs
+
+    return step
+
+
+def bench_ragged(args):
+    """Config 3: 8192 instances, horizons N ~ U{10..200}, systems drawn from
+    {LTI 4/2, spacecraft attitude 7/3, quadrotor 13/4}, 3 SQP iterations;
+    the autotuner picks each group's launch shape, then forward + backward of
+    all groups is captured as one hipGraph and replayed."""
+    import torch
+
+    from diffmpc_amd import autotune
+    from diffmpc_amd import workloads as W
+    from diffmpc_amd.graph import GraphedStep
+
+    from diffmpc_amd import _lib
+
+    dev = torch.device("cuda:0")
+    groups, info = [], {}
+    model_ids = {"linear": (_lib.MODEL_LINEAR, 4, 2), "spacecraft7": (_lib.MODEL_SPACECRAFT7, 7, 3),
+                 "quadrotor": (_lib.MODEL_QUADROTOR, 13, 4)}
+    for name, (prob, sp, h) in W.ragged(batch=8192, seed=3).items():
+        # instances short enough for the register-resident kernels form their own group
+        limit = autotune.register_horizon_limit(*model_ids[name])
+        for part, (sub, hs) in enumerate(W.split_by_horizon(prob, h, limit)):
+            solver, params, guess = W.make_solver(name, sub, sp, device=dev, horizons=hs)
+            best, timings = autotune.tune(solver, params, guess, reps=1)
+            w = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
+            groups.append((solver, params, guess, w))
+            info[f"{name}/{part}"] = {"instances": int(len(hs)), "max_horizon": int(hs.max()),
+                                      "launch_shape": list(best),
+                                      "tuning_ms": {f"{k[0]}/{k[1]}": v for k, v in timings.items()}}
+    graphed = GraphedStep(_fwd_bwd_step(groups))
+    for _ in range(args.warmup):
+        graphed.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graphed.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    B = sum(v["instances"] for v in info.values())
+    print(json.dumps({
+        "metric": "ragged mixed batch fwd+bwd grads/s (8192 instances, N ~ U{10..200}, 3 SQP iterations)",
+        "value": B / dt, "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 dynamics/linearization, f64 Schur/PCG", "data": "synthetic",
+        "config": {"workload": "ragged LTI/spacecraft7/quadrotor mix, one hipGraph per step", "global_batch": B},
+        "groups": info,
+    }), flush=True)
+
+
+def bench_chain(args):
+    """Config 4: 12-mass chain, nx=48 nu=12, N=256, B=512, one SQP iteration
+    (linear system), PCG eps 1e-8; fused (streamed variant) vs the per-launch
+    path, forward + backward."""
+    import torch
+
+    from diffmpc_amd import _lib
+    from diffmpc_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    name, prob, sp = W.chain(batch=512, seed=4)
+    N, nx, nu = prob["horizon"], 48, 12
+    res = {}
+    for mode in ("fused", "plain"):
+        solver, params, guess = W.make_solver(name, prob, sp, device=dev, mode=mode)
+        w = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
+        step = _fwd_bwd_step([(solver, params, guess, w)])
+        step()
+        torch.cuda.synchronize()
+        if mode == "fused":
+            _lib.profile_read(_lib.PROF_QP_ITERATION)
+            _lib.load().dmpc_profile_enable(1)
+        n = args.steps if mode == "fused" else 1
+        t0 = time.perf_counter()
+        for _ in range(n):
+            out = step()
+        torch.cuda.synchronize()
+        res[mode] = (time.perf_counter() - t0) / n
+        if mode == "fused":
+            _lib.load().dmpc_profile_enable(0)
+            qp_ms, qp_n = _lib.profile_read(_lib.PROF_QP_ITERATION)
+            with torch.no_grad():
+                sol = solver.solve(guess, params)
+            iters = int(sol.num_pcg_iters.sum())
+    # algorithmic bytes per PCG iteration and block of the streamed kernel:
+    # A_t and A_{t-1} twice each (float), B_{t-1} once (float), D_t twice (double)
+    per_block = 4 * nx * nx * 4 + nx * nu * 4 + 2 * nx * nx * 8
+    launch_ms = qp_ms / max(qp_n, 1)
+    achieved = per_block * (N + 1) * iters / (launch_ms * 1e-3) / 1e9
+    B = prob["x0"].shape[0]
+    print(json.dumps({
+        "metric": "chain fwd+bwd grads/s (nx=48 nu=12 N=256 B=512), fused vs plain GPU",
+        "value": B / res["fused"], "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": res["fused"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 dynamics/linearization, f64 Schur/PCG", "data": "synthetic",
+        "config": {"workload": "12-mass chain nx=48 nu=12 N=256 B=512, 1 SQP iteration, PCG eps 1e-8",
+                   "global_batch": B},
+        "plain": {"grads_per_s": B / res["plain"], "ms_fwd_bwd": res["plain"] * 1e3},
+        "fused_vs_plain": res["plain"] / res["fused"],
+        "pcg_iterations_forward": iters,
+        "roofline": {"bound": "hbm", "kernel": "stream assembly + stream_qp_kernel (forward SQP iteration)",
+                     "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                     "traffic": None, "launch_avg_ms": launch_ms},
+    }), flush=True)
 
 
 if __name__ == "__main__":

@@ -21,6 +21,31 @@ CANDIDATES: List[Tuple[str, int]] = [("register", 1), ("register", 2), ("registe
 _cache: Dict[tuple, Tuple[str, int]] = {}
 
 
+def register_horizon_limit(model: int, nx: int, nu: int, linesearch: bool = False, n_alphas: int = 1,
+                           nmax: int = 4096) -> int:
+    """Longest horizon the register-resident fused kernels take for a model
+    (0 if none): the automatic variant choice needs no workspace there."""
+    sp = _lib.SolverParams()
+    sp.num_scp_iteration_max, sp.pcg_max_iter, sp.n_alphas = 1, 1, n_alphas
+    sp.linesearch = int(linesearch)
+    for i in range(n_alphas):
+        sp.linesearch_alphas[i] = 1.0 if not linesearch else 0.5 + 0.5 * i / max(n_alphas - 1, 1)
+
+    def fits(n):
+        prob = _lib.Problem(model, nx, nu, n, 1, 0, 1 << 20, 0, 0.1)  # n_params: any sufficient count
+        return _lib.workspace_bytes(_lib.MODE_FUSED, prob, sp) == 0
+
+    if not fits(1):
+        return 0
+    lo, hi = 1, nmax
+    if fits(hi):
+        return hi
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        lo, hi = (mid, hi) if fits(mid) else (lo, mid)
+    return lo
+
+
 def _key(solver, B):
     prog, p = solver.program, solver.params
     return (prog.dynamics.model_id, prog.num_state_variables, prog.num_control_variables, prog.horizon, B,

@@ -170,6 +170,23 @@ def ragged(batch=8192, seed=3, tmin=10, tmax=200, sqp_iters=3):
     return out
 
 
+def split_by_horizon(prob, horizons, limit):
+    """Two padded groups of a ragged problem: instances with horizon <= limit
+    (padded to their own longest horizon) and the rest.  Returns a list of
+    (prob, horizons) without empty groups."""
+    out = []
+    for sel in (np.nonzero(horizons <= limit)[0], np.nonzero(horizons > limit)[0]):
+        if sel.size == 0:
+            continue
+        sub = slice_problem(prob, sel)
+        n = int(horizons[sel].max())
+        sub["horizon"] = n
+        for k in ("x_ref", "u_ref"):
+            sub[k] = sub[k][:, : n + 1]
+        out.append((sub, horizons[sel]))
+    return out
+
+
 def instance_problem(prob, b, n):
     """Instance b of a padded (ragged) problem dict, cut to horizon n."""
     one = slice_problem(prob, np.array([b]))
