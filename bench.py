@@ -43,9 +43,10 @@ def main():
                     help="time only the fused path (for profiler runs); prints the fused figures")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device-to-device once (reference kernel for HBM counters)")
-    ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain"], default="quadrotor",
+    ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain", "rollout"], default="quadrotor",
                     help="quadrotor: the headline line (default); ragged: mixed batch of 8192 with horizons "
-                         "U{10..200} in one graph; chain: 48-state chain B=512 N=256")
+                         "U{10..200} in one graph; chain: 48-state chain B=512 N=256; rollout: the reference's "
+                         "RL benchmark (LTI 8/4, N=39, 50 closed-loop steps, B=--batch or 64)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -73,6 +74,8 @@ def main():
         return bench_ragged(args)
     if args.workload == "chain":
         return bench_chain(args)
+    if args.workload == "rollout":
+        return bench_rollout(args)
 
     dev = torch.device("cuda:0")
     name, prob, sp = W.quadrotor(batch=args.batch, seed=1, horizon=args.horizon)
@@ -325,6 +328,67 @@ def bench_chain(args):
         "roofline": {"bound": "hbm", "kernel": "stream assembly + stream_qp_kernel (forward SQP iteration)",
                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
                      "traffic": None, "launch_avg_ms": launch_ms},
+    }), flush=True)
+
+
+# reference JAX GPU run of the same rollout (B=64, PCG 1e-12, warm start),
+# benchmarking/reinforcement-learning/timing_results/timing_64_40_12_pcg=1e-12_warmstart=True
+RL_REFERENCE_MS = {"fwd": 132.8, "fwd_bwd": 214.9}
+
+
+def bench_rollout(args):
+    """The reference's RL benchmark (benchmark_diffmpc.py): LTI nx=8 nu=4,
+    horizon 40 (N=39), 50 closed-loop steps, one SQP iteration, PCG 1e-12,
+    warm-started solves; forward rollout and gradient of the summed reward
+    w.r.t. the state weights, each captured as one hipGraph."""
+    import torch
+
+    from diffmpc_amd import workloads as W
+    from diffmpc_amd.graph import GraphedStep
+    from diffmpc_amd.rollout import Rollout, linear_simulator
+
+    dev = torch.device("cuda:0")
+    B = args.batch if args.batch != 1024 else 64
+    steps_sim = 50
+    name, prob, sp, (A, Bm, b) = W.rl_lti(batch=B, seed=0)
+    solver, params, _ = W.make_solver(name, prob, sp, device=dev)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)  # noqa: E731
+    ro = Rollout(solver, params, linear_simulator(t(A), t(Bm), t(b)),
+                 lambda x, u: -(x.square().sum(-1) + u.square().sum(-1)), steps_sim, adjoint_warm_start=False)
+    wx = params["weights_penalization_reference_state_trajectory"][0].clone().requires_grad_(True)
+    weights = {"weights_penalization_reference_state_trajectory": wx}
+    x_init = params["initial_state"]
+
+    def fwd():
+        with torch.no_grad():
+            return ro(x_init, weights)[0]
+
+    def fwd_bwd():
+        total = ro(x_init, weights)[0]
+        return torch.autograd.grad(total.sum(), [wx])[0]
+
+    res = {}
+    for key, fn in (("fwd", fwd), ("fwd_bwd", fwd_bwd)):
+        g = GraphedStep(fn)
+        for _ in range(args.warmup):
+            g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        res[key] = (time.perf_counter() - t0) / args.steps
+    solves = B * steps_sim
+    print(json.dumps({
+        "metric": "RL closed-loop rollout fwd+bwd, LTI nx=8 nu=4 N=39, 50 steps (solves/s in the gradient pass)",
+        "value": solves / res["fwd_bwd"], "unit": "grads/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["fwd_bwd"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": (RL_REFERENCE_MS["fwd_bwd"] / (res["fwd_bwd"] * 1e3)) if B == 64 else None,
+        "dtype": "f32 dynamics/linearization, f64 Schur/PCG", "data": "synthetic (the reference's seeded data)",
+        "config": {"workload": f"RL rollout LTI 8/4 N=39, 50 steps, PCG 1e-12, B={B}", "global_batch": B},
+        "fwd": {"ms": res["fwd"] * 1e3, "solves_per_s": solves / res["fwd"],
+                "vs_reference_jax_gpu": (RL_REFERENCE_MS["fwd"] / (res["fwd"] * 1e3)) if B == 64 else None},
+        "reference_jax_gpu_ms": RL_REFERENCE_MS,
     }), flush=True)
 
 

@@ -1,16 +1,24 @@
 #!/usr/bin/env bash
-# Build libdmpc.so for gfx950 in-tree (next to the Python package).
+# Build libdmpc.so for gfx950 in-tree (next to the Python package).  The four
+# translation units compile in parallel, then link with --no-undefined.
 # Packed-FP32 instructions are disabled on purpose: hipcc -O3 would otherwise
 # emit v_pk_* FP32 forms by itself.
 set -euo pipefail
 here="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 out="${here}/../libdmpc.so"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
-"${HIPCC}" -O3 -std=c++17 --offload-arch=gfx950 -fPIC -shared \
-  -Xclang -target-feature -Xclang -packed-fp32-ops \
-  -mcode-object-version=5 \
-  -Wall -Wno-unused-function \
-  "${here}/dmpc_fused.hip" "${here}/dmpc_stream.hip" "${here}/dmpc_plain.hip" "${here}/dmpc_api.hip" \
+obj="$(mktemp -d)"
+trap 'rm -rf "${obj}"' EXIT
+flags=(-O3 -std=c++17 --offload-arch=gfx950 -fPIC
+       -Xclang -target-feature -Xclang -packed-fp32-ops
+       -mcode-object-version=5 -Wall -Wno-unused-function)
+pids=()
+for src in dmpc_fused dmpc_stream dmpc_plain dmpc_api; do
+  "${HIPCC}" "${flags[@]}" -c "${here}/${src}.hip" -o "${obj}/${src}.o" &
+  pids+=($!)
+done
+for p in "${pids[@]}"; do wait "${p}"; done
+"${HIPCC}" --offload-arch=gfx950 -fPIC -shared "${obj}"/*.o \
   -Wl,--no-undefined -L"${ROCM_PATH:-/opt/rocm}/lib" -lamdhip64 \
   -o "${out}.tmp"
 mv "${out}.tmp" "${out}"

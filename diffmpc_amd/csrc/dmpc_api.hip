@@ -60,9 +60,18 @@ void prof_end(int k, hipStream_t s) {
 
 namespace {
 
-int model_sizes(int model, int* nx, int* nu, int* np) {
+// Sizes of a model; the linear model is built for (nx, nu) = (4, 2) and
+// (8, 4), so for it the requested sizes pick the instantiation.
+int model_sizes(int model, int* nx, int* nu, int* np, int want_nx = 0, int want_nu = 0) {
   switch (model) {
-    case DMPC_MODEL_LINEAR: *nx = 4; *nu = 2; *np = 4 * 4 + 4 * 2 + 4; return 0;
+    case DMPC_MODEL_LINEAR:
+      if (want_nx == 8 && want_nu == 4) {
+        *nx = 8; *nu = 4;
+      } else {
+        *nx = 4; *nu = 2;
+      }
+      *np = *nx * *nx + *nx * *nu + *nx;
+      return 0;
     case DMPC_MODEL_QUADROTOR: *nx = 13; *nu = 4; *np = 10; return 0;
     case DMPC_MODEL_SPACECRAFT3: *nx = 3; *nu = 3; *np = 3; return 0;
     case DMPC_MODEL_SPACECRAFT7: *nx = 7; *nu = 3; *np = 3; return 0;
@@ -75,7 +84,7 @@ int model_sizes(int model, int* nx, int* nu, int* np) {
 int check_problem(const dmpc_problem* p) {
   if (!p) return DMPC_ERR_ARG;
   int nx, nu, np;
-  if (model_sizes(p->model, &nx, &nu, &np)) return DMPC_ERR_UNSUPPORTED;
+  if (model_sizes(p->model, &nx, &nu, &np, p->nx, p->nu)) return DMPC_ERR_UNSUPPORTED;
   if (p->nx != nx || p->nu != nu) return DMPC_ERR_UNSUPPORTED;
   if (p->n_params < np) return DMPC_ERR_ARG;
   if (p->horizon < 1 || p->batch < 1) return DMPC_ERR_ARG;
@@ -103,7 +112,7 @@ const char* dmpc_version(void) { return "dmpc 0.1 gfx950"; }
 
 int dmpc_supported(int32_t mode, int32_t model, int32_t nx, int32_t nu, int32_t horizon) {
   int mx, mu, np;
-  if (model_sizes(model, &mx, &mu, &np) || mx != nx || mu != nu || horizon < 1) return 0;
+  if (model_sizes(model, &mx, &mu, &np, nx, nu) || mx != nx || mu != nu || horizon < 1) return 0;
   if (mode == DMPC_MODE_FUSED) return fused_supported(model, nx, nu, horizon) ? 1 : 0;
   if (mode == DMPC_MODE_PLAIN) return 1;
   return 0;
@@ -182,6 +191,8 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   a.P = prob->n_params;
   a.scheme = prob->scheme;
   a.penalize = prob->penalize_control_reference;
+  a.nx = prob->nx;
+  a.nu = prob->nu;
   a.forced_K = params->blocks_per_group;
   a.variant = params->fused_variant;
   a.dt = (float)prob->dt;

@@ -888,7 +888,9 @@ bool fits(int N, int lsd) {
 bool register_fits(int model, int nx, int nu, int N, int lsd) {
   if (N < 1) return false;
   switch (model) {
-    case MODEL_LINEAR: return nx == 4 && nu == 2 && fits<LinearModel<4, 2>, 4>(N, lsd);
+    case MODEL_LINEAR:
+      if (nx == 8 && nu == 4) return fits<LinearModel<8, 4>, 8>(N, lsd);
+      return nx == 4 && nu == 2 && fits<LinearModel<4, 2>, 4>(N, lsd);
     case MODEL_QUADROTOR: return nx == 13 && nu == 4 && fits<QuadrotorModel, 16>(N, lsd);
     case MODEL_SPACECRAFT3: return nx == 3 && nu == 3 && fits<Spacecraft3Model, 4>(N, lsd);
     case MODEL_SPACECRAFT7: return nx == 7 && nu == 3 && fits<Spacecraft7Model, 8>(N, lsd);
@@ -939,7 +941,9 @@ int launch_linearize(const SolveArgs& a, hipStream_t st) {
 
 int stream_launch_linearize(int model, const SolveArgs& a, hipStream_t st) {
   switch (model) {
-    case MODEL_LINEAR: return launch_linearize<LinearModel<4, 2>>(a, st);
+    case MODEL_LINEAR:
+      if (a.nx == 8) return launch_linearize<LinearModel<8, 4>>(a, st);
+      return launch_linearize<LinearModel<4, 2>>(a, st);
     case MODEL_QUADROTOR: return launch_linearize<QuadrotorModel>(a, st);
     case MODEL_SPACECRAFT3: return launch_linearize<Spacecraft3Model>(a, st);
     case MODEL_SPACECRAFT7: return launch_linearize<Spacecraft7Model>(a, st);
@@ -952,7 +956,7 @@ int stream_launch_linearize(int model, const SolveArgs& a, hipStream_t st) {
 int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st) {
   int nx, nu;
   switch (model) {
-    case MODEL_LINEAR: nx = 4; nu = 2; break;
+    case MODEL_LINEAR: nx = a.nx; nu = nx == 8 ? 4 : 2; break;
     case MODEL_QUADROTOR: nx = QuadrotorModel::NX; nu = QuadrotorModel::NU; break;
     case MODEL_SPACECRAFT3: nx = Spacecraft3Model::NX; nu = Spacecraft3Model::NU; break;
     case MODEL_SPACECRAFT7: nx = Spacecraft7Model::NX; nu = Spacecraft7Model::NU; break;
@@ -969,7 +973,9 @@ int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st)
   if (use_stream(model, nx, nu, a.N, lsd, a.variant)) return stream_forward(model, nx, nu, a, st);
   if (!register_fits(model, nx, nu, a.N, lsd)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {
-    case MODEL_LINEAR: return fwd_model<LinearModel<4, 2>, 4>(a, st);
+    case MODEL_LINEAR:
+      if (nx == 8) return fwd_model<LinearModel<8, 4>, 8>(a, st);
+      return fwd_model<LinearModel<4, 2>, 4>(a, st);
     case MODEL_QUADROTOR: return fwd_model<QuadrotorModel, 16>(a, st);
     case MODEL_SPACECRAFT3: return fwd_model<Spacecraft3Model, 4>(a, st);
     case MODEL_SPACECRAFT7: return fwd_model<Spacecraft7Model, 8>(a, st);
@@ -982,7 +988,9 @@ int fused_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t
   if (use_stream(model, nx, nu, a.N, 0, a.variant)) return stream_backward(model, nx, nu, a, st);
   if (!register_fits(model, nx, nu, a.N, 0)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {
-    case MODEL_LINEAR: return bwd_model<LinearModel<4, 2>, 4>(a, st);
+    case MODEL_LINEAR:
+      if (nx == 8) return bwd_model<LinearModel<8, 4>, 8>(a, st);
+      return bwd_model<LinearModel<4, 2>, 4>(a, st);
     case MODEL_QUADROTOR: return bwd_model<QuadrotorModel, 16>(a, st);
     case MODEL_SPACECRAFT3: return bwd_model<Spacecraft3Model, 4>(a, st);
     case MODEL_SPACECRAFT7: return bwd_model<Spacecraft7Model, 8>(a, st);

@@ -14,6 +14,7 @@ constexpr int kMaxAlphas = 8;
 // Everything one batched SQP solve needs, flattened for kernel arguments.
 struct SolveArgs {
   int B, N, P, scheme, penalize;
+  int nx, nu;             // model sizes (the linear model has two builds)
   int forced_K, variant;  // per-call tuning (0 = library default)
   float dt;       // step of the float32 linearization
   double dt64;    // step of the float64 line-search evaluations

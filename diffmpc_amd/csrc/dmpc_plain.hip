@@ -945,6 +945,7 @@ int plain_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st)
   switch (model) {
     case MODEL_LINEAR:
       if (nx == 4 && nu == 2) return plain_forward_t<LinearModel<4, 2>>(a, st);
+      if (nx == 8 && nu == 4) return plain_forward_t<LinearModel<8, 4>>(a, st);
       return DMPC_ERR_UNSUPPORTED;
     case MODEL_QUADROTOR: return plain_forward_t<QuadrotorModel>(a, st);
     case MODEL_SPACECRAFT3: return plain_forward_t<Spacecraft3Model>(a, st);
@@ -1072,6 +1073,7 @@ int linearize_any(const dmpc_problem* p, const dmpc_problem_data* d, const doubl
   switch (p->model) {
     case MODEL_LINEAR:
       if (p->nx == 4 && p->nu == 2) return linearize_t<LinearModel<4, 2>>(p, d, x, u, fd, A, Bm, C, st);
+      if (p->nx == 8 && p->nu == 4) return linearize_t<LinearModel<8, 4>>(p, d, x, u, fd, A, Bm, C, st);
       return DMPC_ERR_UNSUPPORTED;
     case MODEL_QUADROTOR: return linearize_t<QuadrotorModel>(p, d, x, u, fd, A, Bm, C, st);
     case MODEL_SPACECRAFT3: return linearize_t<Spacecraft3Model>(p, d, x, u, fd, A, Bm, C, st);

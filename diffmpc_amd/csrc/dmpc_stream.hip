@@ -768,7 +768,7 @@ int stream_backward_t(const BackwardArgs& a, hipStream_t st) {
 
 bool stream_supported(int model, int nx, int nu) {
   switch (model) {
-    case MODEL_LINEAR: return nx == 4 && nu == 2;
+    case MODEL_LINEAR: return (nx == 4 && nu == 2) || (nx == 8 && nu == 4);
     case MODEL_QUADROTOR: return nx == 13 && nu == 4;
     case MODEL_SPACECRAFT3: return nx == 3 && nu == 3;
     case MODEL_SPACECRAFT7: return nx == 7 && nu == 3;
@@ -782,7 +782,9 @@ int stream_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st
   if (!stream_supported(model, nx, nu)) return DMPC_ERR_UNSUPPORTED;
   if (!a.work) return DMPC_ERR_ARG;
   switch (model) {
-    case MODEL_LINEAR: return stream_forward_t<LinearModel<4, 2>, 4>(model, a, st);
+    case MODEL_LINEAR:
+      if (nx == 8) return stream_forward_t<LinearModel<8, 4>, 8>(model, a, st);
+      return stream_forward_t<LinearModel<4, 2>, 4>(model, a, st);
     case MODEL_QUADROTOR: return stream_forward_t<QuadrotorModel, 16>(model, a, st);
     case MODEL_SPACECRAFT3: return stream_forward_t<Spacecraft3Model, 4>(model, a, st);
     case MODEL_SPACECRAFT7: return stream_forward_t<Spacecraft7Model, 8>(model, a, st);
@@ -796,7 +798,9 @@ int stream_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_
   if (!stream_supported(model, nx, nu)) return DMPC_ERR_UNSUPPORTED;
   if (!a.work) return DMPC_ERR_ARG;
   switch (model) {
-    case MODEL_LINEAR: return stream_backward_t<LinearModel<4, 2>, 4>(a, st);
+    case MODEL_LINEAR:
+      if (nx == 8) return stream_backward_t<LinearModel<8, 4>, 8>(a, st);
+      return stream_backward_t<LinearModel<4, 2>, 4>(a, st);
     case MODEL_QUADROTOR: return stream_backward_t<QuadrotorModel, 16>(a, st);
     case MODEL_SPACECRAFT3: return stream_backward_t<Spacecraft3Model, 4>(a, st);
     case MODEL_SPACECRAFT7: return stream_backward_t<Spacecraft7Model, 8>(a, st);

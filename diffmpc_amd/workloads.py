@@ -146,6 +146,37 @@ def chain(batch=512, seed=4, horizon=256, pcg_eps=1e-8):
     return "chain", prob, solver_params(1, 0.0, pcg_eps)
 
 
+def rl_lti(batch=64, seed=0, horizon=39, pcg_eps=1e-12):
+    """The reference's reinforcement-learning benchmark problem
+    (benchmarking/reinforcement-learning/utils.py generate_problem_data and
+    benchmark_diffmpc.py:40-99): nx=8, nu=4, one shared discrete-time system
+    A = I + 0.1 randn with its eigenvalues pulled inside the unit circle,
+    B = randn, b = 0.01 randn, Q = I, R = I, x0 = 5 randn per instance (numpy's
+    legacy seeded generator, so the data equal the reference's), passed in
+    continuous form with dt = 1 and Euler; horizon 40 (N = 39), one SQP
+    iteration, line search with alphas [1.0], PCG eps 1e-12.  Returns
+    (name, prob, solver_params, (A, B, b) discrete)."""
+    nx, nu, N = 8, 4, horizon
+    rs = np.random.RandomState(seed)
+    A = np.eye(nx) + 0.1 * rs.randn(nx, nx)
+    lam, V = np.linalg.eig(A)
+    mag = np.abs(lam)
+    lam = np.where(mag < 1 - 1e-2, lam, lam / (mag + 1e-2))
+    A = (V @ np.diag(lam) @ np.linalg.inv(V)).real
+    Bm = rs.randn(nx, nu)
+    b = 0.01 * rs.randn(nx)
+    x0 = rs.randn(batch, nx) * 5.0
+    dyn = np.concatenate([(A - np.eye(nx)).reshape(-1), Bm.reshape(-1), b])
+    prob = dict(
+        horizon=N, dt=1.0, scheme=0, penalize_control_reference=False,
+        x0=x0, x_final=np.zeros((batch, nx)), x_ref=np.zeros((batch, N + 1, nx)),
+        u_ref=np.zeros((batch, N + 1, nu)), w_x=_rep(np.ones(nx), batch),
+        w_u=_rep(np.ones(nu), batch), w_f=np.zeros((batch, nx)), dyn=_rep(dyn, batch),
+    )
+    sp = solver_params(1, 1e-8, pcg_eps, linesearch=True, alphas=(1.0,))
+    return "linear", prob, sp, (A, Bm, b)
+
+
 def ragged(batch=8192, seed=3, tmin=10, tmax=200, sqp_iters=3):
     """Ragged mixed batch: horizons N ~ U{tmin..tmax} and systems drawn
     uniformly from {LTI nx=4/nu=2, spacecraft attitude nx=7/nu=3, quadrotor

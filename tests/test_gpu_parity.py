@@ -551,3 +551,63 @@ def test_graph_captured_ragged_step_matches_eager_on_every_replay(cuda):
     for (se, ge), (sg, gg) in zip(eager2, got):
         assert torch.equal(se, sg) and torch.equal(ge, gg)
     assert not all(torch.equal(a[0], b[0]) for a, b in zip(eager, eager2))
+
+
+def _rl_setup(cuda, batch, horizon, steps, adjoint_warm_start=True):
+    from diffmpc_amd.rollout import Rollout, linear_simulator
+
+    name, prob, sp, (A, Bm, b) = W.rl_lti(batch=batch, seed=0, horizon=horizon)
+    solver, params, _ = W.make_solver(name, prob, sp, device=cuda)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device=cuda)  # noqa: E731
+    sim = linear_simulator(t(A), t(Bm), t(b))
+    reward = lambda x, u: -(x.square().sum(-1) + u.square().sum(-1))  # noqa: E731
+    ro = Rollout(solver, params, sim, reward, steps, adjoint_warm_start=adjoint_warm_start)
+    return name, prob, sp, params, ro
+
+
+def test_rollout_matches_oracle_rollout_and_gradient(cuda):
+    """Closed-loop rollout of the reference's RL benchmark problem (LTI 8/4,
+    warm-started solves, first control applied, summed reward) and the
+    gradient of the total reward w.r.t. the shared state weights, against the
+    oracle's rollout and its reverse-mode restatement."""
+    from oracle import rollout as orollout
+
+    say("[rollout] rl-lti B=4 N=19 8 steps")
+    steps = 8
+    name, prob, sp, params, ro = _rl_setup(cuda, 4, 19, steps)
+    wx = params["weights_penalization_reference_state_trajectory"][0].clone().requires_grad_(True)
+    total, xs, us = ro(params["initial_state"], {"weights_penalization_reference_state_trajectory": wx})
+    (g,) = torch.autograd.grad(total.sum(), [wx])
+    torch.cuda.synchronize()
+    op = oracle_problem(name, prob)
+    rew = lambda x, u: -(np.sum(x ** 2, 1) + np.sum(u ** 2, 1))  # noqa: E731
+    tot_ref, records = orollout.rollout(op, sp, prob["x0"], steps, 0, rew)
+    gref = orollout.rollout_gradient(op, sp, records, 0, lambda x1, u: (-2 * x1, -2 * u), keys=("w_x",))
+    e_tot = float(np.max(np.abs(_np(total) - tot_ref) / np.abs(tot_ref)))
+    e_g = float(np.max(np.abs(_np(g) - gref["w_x"])) / np.max(np.abs(gref["w_x"])))
+    say(f"  total rel {e_tot:.2e}, gradient rel {e_g:.2e}")
+    assert e_tot <= 1e-5
+    assert e_g <= 1e-4
+    ref_states = np.stack([records[0][2]] + [r[4] for r in records])
+    assert rel(_np(xs).transpose(1, 0, 2), ref_states.transpose(1, 0, 2)) <= 1e-5
+
+
+def test_graph_captured_rollout_gradient_matches_eager(cuda):
+    from diffmpc_amd.graph import GraphedStep
+
+    say("[rollout-graph] rl-lti B=8 N=39 5 steps")
+    _, _, _, params, ro = _rl_setup(cuda, 8, 39, 5, adjoint_warm_start=False)
+    wx = params["weights_penalization_reference_state_trajectory"][0].clone().requires_grad_(True)
+    x_init = params["initial_state"]
+
+    def step():
+        total, _, _ = ro(x_init, {"weights_penalization_reference_state_trajectory": wx})
+        (g,) = torch.autograd.grad(total.sum(), [wx])
+        return total, g
+
+    e_total, e_g = (v.detach().clone() for v in step())
+    graphed = GraphedStep(step)
+    for _ in range(3):
+        total, g = graphed.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(total, e_total) and torch.equal(g, e_g)
