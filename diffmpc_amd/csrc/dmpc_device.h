@@ -172,10 +172,32 @@ __device__ __forceinline__ double gdot16x13(const double* a, double v, double ac
   return acc;
 }
 
+// As gdot16x13 for float32 a[]: each term converts a[j] into a scratch pair
+// and feeds it to the DPP FMA (the conversion writes the non-DPP operand, so
+// it needs no wait state).  Same terms, order and roundings as the generic
+// chain (exact conversion, one FMA rounding per term).
+__device__ __forceinline__ double gdot16x13f(const float* a, double v, double acc) {
+  double tmp;
+#define DMPC_CVT_FMAC(op, j)                                              \
+  "v_cvt_f64_f32 %1, %" #op "\n"                                          \
+  "v_fmac_f64_dpp %0, %2, %1 row_newbcast:" #j " row_mask:0xf bank_mask:0xf\n"
+  asm("s_nop 4\n" DMPC_CVT_FMAC(3, 0) DMPC_CVT_FMAC(4, 1) DMPC_CVT_FMAC(5, 2) DMPC_CVT_FMAC(6, 3)
+          DMPC_CVT_FMAC(7, 4) DMPC_CVT_FMAC(8, 5) DMPC_CVT_FMAC(9, 6) DMPC_CVT_FMAC(10, 7)
+              DMPC_CVT_FMAC(11, 8) DMPC_CVT_FMAC(12, 9) DMPC_CVT_FMAC(13, 10) DMPC_CVT_FMAC(14, 11)
+                  DMPC_CVT_FMAC(15, 12)
+      : "+v"(acc), "=&v"(tmp)
+      : "v"(v), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]),
+        "v"(a[8]), "v"(a[9]), "v"(a[10]), "v"(a[11]), "v"(a[12]));
+#undef DMPC_CVT_FMAC
+  return acc;
+}
+
 template <int G, int M, class AT>
 __device__ __forceinline__ double gdot(const AT* a, double v, int lane, int salt) {
   if constexpr (G == 16 && M == 13 && std::is_same<AT, double>::value) {
     return gdot16x13(a, v, 0.0);
+  } else if constexpr (G == 16 && M == 13 && std::is_same<AT, float>::value) {
+    return gdot16x13f(a, v, 0.0);
   } else {
     return gdot_chain<G, M>(a, v, lane, salt, 0.0);
   }

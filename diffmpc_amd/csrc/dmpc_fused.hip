@@ -245,7 +245,14 @@ __device__ __forceinline__ double dmul(const Regs<NX, NU, K>& R, int s, const Ld
   double acc;
   if constexpr (d_in_lds<NX>()) {
     const double* col = L.LF + (size_t)R.t[s] * NX * NX + (lane < NX ? lane : 0);
-    acc = gdot_strided<G, NX, NX>(col, v, lane);
+    if constexpr (G == 16 && NX == 13) {
+      double r[NX];
+#pragma unroll
+      for (int j = 0; j < NX; ++j) r[j] = col[j * NX];
+      acc = gdot16x13(r, v, 0.0);
+    } else {
+      acc = gdot_strided<G, NX, NX>(col, v, lane);
+    }
   } else {
     acc = gdotd<G, NX>(R.drow[s], v, lane);
   }
@@ -798,9 +805,10 @@ Launch plan(int N, int P, int ls = 0, int forced_K = 0) {
   constexpr int NX = Model::NX, NU = Model::NU;
   const int NB = N + 1;
   Launch l{0, 0, LdsPlan<NX, NU>::bytes(N, P, ls)};
-  // two blocks per thread group first: twice the registers per block group
-  // (measured fastest where both fit), then one, then four
-  const int cand[3] = {2, 1, 4};
+  // one block per thread group first (the most waves per instance; with the
+  // fused DPP-FMA chains it measured at least as fast as two for every
+  // model), then two, then four
+  const int cand[3] = {1, 2, 4};
   for (int c = 0; c < 3; ++c) {
     const int K = cand[c];
     const int want = forced_K > 0 ? forced_K : g_forced_K;
