@@ -100,6 +100,16 @@ __device__ __forceinline__ double cvt_in_loop(float f, int salt) {
 }
 __device__ __forceinline__ double cvt_in_loop(double d, int) { return d; }
 
+// `off`, marked as depending on `dep` (an empty asm, no instruction): loads
+// addressed with the result cannot be issued before `dep` is computed.  In a
+// sequential substitution this keeps the scheduler from hoisting every row's
+// loads above the whole chain, which with one time block per lane group would
+// hold more registers than the wave has and spill.
+__device__ __forceinline__ int after(int off, double dep) {
+  asm("" : "+v"(off) : "v"(dep));
+  return off;
+}
+
 // ---------------------------------------------------------------------------
 // Broadcast of lane n of each G-lane block group to the whole group with DPP
 // (no LDS traffic): G = 16 is one DPP row (row_newbcast:n, one 64-bit move on

@@ -212,16 +212,18 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* A
     double w[NX], d[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j) {
+      const int rj = j == 0 ? 0 : after(j * NX, w[j - 1]);  // row j of L
       double acc = (j == lane) ? 1.0 : 0.0;
 #pragma unroll
-      for (int k = 0; k < j; ++k) acc -= LF[j * NX + k] * w[k];
+      for (int k = 0; k < j; ++k) acc -= LF[rj + k] * w[k];
       w[j] = acc * ID[j];
     }
 #pragma unroll
     for (int j = NX - 1; j >= 0; --j) {
+      const int cj = j == NX - 1 ? j : after(j, d[j + 1]);  // column j of L
       double acc = w[j];
 #pragma unroll
-      for (int k = j + 1; k < NX; ++k) acc -= LF[k * NX + j] * d[k];
+      for (int k = j + 1; k < NX; ++k) acc -= LF[k * NX + cj] * d[k];
       d[j] = acc * ID[j];
     }
     if constexpr (d_in_lds<NX>()) {
