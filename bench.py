@@ -21,6 +21,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SQP solves/s & fwd+bwd grads/s, quadrotor T=50 B=1024, fused vs plain GPU"
 MI355X_FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per qp_iteration_kernel launch at the default workload, from two
+# rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scale factors
+# from a 256 MiB device copy): 109.8 MB read + 97.0 MB written
+QP_TRAFFIC_BYTES = 206.8e6
+QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
 
 
 def pcg_flops(N, nx):
@@ -179,7 +184,8 @@ def main():
             "peak": MI355X_FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / MI355X_FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": QP_TRAFFIC_BYTES if (N, args.batch) == (50, 1024) else None,
+            "traffic_source": QP_TRAFFIC_SOURCE,
             "launch_avg_ms": qp_avg_ms,
             "flops_per_launch": flops_qp_launch,
         },
