@@ -81,6 +81,7 @@ class Grads(ctypes.Structure):
         ("w_x", ctypes.c_void_p), ("w_u", ctypes.c_void_p), ("w_f", ctypes.c_void_p),
         ("x0", ctypes.c_void_p), ("x_ref", ctypes.c_void_p), ("u_ref", ctypes.c_void_p),
         ("x_final", ctypes.c_void_p), ("dual_backward", ctypes.c_void_p), ("pcg_iters", ctypes.c_void_p),
+        ("adj_states", ctypes.c_void_p), ("adj_controls", ctypes.c_void_p),
     ]
 
 
@@ -94,6 +95,8 @@ _SIGNATURES = {
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Saved), ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.POINTER(Grads), ctypes.c_int32, ctypes.c_void_p]),
     "dmpc_linearize": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(ProblemData)] + [ctypes.c_void_p] * 7),
+    "dmpc_dynamics_param_grads": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(ProblemData)]
+                                  + [ctypes.c_void_p] * 8),
     "dmpc_pcg_assemble": (ctypes.c_int, [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 8),
     "dmpc_pcg_gamma": (ctypes.c_int, [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 10),
     "dmpc_pcg_solve": (ctypes.c_int, [ctypes.c_int32] * 3 + [ctypes.c_void_p] * 4 + [ctypes.c_int32, ctypes.c_double]

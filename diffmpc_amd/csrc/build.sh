@@ -1,5 +1,5 @@
 #!/usr/bin/env bash
-# Build libdmpc.so for gfx950 in-tree (next to the Python package).  The four
+# Build libdmpc.so for gfx950 in-tree (next to the Python package).  The
 # translation units compile in parallel, then link with --no-undefined.
 # Packed-FP32 instructions are disabled on purpose: hipcc -O3 would otherwise
 # emit v_pk_* FP32 forms by itself.
@@ -13,7 +13,7 @@ flags=(-O3 -std=c++17 --offload-arch=gfx950 -fPIC
        -Xclang -target-feature -Xclang -packed-fp32-ops
        -mcode-object-version=5 -Wall -Wno-unused-function)
 pids=()
-for src in dmpc_fused dmpc_stream dmpc_plain dmpc_api; do
+for src in dmpc_fused dmpc_stream dmpc_plain dmpc_dyngrad dmpc_api; do
   "${HIPCC}" "${flags[@]}" -c "${here}/${src}.hip" -o "${obj}/${src}.o" &
   pids+=($!)
 done

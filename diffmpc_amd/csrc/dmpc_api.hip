@@ -19,6 +19,9 @@ int pcg_layer_recover(int B, int N, int nx, int nu, const double* lam, const dou
                       double* x, double* u, hipStream_t st);
 int linearize_any(const dmpc_problem* p, const dmpc_problem_data* d, const double* x, const double* u, double* fd,
                   double* A, double* Bm, double* C, hipStream_t st);
+int dyn_param_grads_any(const dmpc_problem* p, const dmpc_problem_data* d, const double* x, const double* u,
+                        const double* lam, const double* xt, const double* ut, const double* lamt, double* g,
+                        hipStream_t st);
 }  // namespace dmpc
 
 using namespace dmpc;
@@ -288,6 +291,8 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
   a.g_xref = g->x_ref;
   a.g_uref = g->u_ref;
   a.g_xf = g->x_final;
+  a.adj_x = g->adj_states;
+  a.adj_u = g->adj_controls;
   a.pcg_iters = g->pcg_iters;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (mode == DMPC_MODE_FUSED) {
@@ -303,6 +308,18 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
     return plain_backward(prob->model, prob->nx, prob->nu, a, st);
   }
   return DMPC_ERR_ARG;
+}
+
+int dmpc_dynamics_param_grads(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,
+                              const double* controls, const double* dual, const double* adj_states,
+                              const double* adj_controls, const double* adj_dual, double* grads, void* stream) {
+  int rc = check_problem(prob);
+  if (rc) return rc;
+  if (!data || !data->dyn_params || !states || !controls || !dual || !adj_states || !adj_controls || !adj_dual ||
+      !grads)
+    return DMPC_ERR_ARG;
+  return dyn_param_grads_any(prob, data, states, controls, dual, adj_states, adj_controls, adj_dual, grads,
+                             reinterpret_cast<hipStream_t>(stream));
 }
 
 int dmpc_linearize(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,

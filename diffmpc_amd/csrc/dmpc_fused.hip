@@ -745,6 +745,8 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   // thread per component sums over time in a fixed order
   double* TX = L.YX;  // (NB, NX): 2 (x - ref) x~
   double* TU = L.Y;   // (NB, NU)
+  double* XT = L.RT;  // (NB, NX): x~, staged for the optional adjoint output
+  double* UT = L.YQ;  // (NB, NU): u~
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     if (R.ok[s]) {
@@ -768,6 +770,11 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
       }
       a.lam_bwd[((size_t)b * NBs + t) * NX + i] = lam[s];
       if (t == 0) a.g_x0[b * NX + i] = lam[s];
+      XT[t * NX + i] = xt[s];
+      if (i < NU) {
+        const int tu = control_row(t, N);
+        UT[tu * NU + i] = (tu < N) ? ut[s] : 0.0;
+      }
     }
   }
   // knots past this instance's horizon (ragged batches) are zero
@@ -787,6 +794,12 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
       for (int t = 0; t < N; ++t) acc += TU[t * NU + k];
       a.g_wu[b * NU + k] = -acc;
     }
+  }
+  if (a.adj_x) {
+    for (int k = threadIdx.x; k < NBs * NX; k += blockDim.x) a.adj_x[(size_t)b * NBs * NX + k] = k < NB * NX ? XT[k] : 0.0;
+  }
+  if (a.adj_u) {
+    for (int k = threadIdx.x; k < NBs * NU; k += blockDim.x) a.adj_u[(size_t)b * NBs * NU + k] = k < N * NU ? UT[k] : 0.0;
   }
   if (threadIdx.x == 0) a.pcg_iters[b] = npcg;
 }

@@ -58,6 +58,7 @@ struct BackwardArgs {
   const double *dl_dx, *dl_du;         // cotangents (B,N+1,nx), (B,N+1,nu)
   double* lam_bwd;                     // in: adjoint warm start, out: lam~
   double *g_wx, *g_wu, *g_wf, *g_x0, *g_xref, *g_uref, *g_xf;
+  double *adj_x, *adj_u;               // optional: x~ (B,N+1,nx), u~ (B,N+1,nu)
   int32_t* pcg_iters;                  // (B,)
   char* work;                          // streamed variant workspace
   size_t work_stride;

@@ -1,12 +1,16 @@
 // Dynamics models and integrators as device templates over the scalar type.
 //
 // Every model is a struct with NX, NU and a templated
-//   f(const T* x, const T* u, const float* prm, T* xdot)
+//   f(const T* x, const T* u, PA prm, T* xdot)
 // where T is float or Dual (forward-mode derivative with one tangent, for
-// the linearization) or double (value only, for the line search's merit
-// evaluations).  Constants derived from the parameters are computed in the
-// scalar type S of T (float for float and Dual, double for double).  The
-// flat parameter layouts are the ones documented in include/dmpc.h.
+// the linearization), double (value only, for the line search's merit
+// evaluations) or HD (float64 hyper-dual: two tangents and their mixed
+// second derivative, for the dynamics-parameter gradients).  `prm` is
+// anything indexable that yields the flat parameters: a `const float*`, or
+// ParamTangent, which seeds one parameter's tangent.  Constants derived from
+// the parameters are computed in the scalar type S of T (float for float and
+// Dual, double for double, HD for HD).  The flat parameter layouts are the
+// ones documented in include/dmpc.h.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -48,6 +52,52 @@ __device__ __forceinline__ float val(float a) { return a; }
 __device__ __forceinline__ float val(Dual a) { return a.v; }
 __device__ __forceinline__ float tan_(Dual a) { return a.d; }
 
+// ---------------------------------------------------------------------------
+// Float64 hyper-dual number v + a e1 + b e2 + ab e1 e2 (e1^2 = e2^2 = 0):
+// evaluating f on (x + e1 dx, p + e2 dp) gives f, its directional
+// derivatives along dx (a) and dp (b) and the mixed second derivative
+// d2f/dx dp [dx, dp] (ab), all exact to rounding.
+struct HD {
+  double v, a, b, ab;
+  HD() = default;
+  __host__ __device__ constexpr HD(double x) : v(x), a(0.0), b(0.0), ab(0.0) {}
+  __host__ __device__ constexpr HD(double v_, double a_, double b_, double ab_) : v(v_), a(a_), b(b_), ab(ab_) {}
+};
+__device__ __forceinline__ HD operator+(HD x, HD y) { return {x.v + y.v, x.a + y.a, x.b + y.b, x.ab + y.ab}; }
+__device__ __forceinline__ HD operator-(HD x, HD y) { return {x.v - y.v, x.a - y.a, x.b - y.b, x.ab - y.ab}; }
+__device__ __forceinline__ HD operator-(HD x) { return {-x.v, -x.a, -x.b, -x.ab}; }
+__device__ __forceinline__ HD operator*(HD x, HD y) {
+  return {x.v * y.v, x.v * y.a + x.a * y.v, x.v * y.b + x.b * y.v,
+          x.v * y.ab + x.a * y.b + x.b * y.a + x.ab * y.v};
+}
+// g(x) for a scalar function with g(v) = g0, g'(v) = g1, g''(v) = g2
+__device__ __forceinline__ HD hd_apply(HD x, double g0, double g1, double g2) {
+  return {g0, g1 * x.a, g1 * x.b, g1 * x.ab + g2 * x.a * x.b};
+}
+__device__ __forceinline__ HD operator/(HD x, HD y) {
+  const double r = 1.0 / y.v;
+  return x * hd_apply(y, r, -r * r, 2.0 * r * r * r);
+}
+__device__ __forceinline__ HD dsqrt(HD x) {
+  const double s = sqrt(x.v);
+  return hd_apply(x, s, 0.5 / s, -0.25 / (s * x.v));
+}
+__device__ __forceinline__ HD dsin(HD x) {
+  const double s = sin(x.v), c = cos(x.v);
+  return hd_apply(x, s, c, -s);
+}
+__device__ __forceinline__ HD dcos(HD x) {
+  const double s = sin(x.v), c = cos(x.v);
+  return hd_apply(x, c, -s, -c);
+}
+
+// Flat parameters with the e2 tangent on parameter j (hyper-dual values).
+struct ParamTangent {
+  const float* p;
+  int j;
+  __device__ HD operator[](int q) const { return HD((double)p[q], 0.0, q == j ? 1.0 : 0.0, 0.0); }
+};
+
 // scalar type of T
 template <class T>
 struct ScalarOf {
@@ -56,6 +106,10 @@ struct ScalarOf {
 template <>
 struct ScalarOf<double> {
   using type = double;
+};
+template <>
+struct ScalarOf<HD> {
+  using type = HD;
 };
 template <class T>
 using scalar_t = typename ScalarOf<T>::type;
@@ -68,6 +122,8 @@ template <>
 __device__ __forceinline__ Dual sc<Dual>(float v) { return Dual{v, 0.0f}; }
 template <>
 __device__ __forceinline__ double sc<double>(double v) { return v; }
+template <>
+__device__ __forceinline__ HD sc<HD>(HD v) { return v; }
 
 enum ModelId : int {
   MODEL_LINEAR = 0,
@@ -84,8 +140,8 @@ template <int NX_, int NU_>
 struct LinearModel {
   static constexpr int NX = NX_, NU = NU_;
   static constexpr int NPARAM = NX * NX + NX * NU + NX;
-  template <class T>
-  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+  template <class T, class PA>
+  __device__ static void f(const T* x, const T* u, PA p, T* xd) {
 #pragma unroll
     for (int i = 0; i < NX; ++i) {
       T acc = sc<T>(p[NX * NX + NX * NU + i]);
@@ -102,8 +158,8 @@ struct LinearModel {
 // diffmpc/dynamics/quadrotor_dynamics.py:84-133.  Parameters: mass, J (3x3).
 struct QuadrotorModel {
   static constexpr int NX = 13, NU = 4, NPARAM = 10;
-  template <class T>
-  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+  template <class T, class PA>
+  __device__ static void f(const T* x, const T* u, PA p, T* xd) {
     using S = scalar_t<T>;
     const S m = p[0];
     S J[9];
@@ -148,8 +204,8 @@ struct QuadrotorModel {
 // diffmpc/dynamics/spacecraft_dynamics.py:31-66.
 struct Spacecraft3Model {
   static constexpr int NX = 3, NU = 3, NPARAM = 3;
-  template <class T>
-  __device__ static void f(const T* w, const T* u, const float* p, T* xd) {
+  template <class T, class PA>
+  __device__ static void f(const T* w, const T* u, PA p, T* xd) {
     using S = scalar_t<T>;
     const S J0 = p[0], J1 = p[1], J2 = p[2];
     T hx = J0 * w[0], hy = J1 * w[1], hz = J2 * w[2];
@@ -164,8 +220,8 @@ struct Spacecraft3Model {
 // Euler's equation of spacecraft_dynamics.py:63 (diagonal J).
 struct Spacecraft7Model {
   static constexpr int NX = 7, NU = 3, NPARAM = 3;
-  template <class T>
-  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+  template <class T, class PA>
+  __device__ static void f(const T* x, const T* u, PA p, T* xd) {
     using S = scalar_t<T>;
     T qw = x[0], qx = x[1], qy = x[2], qz = x[3];
     T inv = sc<T>(S(1)) / dsqrt(qw * qw + qx * qx + qy * qy + qz * qz);
@@ -187,8 +243,8 @@ struct Spacecraft7Model {
 // Parameters: masscart, masspole, length, gravity.
 struct CartpoleModel {
   static constexpr int NX = 4, NU = 1, NPARAM = 4;
-  template <class T>
-  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+  template <class T, class PA>
+  __device__ static void f(const T* x, const T* u, PA p, T* xd) {
     using S = scalar_t<T>;
     const S mc = p[0], mp = p[1], len = p[2], g = p[3];
     const S total = mc + mp, pml = mp * len;
@@ -209,8 +265,8 @@ struct CartpoleModel {
 struct ChainModel {
   static constexpr int M = 12;
   static constexpr int NX = 4 * M, NU = M, NPARAM = 2 * M;
-  template <class T>
-  __device__ static void f(const T* x, const T* u, const float* p, T* xd) {
+  template <class T, class PA>
+  __device__ static void f(const T* x, const T* u, PA p, T* xd) {
 #pragma unroll
     for (int i = 0; i < 2 * M; ++i) xd[i] = x[2 * M + i];
 #pragma unroll
@@ -237,8 +293,8 @@ struct ChainModel {
 // ---------------------------------------------------------------------------
 // One integration step (diffmpc/dynamics/integrators.py:20-71):
 // scheme 0 = Euler, 1 = midpoint, 2 = RK4.
-template <class Model, class T>
-__device__ void step(const T* x, const T* u, const float* p, scalar_t<T> dt, int scheme, T* out) {
+template <class Model, class T, class PA>
+__device__ void step(const T* x, const T* u, PA p, scalar_t<T> dt, int scheme, T* out) {
   using S = scalar_t<T>;
   constexpr int NX = Model::NX;
   T k1[NX], xt[NX];

@@ -985,6 +985,11 @@ int plain_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t
                      a.uref, a.xf, a.wx, a.wu, a.wf, q.xn, q.un, q.lam, a.g_wx, a.g_wu, a.g_wf, a.g_x0, a.g_xref,
                      a.g_uref, a.g_xf);
   hipLaunchKernelGGL(copy_kernel, dim3(nblk(nst)), dim3(256), 0, st, nst, q.lam, a.lam_bwd);
+  if (a.adj_x) hipLaunchKernelGGL(copy_kernel, dim3(nblk(nst)), dim3(256), 0, st, nst, q.xn, a.adj_x);
+  if (a.adj_u) {
+    const long long nu_all = (long long)B * (N + 1) * nu;
+    hipLaunchKernelGGL(copy_kernel, dim3(nblk(nu_all)), dim3(256), 0, st, nu_all, q.un, a.adj_u);
+  }
   HIPCHK(hipMemcpyAsync(a.pcg_iters, q.iters, sizeof(int) * B, hipMemcpyDeviceToDevice, st));
   HIPCHK(hipGetLastError());
   return DMPC_OK;

@@ -653,15 +653,21 @@ __global__ void __launch_bounds__(512) stream_bwd_kernel(BackwardArgs a) {
       const double ur = a.penalize ? c.uref[tu * NU + i] : 0.0;
       w.DU[tu * NU + i] = (tu < N) ? 2.0 * (us[tu * NU + i] - ur) * ut : 0.0;
       a.g_uref[((size_t)b * NBs + tu) * NU + i] = (tu < N && a.penalize) ? 2.0 * c.wu[i] * ut : 0.0;
+      if (a.adj_u) a.adj_u[((size_t)b * NBs + tu) * NU + i] = (tu < N) ? ut : 0.0;
     }
     a.lam_bwd[((size_t)b * NBs + t) * NX + i] = lt;
     if (t == 0) a.g_x0[b * NX + i] = lt;
+    if (a.adj_x) a.adj_x[((size_t)b * NBs + t) * NX + i] = xt;
   }
   for (int k = NB * NX + threadIdx.x; k < NBs * NX; k += blockDim.x) {
     a.g_xref[(size_t)b * NBs * NX + k] = 0.0;
     a.lam_bwd[(size_t)b * NBs * NX + k] = 0.0;
+    if (a.adj_x) a.adj_x[(size_t)b * NBs * NX + k] = 0.0;
   }
-  for (int k = NB * NU + threadIdx.x; k < NBs * NU; k += blockDim.x) a.g_uref[(size_t)b * NBs * NU + k] = 0.0;
+  for (int k = NB * NU + threadIdx.x; k < NBs * NU; k += blockDim.x) {
+    a.g_uref[(size_t)b * NBs * NU + k] = 0.0;
+    if (a.adj_u) a.adj_u[(size_t)b * NBs * NU + k] = 0.0;
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < NX + NU; i += blockDim.x) {
     double acc = 0.0;

@@ -32,6 +32,9 @@ class Dynamics:
 
     model_id: int = -1
     num_params: int = 0
+    # problem-parameter keys that flat_params reads: the dynamics parameters
+    # a solve can differentiate through when they are passed as weights
+    param_keys: tuple = ()
 
     def __init__(self, parameters: Dict[str, Any] = default_parameters):
         self._params = parameters

@@ -18,6 +18,7 @@ default_state_dot_parameters: Dict[str, Any] = {"masscart": 1.0, "masspole": 0.1
 class CartpoleDynamics(Dynamics):
     model_id = MODEL_CARTPOLE
     num_params = 4
+    param_keys = ("masscart", "masspole", "length", "gravity")
 
     def __init__(self, parameters: Dict[str, Any] = default_parameters):
         super().__init__(parameters)

@@ -24,6 +24,7 @@ chain_parameters: Dict[str, Any] = {
 class ChainDynamics(Dynamics):
     model_id = MODEL_CHAIN
     num_params = 2 * M
+    param_keys = ("spring_constants", "damping")
 
     def __init__(self, parameters: Dict[str, Any] = chain_parameters):
         super().__init__(parameters)

@@ -23,6 +23,7 @@ class LinearDynamics(Dynamics):
     """Linear dynamics class: state_dot = A @ state + B @ control + b."""
 
     model_id = MODEL_LINEAR
+    param_keys = ("A", "B", "b")
 
     def __init__(self, parameters: Dict[str, Any] = default_parameters):
         super().__init__(parameters)

@@ -63,6 +63,7 @@ class QuadrotorDynamics(Dynamics):
 
     model_id = MODEL_QUADROTOR
     num_params = 10
+    param_keys = ("mass", "inertia")
 
     def __init__(self, parameters: Dict[str, Any] = quadrotor_parameters):
         super().__init__(parameters)

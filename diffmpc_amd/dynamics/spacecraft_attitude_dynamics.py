@@ -25,6 +25,7 @@ spacecraft_attitude_state_dot_parameters = {"inertia_vector": torch.tensor([1.0,
 class SpacecraftAttitudeDynamics(Dynamics):
     model_id = MODEL_SPACECRAFT7
     num_params = 3
+    param_keys = ("inertia_vector",)
 
     def __init__(self, parameters: Dict[str, Any] = spacecraft_attitude_parameters):
         super().__init__(parameters)

@@ -24,6 +24,7 @@ class SpacecraftDynamics(Dynamics):
 
     model_id = MODEL_SPACECRAFT3
     num_params = 3
+    param_keys = ("inertia_vector",)
 
     def __init__(self, parameters: Dict[str, Any] = spacecraft_parameters):
         super().__init__(parameters)

@@ -169,7 +169,8 @@ class _SQPSolveFn(torch.autograd.Function):
         ctx.saved_lin = (AB, fd, qinv, rinv, work, nwork)
         ctx.dual_bwd_in = dual_bwd_in
         ctx.dual_bwd_out = dual_bwd_out
-        ctx.save_for_backward(x, u)
+        ctx.dyn_dtype = dyn.dtype
+        ctx.save_for_backward(x, u, lam)
         ctx.mark_non_differentiable(lam, status, num_iter, conv_err, conv_errs, ls_alphas, pcg_iters, AB, fd, qinv,
                                     rinv, dual_bwd_out)
         return x, u, lam, status, num_iter, conv_err, conv_errs, ls_alphas, pcg_iters, AB, fd, qinv, rinv, dual_bwd_out
@@ -177,7 +178,7 @@ class _SQPSolveFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g_x, g_u, *unused):
         lib = _lib.load()
-        x, u = ctx.saved_tensors
+        x, u, lam = ctx.saved_tensors
         prob, sp, mode = ctx.prob_args
         B = x.shape[0]
         N = prob.horizon
@@ -196,9 +197,14 @@ class _SQPSolveFn(torch.autograd.Function):
             ("w_x", (B, nx)), ("w_u", (B, nu)), ("w_f", (B, nx)), ("x0", (B, nx)), ("x_ref", (B, N + 1, nx)),
             ("u_ref", (B, N + 1, nu)), ("x_final", (B, nx)))}
         iters = torch.zeros(B, dtype=torch.int32, device=dev)
+        want_dyn = ctx.needs_input_grad[9]
+        if want_dyn:  # the adjoint primal solution feeds the dynamics-parameter contraction
+            adj_x = torch.empty(B, N + 1, nx, dtype=torch.float64, device=dev)
+            adj_u = torch.empty(B, N + 1, nu, dtype=torch.float64, device=dev)
         grads = _lib.Grads(_lib.ptr(g["w_x"]), _lib.ptr(g["w_u"]), _lib.ptr(g["w_f"]), _lib.ptr(g["x0"]),
                            _lib.ptr(g["x_ref"]), _lib.ptr(g["u_ref"]), _lib.ptr(g["x_final"]), _lib.ptr(lam_b),
-                           _lib.ptr(iters))
+                           _lib.ptr(iters), _lib.ptr(adj_x) if want_dyn else None,
+                           _lib.ptr(adj_u) if want_dyn else None)
         data = _lib.ProblemData(*[_lib.ptr(t) for t in ctx.data_t])
         AB, fd, qinv, rinv, work, nwork = ctx.saved_lin
         saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv),
@@ -206,11 +212,19 @@ class _SQPSolveFn(torch.autograd.Function):
         rc = lib.dmpc_sqp_backward(prob, sp, data, _lib.ptr(x), _lib.ptr(u), saved, _lib.ptr(dl_dx), _lib.ptr(dl_du),
                                    grads, mode, _lib.stream_ptr(dev))
         _lib.check(rc, "dmpc_sqp_backward")
+        g_dyn = None
+        if want_dyn:
+            g_dyn64 = torch.zeros(B, prob.n_params, dtype=torch.float64, device=dev)
+            rc = lib.dmpc_dynamics_param_grads(prob, data, _lib.ptr(x), _lib.ptr(u), _lib.ptr(lam), _lib.ptr(adj_x),
+                                               _lib.ptr(adj_u), _lib.ptr(lam_b), _lib.ptr(g_dyn64),
+                                               _lib.stream_ptr(dev))
+            _lib.check(rc, "dmpc_dynamics_param_grads")
+            g_dyn = g_dyn64.to(ctx.dyn_dtype)
         if ctx.dual_bwd_in is not None:
             with torch.no_grad():
                 ctx.dual_bwd_in.copy_(lam_b.to(ctx.dual_bwd_in.dtype))
         ctx.solver.last_backward_pcg_iters = iters
-        return (None, None, g["x0"], g["x_final"], g["x_ref"], g["u_ref"], g["w_x"], g["w_u"], g["w_f"], None, None)
+        return (None, None, g["x0"], g["x_final"], g["x_ref"], g["u_ref"], g["w_x"], g["w_u"], g["w_f"], g_dyn, None)
 
 
 class SQPSolver:
@@ -297,11 +311,12 @@ class SQPSolver:
         if problem_params is None:
             problem_params = self.problem_params
         weights = {} if weights is None else weights
-        for k in weights:
-            if k not in WEIGHT_KEYS:
-                raise NotImplementedError(f"no gradient rule for weights key {k!r}; differentiable keys: "
-                                          f"{sorted(WEIGHT_KEYS)}")
         prog = self.program
+        dyn_keys = set(prog.dynamics.param_keys)
+        for k in weights:
+            if k not in WEIGHT_KEYS and k not in dyn_keys:
+                raise NotImplementedError(f"no gradient rule for weights key {k!r}; differentiable keys: "
+                                          f"{sorted(WEIGHT_KEYS) + sorted(dyn_keys)}")
         N, nx, nu = prog.horizon, prog.num_state_variables, prog.num_control_variables
         states = torch.as_tensor(initial_guess.states)
         unbatched = states.dim() == 2
@@ -361,7 +376,9 @@ class SQPSolver:
         wx = b("weights_penalization_reference_state_trajectory", 1)
         wu = b("weights_penalization_control_squared", 1)
         wf = b("weights_penalization_final_state", 1)
-        dyn = prog.dynamics.flat_params({k: (v.detach() if isinstance(v, torch.Tensor) else v)
+        # dynamics parameters passed as weights keep their graph: the solve's
+        # gradient with respect to the flat parameter block flows back to them
+        dyn = prog.dynamics.flat_params({k: (v.detach() if isinstance(v, torch.Tensor) and k not in weights else v)
                                          for k, v in params.items()}, B, device)
         horizons = _ragged_horizons(params.get("horizons"), B, N, device)
         out = _SQPSolveFn.apply(self, g, x0, xf, xref, uref, wx, wu, wf, dyn, horizons)

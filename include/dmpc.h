@@ -16,6 +16,8 @@
  *   dmpc_pcg_gamma      PCGOptimalControl.compute_gamma        diffmpc/solvers/pcg.py:245-288
  *   dmpc_pcg_solve      PCGOptimalControl.solve_KKT_Schur      diffmpc/solvers/pcg.py:350-445
  *   dmpc_pcg_recover    get_states_controls_from_kkt_mult...   diffmpc/solvers/pcg.py:290-348
+ *   dmpc_dynamics_param_grads  _compute_mixed_derivatives + dL_dtheta for
+ *                       dynamics parameters among the weights  diffmpc/solvers/sqp.py:605-660, 714-737
  */
 #ifndef DMPC_H_
 #define DMPC_H_
@@ -123,6 +125,12 @@ typedef struct dmpc_grads { /* d loss / d input, per instance */
   double* x_final; /* (B, nx) */
   double* dual_backward; /* (B, N+1, nx) in: adjoint warm start, out: lambda~ */
   int32_t* pcg_iters;    /* (B,) */
+  /* Optional (NULL = not written): primal part of the adjoint KKT solution,
+   * x~ (B, N+1, nx) and u~ (B, N+1, nu) with a zero last row -- the
+   * linsys_sol.states / .controls of sqp.py:703-737; the input of
+   * dmpc_dynamics_param_grads. */
+  double* adj_states;
+  double* adj_controls;
 } dmpc_grads;
 
 /* Library version and build target ("gfx950"). */
@@ -141,6 +149,21 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
 int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
                       const double* states, const double* controls, const dmpc_saved* saved, const double* dl_dstates,
                       const double* dl_dcontrols, dmpc_grads* grads, int32_t mode, void* stream);
+
+/* d loss / d dynamics parameters, per instance: the contraction of
+ * sqp.py:714-737 for parameters that enter only the dynamics constraints
+ * g_{t+1} = f_d(x_t, u_t; p) - x_{t+1} (optimal_control_problem.py:104-151):
+ *   grads[b][j] = -sum_{t < N_b} ( dual_{t+1} . d/dp_j (J_t [adj_x_t; adj_u_t])
+ *                                  + adj_dual_{t+1} . d f_d / dp_j ),
+ * J_t = d f_d / d(x_t, u_t) at (states, controls), evaluated in float64
+ * hyper-dual arithmetic at the model's float32 parameters.  states, controls,
+ * dual: the forward solution; adj_*: the adjoint solution written by
+ * dmpc_sqp_backward (adj_dual = grads->dual_backward).  grads: (B, n_params),
+ * the flat layout of DMPC_MODEL_*; columns past the model's own parameter
+ * count are not written.  data uses dyn_params and horizons only. */
+int dmpc_dynamics_param_grads(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,
+                              const double* controls, const double* dual, const double* adj_states,
+                              const double* adj_controls, const double* adj_dual, double* grads, void* stream);
 
 /* f_d, A, B, C at (states, controls): fd (B,N,nx), A (B,N,nx,nx), Bm (B,N,nx,nu), C (B,N+1,nx). */
 int dmpc_linearize(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,

@@ -101,7 +101,7 @@ def quadrotor_state_dot(x, u, p):
     """
     m = p[..., 0]
     J = p[..., 1:10].reshape(p.shape[:-1] + (3, 3))
-    Jinv = np.linalg.inv(J.real) if np.iscomplexobj(J) else np.linalg.inv(J)
+    Jinv = np.linalg.inv(J)  # complex J (complex-step in the inertia) stays complex
     v = x[..., 3:6]
     q = x[..., 6:10]
     w = x[..., 10:13]

@@ -34,8 +34,10 @@ def test_solver_rejects_other_programs_and_unknown_weight_keys():
         SQPSolver(program=object(), params=load_solver_params("sqp.yaml"))
     p = load_problem_params("spacecraft.yaml")
     solver = SQPSolver(OptimalControlProblem(SpacecraftDynamics(), p), params=load_solver_params("sqp.yaml"))
+    # dynamics parameters (here inertia_vector) are differentiable keys; the
+    # step size is not
     with pytest.raises(NotImplementedError):
-        solver.solve(solver.initial_guess(), p, {"inertia_vector": p["inertia_vector"]})
+        solver.solve(solver.initial_guess(), p, {"discretization_resolution": p["discretization_resolution"]})
 
 
 def test_initial_guess_matches_oracle():
