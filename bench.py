@@ -48,10 +48,12 @@ def main():
                     help="time only the fused path (for profiler runs); prints the fused figures")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device-to-device once (reference kernel for HBM counters)")
-    ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain", "rollout"], default="quadrotor",
+    ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain", "rollout", "dynparams"],
+                    default="quadrotor",
                     help="quadrotor: the headline line (default); ragged: mixed batch of 8192 with horizons "
                          "U{10..200} in one graph; chain: 48-state chain B=512 N=256; rollout: the reference's "
-                         "RL benchmark (LTI 8/4, N=39, 50 closed-loop steps, B=--batch or 64)")
+                         "RL benchmark (LTI 8/4, N=39, 50 closed-loop steps, B=--batch or 64); dynparams: the "
+                         "headline workload with gradients w.r.t. mass and inertia as well")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -81,6 +83,8 @@ def main():
         return bench_chain(args)
     if args.workload == "rollout":
         return bench_rollout(args)
+    if args.workload == "dynparams":
+        return bench_dynparams(args)
 
     dev = torch.device("cuda:0")
     name, prob, sp = W.quadrotor(batch=args.batch, seed=1, horizon=args.horizon)
@@ -395,6 +399,62 @@ def bench_rollout(args):
         "fwd": {"ms": res["fwd"] * 1e3, "solves_per_s": solves / res["fwd"],
                 "vs_reference_jax_gpu": (RL_REFERENCE_MS["fwd"] / (res["fwd"] * 1e3)) if B == 64 else None},
         "reference_jax_gpu_ms": RL_REFERENCE_MS,
+    }), flush=True)
+
+
+def bench_dynparams(args):
+    """Headline workload (quadrotor B=1024 N=50, 5 SQP iterations) with the
+    gradient w.r.t. the per-instance mass and inertia next to the cost
+    weights and x0: forward + backward + the dynamics-parameter contraction."""
+    import torch
+
+    from diffmpc_amd import _lib
+    from diffmpc_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    name, prob, sp = W.quadrotor(batch=args.batch, seed=1, horizon=args.horizon)
+    solver, params, guess = W.make_solver(name, prob, sp, device=dev)
+    keys = ["weights_penalization_reference_state_trajectory", "weights_penalization_control_squared",
+            "initial_state", "mass", "inertia"]
+
+    def fwd_bwd(with_dyn):
+        guess.dual_backward_guess.zero_()
+        ks = keys if with_dyn else keys[:3]
+        w = {k: params[k].clone().requires_grad_(True) for k in ks}
+        sol = solver.solve(guess, params, w)
+        loss = (sol.states - params["reference_state_trajectory"]).square().sum()
+        return torch.autograd.grad(loss, [w[k] for k in ks])
+
+    lib = _lib.load()
+    res = {}
+    for with_dyn in (False, True):
+        for _ in range(args.warmup):
+            fwd_bwd(with_dyn)
+        torch.cuda.synchronize()
+        _lib.profile_read(_lib.PROF_DYN_GRADS)
+        _lib.profile_read(_lib.PROF_BACKWARD)
+        lib.dmpc_profile_enable(1)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fwd_bwd(with_dyn)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        lib.dmpc_profile_enable(0)
+        dg_ms, dg_n = _lib.profile_read(_lib.PROF_DYN_GRADS)
+        bw_ms, bw_n = _lib.profile_read(_lib.PROF_BACKWARD)
+        res[with_dyn] = (dt, dg_ms / max(dg_n, 1), bw_ms / max(bw_n, 1))
+    B = args.batch
+    t_dyn, dg_avg, bw_avg = res[True]
+    print(json.dumps({
+        "metric": "quadrotor fwd+bwd grads/s incl. d/d(mass, inertia) (T=50 B=1024)",
+        "value": B / t_dyn, "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_dyn * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 dynamics/linearization, f64 Schur/PCG, f64 hyper-dual parameter contraction",
+        "data": "synthetic",
+        "config": {"workload": f"quadrotor nx=13 nu=4 T={args.horizon} B={B}, 5 SQP iterations, grads w.r.t. "
+                               "w_x, w_u, x0, mass, inertia", "global_batch": B},
+        "without_dynamics_params_ms": res[False][0] * 1e3,
+        "dyn_param_grad_kernel_ms": dg_avg, "backward_kernel_ms": bw_avg,
     }), flush=True)
 
 

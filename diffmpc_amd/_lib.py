@@ -113,6 +113,7 @@ _SIGNATURES = {
 PROF_QP_ITERATION = 0
 PROF_BACKWARD = 1
 PROF_LINEARIZE = 2
+PROF_DYN_GRADS = 3
 
 
 def profile_read(kernel):

@@ -65,7 +65,9 @@ template <class Model>
 int launch(const DynGradArgs& a, hipStream_t st) {
   int threads = ((a.N + 63) / 64) * 64;
   threads = threads > 256 ? 256 : threads;
+  prof_begin(PROF_DYN_GRADS, st);
   hipLaunchKernelGGL(dyn_param_grad_kernel<Model>, dim3(a.B, Model::NPARAM), dim3(threads), 0, st, a);
+  prof_end(PROF_DYN_GRADS, st);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
 }
 

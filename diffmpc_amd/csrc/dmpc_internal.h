@@ -98,7 +98,7 @@ extern int g_fused_variant;
 size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_alphas, int variant);
 
 // optional HIP-event timing of individual kernel launches (dmpc_profile_*)
-enum ProfKernel { PROF_QP_ITERATION = 0, PROF_BACKWARD = 1, PROF_LINEARIZE = 2, PROF_NKERNELS = 3 };
+enum ProfKernel { PROF_QP_ITERATION = 0, PROF_BACKWARD = 1, PROF_LINEARIZE = 2, PROF_DYN_GRADS = 3, PROF_NKERNELS = 4 };
 void prof_begin(int kernel, hipStream_t s);
 void prof_end(int kernel, hipStream_t s);
 

@@ -203,7 +203,8 @@ int dmpc_set_fused_variant(int32_t v);
 /* Instrumentation (no reference counterpart): when enabled, the library
  * records a HIP event pair on the launch stream around every launch of the
  * fused kernels; dmpc_profile_read sums the elapsed times of one kernel
- * (0 = QP iteration, 1 = backward, 2 = linearize) and clears them. */
+ * (0 = QP iteration, 1 = backward, 2 = linearize, 3 = dynamics-parameter
+ * contraction) and clears them. */
 int dmpc_profile_enable(int32_t on);
 int dmpc_profile_read(int32_t kernel, double* total_ms, int32_t* launches);
 
