@@ -108,6 +108,12 @@ __device__ Lds carve(void* base, int N, int ls = 0) {
 template <int NX>
 __host__ __device__ constexpr bool d_in_lds() { return NX > 8; }
 
+// 16-lane groups with four controls keep B's rows column-permuted per quad
+// of lanes and form the B-terms by reduce-scatter (rs16/gather16); other
+// shapes use one group all-reduce per column.
+template <int G, int NU>
+__host__ __device__ constexpr bool quad_bterm() { return G == 16 && NU == 4; }
+
 // Per-thread register state of the K blocks it owns.
 template <int NX, int NU, int K>
 struct Regs {
@@ -116,7 +122,8 @@ struct Regs {
   using AT = typename std::conditional<K == 1, double, float>::type;
   AT acol[K][NX];   // column `lane` of A_t            (zero for t = N)
   AT arow[K][NX];   // row `lane` of A_{t-1}           (zero for t = 0)
-  AT brow[K][NU];   // row `lane` of B_{t-1}           (zero for t = 0)
+  AT brow[K][NU];   // row `lane` of B_{t-1}           (zero for t = 0;
+                    // slot r = column quad16(lane) ^ r with quad_bterm)
                        // (B_{t-1} terms of S are formed by block t alone)
   double drow[K][d_in_lds<NX>() ? 1 : NX];  // row `lane` of D_t = theta_t^-1 (registers mode)
   double qi[K];        // Qinv_t[lane, lane]
@@ -125,9 +132,10 @@ struct Regs {
 };
 
 // Load A/B rows and columns of the saved linearization into registers.
-template <int NX, int NU, int K>
+template <int G, int NX, int NU, int K>
 __device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, const Lds& L, int N, int lane) {
   constexpr int NZ = NX + NU;
+  const int bq = quad_bterm<G, NU>() ? quad16(lane) : 0;  // column permutation of the B row
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     const int t = R.t[s];
@@ -139,7 +147,7 @@ __device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, co
 #pragma unroll
     for (int k = 0; k < NX; ++k) R.arow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + k] : 0.0f;
 #pragma unroll
-    for (int k = 0; k < NU; ++k) R.brow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + NX + k] : 0.0f;
+    for (int k = 0; k < NU; ++k) R.brow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + NX + (k ^ bq)] : 0.0f;
     R.qi[s] = ok ? L.qinv[t * NX + lane] : 0.0;
   }
 }
@@ -273,6 +281,13 @@ __device__ __forceinline__ double own(const double* V, int tb, int lane) {
 template <int G, int NX, int NU, int K>
 __device__ __forceinline__ double bgram(const Regs<NX, NU, K>& R, int s, const Lds& L, double v, int salt,
                                         double* quad) {
+  if constexpr (quad_bterm<G, NU>()) {
+    // the quadratic form is accumulated by the caller as v . g over the lanes
+    const int lane = threadIdx.x & 15;
+    const double y = L.rinv[quad16(lane)] * rs16(R.brow[s], v, salt);
+    *quad = 0.0;
+    return gather16(R.brow[s], y, salt);
+  }
   double g = 0.0, q = 0.0;
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
@@ -368,7 +383,13 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
       if (R.ok[s]) {
         L.YX[(t + 1) * NX + lane] = yxo[s];
         vpart += wx * yxo[s];
-        if (lead) vpart += q;  // u_{t-1} term, once per block
+        // u_{t-1} term p^T B Rinv B^T p: once per block, or spread over
+        // the lanes as p_i (B Rinv B^T p)_i
+        if constexpr (quad_bterm<G, NU>()) {
+          vpart += pv * gb[s];
+        } else {
+          if (lead) vpart += q;
+        }
       }
     }
     const double v = -block_sum(vpart, L.redA);
@@ -415,7 +436,7 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
 
 // gamma = C + G H^-1 q (pcg.py:245-288) for the rows this thread owns.
 // WITH_C: C = [x0; -f_d + A xbar + B ubar] (forward); otherwise C = 0.
-template <bool WITH_C, int NX, int NU, int K>
+template <bool WITH_C, int G, int NX, int NU, int K>
 __device__ __forceinline__ void gamma_rows(const Regs<NX, NU, K>& R, const Lds& L, int lane, double (&gam)[K]) {
 #pragma unroll
   for (int s = 0; s < K; ++s) {
@@ -434,10 +455,12 @@ __device__ __forceinline__ void gamma_rows(const Regs<NX, NU, K>& R, const Lds& 
           if (WITH_C) v += L.xb[tp * NX + k];
           g += (double)R.arow[s][k] * v;
         }
+        const int bq = quad_bterm<G, NU>() ? quad16(lane) : 0;  // brow slot k is column k ^ bq
 #pragma unroll
         for (int k = 0; k < NU; ++k) {
-          double v = L.rinv[k] * L.rv[tp * NU + k];
-          if (WITH_C) v += L.ub[tp * NU + k];
+          const int c = k ^ bq;
+          double v = L.rinv[c] * L.rv[tp * NU + c];
+          if (WITH_C) v += L.ub[tp * NU + c];
           g += (double)R.brow[s][k] * v;
         }
       }
@@ -461,10 +484,18 @@ __device__ __forceinline__ void recover_rows(const Regs<NX, NU, K>& R, const Lds
     const double ln = own<NX>(L.lamS, t + 1, lane);
     const double ax = gdot<G, NX>(R.acol[s], ln, lane, 0);
     double bl = 0.0;  // component `lane` of B_{t-1}^T lam_t
+    if constexpr (quad_bterm<G, NU>()) {
+      // quad k holds component k; lanes 1, 2, 3 reach quads 1, 2, 3 with
+      // row_half_mirror, row_ror:8 and row_mirror
+      const double sq = rs16(R.brow[s], lt, 0);
+      const double s1 = dpp_mov<kDppHalfMirror>(sq), s2 = dpp_mov<kDppRor8>(sq), s3 = dpp_mov<kDppMirror>(sq);
+      bl = lane == 0 ? sq : lane == 1 ? s1 : lane == 2 ? s2 : lane == 3 ? s3 : 0.0;
+    } else {
 #pragma unroll
-    for (int k = 0; k < NU; ++k) {
-      const double sk = gsum<G>((double)R.brow[s][k] * lt);
-      bl = (lane == k) ? sk : bl;
+      for (int k = 0; k < NU; ++k) {
+        const double sk = gsum<G>((double)R.brow[s][k] * lt);
+        bl = (lane == k) ? sk : bl;
+      }
     }
     xn[s] = 0.0;
     un[s] = 0.0;
@@ -603,9 +634,9 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
 
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
   assemble_dinv(R, ABb, L, lane, gbase);
-  load_ab(R, ABb, L, N, lane);
+  load_ab<G>(R, ABb, L, N, lane);
   double gam[K], lam[K];
-  gamma_rows<true>(R, L, lane, gam);
+  gamma_rows<true, G>(R, L, lane, gam);
   double eta;
   const int npcg = pcg<G>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta);
 #pragma unroll
@@ -730,9 +761,9 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   __syncthreads();
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
   assemble_dinv(R, ABb, L, lane, gbase);
-  load_ab(R, ABb, L, N, lane);
+  load_ab<G>(R, ABb, L, N, lane);
   double gam[K], lam[K];
-  gamma_rows<false>(R, L, lane, gam);
+  gamma_rows<false, G>(R, L, lane, gam);
   double eta;
   const int npcg = pcg<G>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta);
 #pragma unroll
