@@ -148,6 +148,17 @@ __device__ __forceinline__ double gbcast(double v, int lane) {
   }
 }
 
+// gbcast with a source lane known only after loop unrolling (a constant
+// there, so the chain of tests folds away).
+template <int G, int n = 0>
+__device__ __forceinline__ double gbcast_at(double v, int src, int lane) {
+  if constexpr (n == G - 1) {
+    return gbcast<G, n>(v, lane);
+  } else {
+    return src == n ? gbcast<G, n>(v, lane) : gbcast_at<G, n + 1>(v, src, lane);
+  }
+}
+
 // sum_j a[j] * v_j where v_j lives in lane j of the group (j < M <= NX).
 template <int G, int M, int j = 0, class AT>
 __device__ __forceinline__ double gdot_chain(const AT* a, double v, int lane, int salt, double acc) {

@@ -159,7 +159,7 @@ __device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, co
 // (pcg.py:182-196 with As_next = -I).  The factor is computed in registers
 // with cross-lane shuffles, parked in LDS, and each lane then solves
 // L L^T d = e_lane for its row of D_t.
-template <int NX, int NU, int K>
+template <int G, int NX, int NU, int K>
 __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* ABb, const Lds& L, int lane,
                                               int gbase) {
   constexpr int NZ = NX + NU;
@@ -190,10 +190,11 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* A
         row[j] = ok ? acc : ((j == lane) ? 1.0 : 0.0);
       }
     }
-    // right-looking Cholesky: lane i ends with row[k] = L[i][k], k <= i
+    // right-looking Cholesky: lane i ends with row[k] = L[i][k], k <= i;
+    // pivots and column entries reach the group's lanes by DPP broadcast
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
-      const double dkk = __shfl(row[k], gbase + k, 64);
+      const double dkk = gbcast_at<G>(row[k], k, lane);
       const double lkk = sqrt(dkk);
       const double il = 1.0 / lkk;
       if (ok && lane == k) L.ID[t * NX + k] = il;
@@ -201,7 +202,7 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* A
       row[k] = lik;
 #pragma unroll
       for (int j = k + 1; j < NX; ++j) {
-        const double ljk = __shfl(lik, gbase + j, 64);
+        const double ljk = gbcast_at<G>(lik, j, lane);
         row[j] -= lik * ljk;
       }
     }
@@ -633,7 +634,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   __syncthreads();
 
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
-  assemble_dinv(R, ABb, L, lane, gbase);
+  assemble_dinv<G>(R, ABb, L, lane, gbase);
   load_ab<G>(R, ABb, L, N, lane);
   double gam[K], lam[K];
   gamma_rows<true, G>(R, L, lane, gam);
@@ -760,7 +761,7 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   for (int k = threadIdx.x; k < NU; k += blockDim.x) L.rinv[k] = a.rinv[(size_t)b * NU + k];
   __syncthreads();
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
-  assemble_dinv(R, ABb, L, lane, gbase);
+  assemble_dinv<G>(R, ABb, L, lane, gbase);
   load_ab<G>(R, ABb, L, N, lane);
   double gam[K], lam[K];
   gamma_rows<false, G>(R, L, lane, gam);
