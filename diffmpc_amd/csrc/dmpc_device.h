@@ -58,16 +58,19 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Sum over the workgroup through one barrier; every thread gets the same,
-// bit-identical result (same summation order everywhere).
+// Sum over the workgroup (at most 16 waves) through one barrier; every
+// thread gets the same, bit-identical result.  After the barrier lane i of
+// each 16-lane row reads wave i's sum (one LDS access per lane, all in
+// parallel) and the row sums them with the DPP butterfly of gsum: pairwise
+// additions whose operands are only swapped between lanes, so every lane
+// ends with the same bits.
 __device__ __forceinline__ double block_sum(double v, double* red) {
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   const int nw = blockDim.x >> 6;
-  double s = 0.0;
-  for (int k = 0; k < nw; ++k) s += red[k];
-  return s;
+  const int k = threadIdx.x & 15;
+  return gsum<16>(k < nw ? red[k] : 0.0);
 }
 
 __device__ __forceinline__ double block_max(double v, double* red) {

@@ -39,7 +39,7 @@ struct DynGradArgs {
 template <class Model>
 __global__ void __launch_bounds__(256) dyn_param_grad_kernel(DynGradArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU;
-  __shared__ double red[4];
+  __shared__ double red[16];
   const int b = blockIdx.x, j = blockIdx.y;
   const int NBs = a.N + 1;
   const int N = horizon_of(a.horizons, a.N, b);

@@ -707,7 +707,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
 
 // status and nan_to_num (sqp.py:535-559)
 __global__ void __launch_bounds__(256) sqp_finish_kernel(SolveArgs a, int nx, int nu) {
-  __shared__ double red[4];
+  __shared__ double red[16];
   const int b = blockIdx.x;
   const int NB = a.N + 1;
   double bad = 0.0;
