@@ -5,7 +5,7 @@ SQPSolver, SQPSolution); the solve and its adjoint run as HIP kernels in
 libdmpc.so behind the C ABI of include/dmpc.h.
 """
 from . import _lib  # noqa: F401
-from .dynamics import (CartpoleDynamics, ChainDynamics, Dynamics, LinearDynamics,  # noqa: F401
+from .dynamics import (CartpoleDynamics, ChainDynamics, CustomDynamics, Dynamics, LinearDynamics,  # noqa: F401
                        QuadrotorDynamics, SpacecraftAttitudeDynamics, SpacecraftDynamics)
 from .problems import OptimalControlProblem  # noqa: F401
 from .solvers import PCGOptimalControl, SolverReturnStatus, SQPSolution, SQPSolver  # noqa: F401

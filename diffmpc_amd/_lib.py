@@ -88,6 +88,7 @@ class Grads(ctypes.Structure):
 # name -> (restype, argtypes); the list is the public C ABI
 _SIGNATURES = {
     "dmpc_version": (ctypes.c_char_p, []),
+    "dmpc_custom_model_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)] * 3),
     "dmpc_supported": (ctypes.c_int, [ctypes.c_int32] * 5),
     "dmpc_sqp_solve": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(SolverParams), ctypes.POINTER(ProblemData),
                                       ctypes.POINTER(Solution), ctypes.POINTER(Saved), ctypes.c_int32, ctypes.c_void_p]),
@@ -131,29 +132,48 @@ class DmpcError(RuntimeError):
     pass
 
 
-def load():
-    """Load libdmpc.so (once); raise if it has not been built."""
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+_custom_libs = {}
+
+
+def _open(path):
+    if not os.path.exists(path):
         raise DmpcError(
-            f"{LIB_PATH} is missing: build it with diffmpc_amd/csrc/build.sh "
+            f"{path} is missing: build it with diffmpc_amd/csrc/build.sh "
             "(there is no CPU fallback for the solver)"
         )
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
     return lib
 
 
-def workspace_bytes(mode, prob, sp):
+def load(path=None):
+    """Load libdmpc.so (once), or the custom build at `path`; raise if it has
+    not been built."""
+    global _lib
+    if path is not None and os.path.abspath(path) != os.path.abspath(LIB_PATH):
+        key = os.path.abspath(path)
+        if key not in _custom_libs:
+            _custom_libs[key] = _open(key)
+        return _custom_libs[key]
+    if _lib is None:
+        _lib = _open(LIB_PATH)
+    return _lib
+
+
+def load_for(dynamics):
+    """The library that runs `dynamics`: the custom build for a user-defined
+    model (diffmpc_amd.dynamics.CustomDynamics), else libdmpc.so."""
+    path = getattr(dynamics, "library_path", None)
+    return load(path() if callable(path) else None)
+
+
+def workspace_bytes(mode, prob, sp, lib=None):
     """Device workspace (bytes) dmpc_sqp_solve / dmpc_sqp_backward need."""
     n = ctypes.c_uint64(0)
-    check(load().dmpc_workspace_bytes(mode, ctypes.byref(prob), ctypes.byref(sp), ctypes.byref(n)),
+    check((lib or load()).dmpc_workspace_bytes(mode, ctypes.byref(prob), ctypes.byref(sp), ctypes.byref(n)),
           "dmpc_workspace_bytes")
     return n.value
 

@@ -5,13 +5,17 @@
 # emit v_pk_* FP32 forms by itself.
 set -euo pipefail
 here="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
-out="${here}/../libdmpc.so"
+out="${DMPC_LIB_OUT:-${here}/../libdmpc.so}"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 obj="$(mktemp -d)"
 trap 'rm -rf "${obj}"' EXIT
 flags=(-O3 -std=c++17 --offload-arch=gfx950 -fPIC
        -Xclang -target-feature -Xclang -packed-fp32-ops
        -mcode-object-version=5 -Wall -Wno-unused-function)
+# custom build: DMPC_CUSTOM_MODEL_FILE names a header defining dmpc::CustomModel
+if [ -n "${DMPC_CUSTOM_MODEL_FILE:-}" ]; then
+  flags+=("-DDMPC_CUSTOM_MODEL_FILE=\"${DMPC_CUSTOM_MODEL_FILE}\"")
+fi
 pids=()
 for src in dmpc_fused dmpc_stream dmpc_plain dmpc_dyngrad dmpc_api; do
   "${HIPCC}" "${flags[@]}" -c "${here}/${src}.hip" -o "${obj}/${src}.o" &

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "dmpc_internal.h"
+#include "dmpc_models.h"
 
 namespace dmpc {
 int pcg_layer_assemble(int B, int N, int nx, int nu, const double* As_next, const double* As, const double* Bs,
@@ -80,6 +81,9 @@ int model_sizes(int model, int* nx, int* nu, int* np, int want_nx = 0, int want_
     case DMPC_MODEL_SPACECRAFT7: *nx = 7; *nu = 3; *np = 3; return 0;
     case DMPC_MODEL_CARTPOLE: *nx = 4; *nu = 1; *np = 4; return 0;
     case DMPC_MODEL_CHAIN: *nx = 48; *nu = 12; *np = 24; return 0;
+#ifdef DMPC_CUSTOM_MODEL_FILE
+    case DMPC_MODEL_CUSTOM: *nx = CustomModel::NX; *nu = CustomModel::NU; *np = CustomModel::NPARAM; return 0;
+#endif
     default: return -1;
   }
 }
@@ -112,6 +116,19 @@ bool linesearch_is_noop(const dmpc_solver_params* s) {
 extern "C" {
 
 const char* dmpc_version(void) { return "dmpc 0.1 gfx950"; }
+
+int dmpc_custom_model_info(int32_t* nx, int32_t* nu, int32_t* n_params) {
+  if (!nx || !nu || !n_params) return DMPC_ERR_ARG;
+#ifdef DMPC_CUSTOM_MODEL_FILE
+  *nx = CustomModel::NX;
+  *nu = CustomModel::NU;
+  *n_params = CustomModel::NPARAM;
+  return DMPC_OK;
+#else
+  *nx = *nu = *n_params = 0;
+  return DMPC_ERR_UNSUPPORTED;
+#endif
+}
 
 int dmpc_supported(int32_t mode, int32_t model, int32_t nx, int32_t nu, int32_t horizon) {
   int mx, mu, np;

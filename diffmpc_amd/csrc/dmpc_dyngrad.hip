@@ -101,6 +101,7 @@ int dyn_param_grads_any(const dmpc_problem* p, const dmpc_problem_data* d, const
     case MODEL_SPACECRAFT7: return launch<Spacecraft7Model>(a, st);
     case MODEL_CARTPOLE: return launch<CartpoleModel>(a, st);
     case MODEL_CHAIN: return launch<ChainModel>(a, st);
+    DMPC_CUSTOM_CASE(return launch<CustomModel>(a, st))
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }

@@ -950,6 +950,12 @@ bool register_fits(int model, int nx, int nu, int N, int lsd) {
     case MODEL_SPACECRAFT3: return nx == 3 && nu == 3 && fits<Spacecraft3Model, 4>(N, lsd);
     case MODEL_SPACECRAFT7: return nx == 7 && nu == 3 && fits<Spacecraft7Model, 8>(N, lsd);
     case MODEL_CARTPOLE: return nx == 4 && nu == 1 && fits<CartpoleModel, 4>(N, lsd);
+#ifdef DMPC_CUSTOM_MODEL_FILE
+    case MODEL_CUSTOM:
+      if constexpr (kCustomG <= 16)
+        return nx == CustomModel::NX && nu == CustomModel::NU && fits<CustomModel, kCustomG>(N, lsd);
+      return false;
+#endif
     default: return false;
   }
 }
@@ -1004,6 +1010,7 @@ int stream_launch_linearize(int model, const SolveArgs& a, hipStream_t st) {
     case MODEL_SPACECRAFT7: return launch_linearize<Spacecraft7Model>(a, st);
     case MODEL_CARTPOLE: return launch_linearize<CartpoleModel>(a, st);
     case MODEL_CHAIN: return launch_linearize<ChainModel>(a, st);
+    DMPC_CUSTOM_CASE(return launch_linearize<CustomModel>(a, st))
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }
@@ -1017,6 +1024,7 @@ int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st) {
     case MODEL_SPACECRAFT7: nx = Spacecraft7Model::NX; nu = Spacecraft7Model::NU; break;
     case MODEL_CARTPOLE: nx = CartpoleModel::NX; nu = CartpoleModel::NU; break;
     case MODEL_CHAIN: nx = ChainModel::NX; nu = ChainModel::NU; break;
+    DMPC_CUSTOM_CASE(nx = CustomModel::NX; nu = CustomModel::NU; break)
     default: return DMPC_ERR_UNSUPPORTED;
   }
   hipLaunchKernelGGL(sqp_finish_kernel, dim3(a.B), dim3(256), 0, st, a, nx, nu);
@@ -1035,6 +1043,11 @@ int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st)
     case MODEL_SPACECRAFT3: return fwd_model<Spacecraft3Model, 4>(a, st);
     case MODEL_SPACECRAFT7: return fwd_model<Spacecraft7Model, 8>(a, st);
     case MODEL_CARTPOLE: return fwd_model<CartpoleModel, 4>(a, st);
+#ifdef DMPC_CUSTOM_MODEL_FILE
+    case MODEL_CUSTOM:
+      if constexpr (kCustomG <= 16) return fwd_model<CustomModel, kCustomG>(a, st);
+      return DMPC_ERR_UNSUPPORTED;
+#endif
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }
@@ -1050,6 +1063,11 @@ int fused_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t
     case MODEL_SPACECRAFT3: return bwd_model<Spacecraft3Model, 4>(a, st);
     case MODEL_SPACECRAFT7: return bwd_model<Spacecraft7Model, 8>(a, st);
     case MODEL_CARTPOLE: return bwd_model<CartpoleModel, 4>(a, st);
+#ifdef DMPC_CUSTOM_MODEL_FILE
+    case MODEL_CUSTOM:
+      if constexpr (kCustomG <= 16) return bwd_model<CustomModel, kCustomG>(a, st);
+      return DMPC_ERR_UNSUPPORTED;
+#endif
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }

@@ -48,6 +48,12 @@ __device__ __forceinline__ float dsin(float a) { return sinf(a); }
 __device__ __forceinline__ float dcos(float a) { return cosf(a); }
 __device__ __forceinline__ double dsin(double a) { return sin(a); }
 __device__ __forceinline__ double dcos(double a) { return cos(a); }
+__device__ __forceinline__ Dual dexp(Dual a) {
+  const float e = expf(a.v);
+  return {e, a.d * e};
+}
+__device__ __forceinline__ float dexp(float a) { return expf(a); }
+__device__ __forceinline__ double dexp(double a) { return exp(a); }
 __device__ __forceinline__ float val(float a) { return a; }
 __device__ __forceinline__ float val(Dual a) { return a.v; }
 __device__ __forceinline__ float tan_(Dual a) { return a.d; }
@@ -85,6 +91,10 @@ __device__ __forceinline__ HD dsqrt(HD x) {
 __device__ __forceinline__ HD dsin(HD x) {
   const double s = sin(x.v), c = cos(x.v);
   return hd_apply(x, s, c, -s);
+}
+__device__ __forceinline__ HD dexp(HD x) {
+  const double e = exp(x.v);
+  return hd_apply(x, e, e, e);
 }
 __device__ __forceinline__ HD dcos(HD x) {
   const double s = sin(x.v), c = cos(x.v);
@@ -132,6 +142,7 @@ enum ModelId : int {
   MODEL_SPACECRAFT7 = 3,
   MODEL_CARTPOLE = 4,
   MODEL_CHAIN = 5,
+  MODEL_CUSTOM = 6,  // the user-defined model of a custom build (CustomModel)
 };
 
 // ---------------------------------------------------------------------------
@@ -358,3 +369,19 @@ __device__ void step_jacobian_column(const float* x, const float* u, const float
 }
 
 }  // namespace dmpc
+
+// A custom build compiles this library once more with -DDMPC_CUSTOM_MODEL_FILE
+// naming a header that defines dmpc::CustomModel (NX, NU, NPARAM and f, in
+// the form of the models above); the dispatchers then take MODEL_CUSTOM.
+#ifdef DMPC_CUSTOM_MODEL_FILE
+namespace dmpc {
+#include DMPC_CUSTOM_MODEL_FILE
+// block-group width of the custom model in the fused kernels
+constexpr int kCustomG = CustomModel::NX <= 4 ? 4 : CustomModel::NX <= 8 ? 8 : CustomModel::NX <= 16 ? 16 : 64;
+}  // namespace dmpc
+#define DMPC_CUSTOM_CASE(...) \
+  case MODEL_CUSTOM:          \
+    __VA_ARGS__;
+#else
+#define DMPC_CUSTOM_CASE(...)
+#endif

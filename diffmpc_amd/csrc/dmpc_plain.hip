@@ -952,6 +952,7 @@ int plain_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st)
     case MODEL_SPACECRAFT7: return plain_forward_t<Spacecraft7Model>(a, st);
     case MODEL_CARTPOLE: return plain_forward_t<CartpoleModel>(a, st);
     case MODEL_CHAIN: return plain_forward_t<ChainModel>(a, st);
+    DMPC_CUSTOM_CASE(return plain_forward_t<CustomModel>(a, st))
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }
@@ -1085,6 +1086,7 @@ int linearize_any(const dmpc_problem* p, const dmpc_problem_data* d, const doubl
     case MODEL_SPACECRAFT7: return linearize_t<Spacecraft7Model>(p, d, x, u, fd, A, Bm, C, st);
     case MODEL_CARTPOLE: return linearize_t<CartpoleModel>(p, d, x, u, fd, A, Bm, C, st);
     case MODEL_CHAIN: return linearize_t<ChainModel>(p, d, x, u, fd, A, Bm, C, st);
+    DMPC_CUSTOM_CASE(return linearize_t<CustomModel>(p, d, x, u, fd, A, Bm, C, st))
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }

@@ -780,6 +780,7 @@ bool stream_supported(int model, int nx, int nu) {
     case MODEL_SPACECRAFT7: return nx == 7 && nu == 3;
     case MODEL_CARTPOLE: return nx == 4 && nu == 1;
     case MODEL_CHAIN: return nx == 48 && nu == 12;
+    DMPC_CUSTOM_CASE(return nx == CustomModel::NX && nu == CustomModel::NU && CustomModel::NX <= 64)
     default: return false;
   }
 }
@@ -796,6 +797,7 @@ int stream_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st
     case MODEL_SPACECRAFT7: return stream_forward_t<Spacecraft7Model, 8>(model, a, st);
     case MODEL_CARTPOLE: return stream_forward_t<CartpoleModel, 4>(model, a, st);
     case MODEL_CHAIN: return stream_forward_t<ChainModel, 64>(model, a, st);
+    DMPC_CUSTOM_CASE(return stream_forward_t<CustomModel, kCustomG>(model, a, st))
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }
@@ -812,6 +814,7 @@ int stream_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_
     case MODEL_SPACECRAFT7: return stream_backward_t<Spacecraft7Model, 8>(a, st);
     case MODEL_CARTPOLE: return stream_backward_t<CartpoleModel, 4>(a, st);
     case MODEL_CHAIN: return stream_backward_t<ChainModel, 64>(a, st);
+    DMPC_CUSTOM_CASE(return stream_backward_t<CustomModel, kCustomG>(a, st))
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }

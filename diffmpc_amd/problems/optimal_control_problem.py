@@ -176,7 +176,7 @@ class OptimalControlProblem:
         """
         if params is None:
             params = self.params
-        lib = _lib.load()
+        lib = _lib.load_for(self.dynamics)
         B = states.shape[0]
         N, nx, nu = self.horizon, self.num_state_variables, self.num_control_variables
         dev = states.device

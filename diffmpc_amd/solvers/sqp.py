@@ -128,8 +128,8 @@ class _SQPSolveFn(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, solver, guess, x0, x_final, x_ref, u_ref, w_x, w_u, w_f, dyn, horizons):
-        lib = _lib.load()
         prog = solver.program
+        lib = _lib.load_for(prog.dynamics)
         B = guess["states"].shape[0]
         dev = guess["states"].device
         N, nx, nu = prog.horizon, prog.num_state_variables, prog.num_control_variables
@@ -154,7 +154,7 @@ class _SQPSolveFn(torch.autograd.Function):
         prob = prog.c_problem(B)
         sp = _c_solver_params(solver.params)
         mode = _mode(solver.params)
-        nwork = _lib.workspace_bytes(mode, prob, sp)
+        nwork = _lib.workspace_bytes(mode, prob, sp, lib)
         work = torch.empty(max(nwork, 1), dtype=torch.uint8, device=dev)
         saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv),
                            _lib.ptr(work) if nwork else None, nwork)
@@ -177,7 +177,7 @@ class _SQPSolveFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g_x, g_u, *unused):
-        lib = _lib.load()
+        lib = _lib.load_for(ctx.solver.program.dynamics)
         x, u, lam = ctx.saved_tensors
         prob, sp, mode = ctx.prob_args
         B = x.shape[0]

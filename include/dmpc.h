@@ -40,6 +40,9 @@ extern "C" {
 #define DMPC_MODEL_SPACECRAFT7 3 /* inertia diagonal (3)         (7/3)  quaternion + rate, no reference counterpart */
 #define DMPC_MODEL_CARTPOLE 4    /* masscart, masspole, length, gravity (4/1)  cartpole_dynamics.py:36-80 */
 #define DMPC_MODEL_CHAIN 5       /* spring k[12], damper c[12]   (48/12) planar chain, no reference counterpart */
+#define DMPC_MODEL_CUSTOM 6      /* the user-defined model a custom build was compiled with (any nx <= 64);
+                                    stands in for Dynamics subclasses with their own state_dot, whose
+                                    Jacobians the reference takes by autodiff (ocp.py:360-381) */
 
 #define DMPC_MODE_FUSED 0 /* one workgroup per instance per SQP iteration, no host synchronisation */
 #define DMPC_MODE_PLAIN 1 /* one kernel per operation, host-driven loops (the reference's op graph) */
@@ -135,6 +138,10 @@ typedef struct dmpc_grads { /* d loss / d input, per instance */
 
 /* Library version and build target ("gfx950"). */
 const char* dmpc_version(void);
+
+/* Sizes of the user-defined model (DMPC_MODEL_CUSTOM) of a custom build;
+ * DMPC_ERR_UNSUPPORTED in a build without one. */
+int dmpc_custom_model_info(int32_t* nx, int32_t* nu, int32_t* n_params);
 
 /* 1 if the mode supports (model, nx, nu, horizon), else 0. */
 int dmpc_supported(int32_t mode, int32_t model, int32_t nx, int32_t nu, int32_t horizon);

@@ -69,3 +69,61 @@ def dense_kkt(As_next, As, Bs, Qinv, qvec, Rinv, rvec, Cs):
     s = np.linalg.solve(K, np.concatenate([-q, Cs.reshape(-1)]))
     z = s[:nz].reshape(N + 1, nx + nu)
     return z[:, :nx], z[:N, nx:], s[nz:].reshape(N + 1, nx), H, G
+
+
+class OracleCustomModel(odyn.Model):
+    """Oracle model of a user-defined model: the user's own state_dot function
+    evaluated on numpy arrays (complex-step safe), float64."""
+
+    def __init__(self, dyn):
+        self.name = dyn.name
+        self.nx = dyn.num_states
+        self.nu = dyn.num_controls
+        self.model_id = dyn.model_id
+        self._fn = dyn._fn
+
+    def state_dot(self, x, u, p):
+        from diffmpc_amd.dynamics.custom_dynamics import LastAxis
+
+        out = self._fn(LastAxis(x), LastAxis(u), LastAxis(p), np)
+        shape = np.broadcast_shapes(x.shape[:-1], u.shape[:-1], p.shape[:-1])
+        return np.stack([np.broadcast_to(np.asarray(o), shape) for o in out], -1)
+
+
+def unicycle_problem(batch=4, seed=0, horizon=30):
+    """Reach a goal pose with the example unicycle: RK4, dt = 0.1, per-instance
+    turn gains; 10 SQP iterations to 1e-8, PCG 1e-10."""
+    rng = np.random.default_rng(seed)
+    B, N = batch, horizon
+    x0 = np.concatenate([rng.uniform(-1, 1, (B, 2)), rng.uniform(-0.5, 0.5, (B, 1))], 1)
+    goal = np.concatenate([rng.uniform(1, 2, (B, 2)), rng.uniform(-0.5, 0.5, (B, 1))], 1)
+    prob = dict(horizon=N, dt=0.1, scheme=2, penalize_control_reference=False, x0=x0, x_final=goal,
+                x_ref=np.repeat(goal[:, None], N + 1, 1), u_ref=np.zeros((B, N + 1, 2)),
+                w_x=np.tile([1.0, 1.0, 0.1], (B, 1)), w_u=np.tile([0.1, 0.1], (B, 1)),
+                w_f=np.tile([10.0, 10.0, 1.0], (B, 1)), dyn=rng.uniform(0.8, 1.2, (B, 1)))
+    return prob, W.solver_params(10, 1e-8, 1e-10)
+
+
+def custom_solver(dyn, prob, sp, device, mode="fused"):
+    """SQPSolver for a CustomDynamics on a workload-style problem dict."""
+    import torch
+
+    from diffmpc_amd import OptimalControlProblem, SQPSolver
+
+    t = lambda a: torch.as_tensor(np.asarray(a), dtype=torch.float64, device=device)  # noqa: E731
+    params = dict(
+        horizon=int(prob["horizon"]), discretization_resolution=float(prob["dt"]),
+        discretization_scheme=int(prob["scheme"]), penalize_control_reference=bool(prob["penalize_control_reference"]),
+        initial_state=t(prob["x0"]), final_state=t(prob["x_final"]),
+        reference_state_trajectory=t(prob["x_ref"]), reference_control_trajectory=t(prob["u_ref"]),
+        weights_penalization_reference_state_trajectory=t(prob["w_x"]),
+        weights_penalization_control_squared=t(prob["w_u"]),
+        weights_penalization_final_state=t(prob["w_f"]),
+    )
+    for k, (o, n) in dyn.parameter_offsets().items():
+        params[k] = t(prob["dyn"][:, o:o + n])
+    sp = dict(sp)
+    sp["pcg"] = dict(sp["pcg"])
+    sp["mode"] = mode
+    solver = SQPSolver(OptimalControlProblem(dyn, params), params=sp)
+    return solver, params, solver.initial_guess(params, device=device)
