@@ -48,12 +48,13 @@ def main():
                     help="time only the fused path (for profiler runs); prints the fused figures")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device-to-device once (reference kernel for HBM counters)")
-    ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain", "rollout", "dynparams"],
+    ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain", "rollout", "dynparams", "imitation"],
                     default="quadrotor",
                     help="quadrotor: the headline line (default); ragged: mixed batch of 8192 with horizons "
                          "U{10..200} in one graph; chain: 48-state chain B=512 N=256; rollout: the reference's "
                          "RL benchmark (LTI 8/4, N=39, 50 closed-loop steps, B=--batch or 64); dynparams: the "
-                         "headline workload with gradients w.r.t. mass and inertia as well")
+                         "headline workload with gradients w.r.t. mass and inertia as well; imitation: the "
+                         "reference's cart-pole imitation-learning step (32 demonstrations, N=100)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -85,6 +86,8 @@ def main():
         return bench_rollout(args)
     if args.workload == "dynparams":
         return bench_dynparams(args)
+    if args.workload == "imitation":
+        return bench_imitation(args)
 
     dev = torch.device("cuda:0")
     name, prob, sp = W.quadrotor(batch=args.batch, seed=1, horizon=args.horizon)
@@ -456,6 +459,80 @@ def bench_dynparams(args):
         "without_dynamics_params_ms": res[False][0] * 1e3,
         "dyn_param_grad_kernel_ms": dg_avg, "backward_kernel_ms": bw_avg,
     }), flush=True)
+
+
+# train_imitation_diffmpc.py's recorded time: 7.04 s per 200 training steps
+# (mean of the 20 runs in train_il_diffmpc.pkl; device not recorded)
+IL_REFERENCE_MS = 35.2
+
+
+def bench_imitation(args):
+    """The reference's imitation-learning benchmark
+    (benchmarking/imitation-learning/train_imitation_diffmpc.py): cart-pole
+    N=100, 32 expert demonstrations (collected first, untimed: 501 SQP
+    iterations, PCG 1e-24, line search), one training step = Euler rollout of
+    the demonstrations as the guess, batched solve (<= 5 SQP iterations,
+    tolerance 0.1, PCG 1e-14, line search), imitation loss, backward, weight
+    update, plus the model loss -- captured as one hipGraph."""
+    import numpy as np
+    import torch
+
+    from diffmpc_amd import imitation as il
+
+    dev = torch.device("cuda:0")
+    B = args.batch if args.batch != 1024 else 32
+    t0 = time.perf_counter()
+    x0, us, expert = il.collect_expert_data(num_trajs=B, seed=123, device=dev)
+    torch.cuda.synchronize()
+    t_expert = time.perf_counter() - t0
+    problem = il.cartpole_problem()
+    solver = il.SQPSolver(program=problem, params=il.training_solver_params())
+    tr = il.ImitationLearning(solver, x0, us, device=dev)
+    step = tr.graphed_step()
+    w = torch.tensor(il.INITIAL_WEIGHTS[0], dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        w, _ = step(w)
+    w = torch.tensor(il.INITIAL_WEIGHTS[0], dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m = tr.model_loss(w)
+        w, loss = step(w)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    out = {
+        "metric": "imitation-learning training step, cart-pole N=100, 32 demonstrations (ms/step)",
+        "value": dt * 1e3, "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": (IL_REFERENCE_MS / (dt * 1e3)) if B == 32 else None,
+        "dtype": "f32 dynamics/linearization, f64 Schur/PCG", "data": "synthetic (expert demonstrations solved here "
+                                                                     "from the reference's seeded initial states)",
+        "config": {"workload": f"cart-pole imitation learning N=100 B={B}, <=5 SQP iterations, tol 0.1, PCG 1e-14, "
+                               "line search; one hipGraph per step", "global_batch": B},
+        "imitation_loss": float(loss), "model_loss": float(m), "weights": w.tolist(),
+        "expert_collection_s": t_expert, "expert_max_convergence_error": float(expert.convergence_error.max()),
+        "reference_ms_per_step": IL_REFERENCE_MS,
+    }
+    if not args.no_cpu_baseline:
+        from threadpoolctl import threadpool_limits
+
+        from diffmpc_amd import workloads as W
+        from oracle import imitation as oil
+
+        m = min(4, B)
+        x0n, usn = x0[:m].cpu().numpy(), us[:m].cpu().numpy()
+        prob = oil.cartpole_problem(x0n, problem.params)
+        sp = W.solver_params(5, 1e-1, 1e-14, linesearch=True)
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            oil.il_loss_and_grad(prob, sp, usn, il.INITIAL_WEIGHTS[0])
+            t1 = time.perf_counter()
+        out["cpu_baseline"] = {
+            "value": (t1 - t0) * 1e3 * B / m, "unit": "ms/step", "cores": 1, "kind": "port",
+            "sample": f"one training step's loss and gradient for {m} of the {B} demonstrations, float64 numpy "
+                      "oracle, one thread, scaled to the batch",
+        }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

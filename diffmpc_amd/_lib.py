@@ -96,6 +96,7 @@ _SIGNATURES = {
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Saved), ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.POINTER(Grads), ctypes.c_int32, ctypes.c_void_p]),
     "dmpc_linearize": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(ProblemData)] + [ctypes.c_void_p] * 7),
+    "dmpc_simulate": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(ProblemData)] + [ctypes.c_void_p] * 3),
     "dmpc_dynamics_param_grads": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(ProblemData)]
                                   + [ctypes.c_void_p] * 8),
     "dmpc_pcg_assemble": (ctypes.c_int, [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 8),

@@ -17,7 +17,7 @@ if [ -n "${DMPC_CUSTOM_MODEL_FILE:-}" ]; then
   flags+=("-DDMPC_CUSTOM_MODEL_FILE=\"${DMPC_CUSTOM_MODEL_FILE}\"")
 fi
 pids=()
-for src in dmpc_fused dmpc_stream dmpc_plain dmpc_dyngrad dmpc_api; do
+for src in dmpc_fused dmpc_stream dmpc_plain dmpc_dyngrad dmpc_simulate dmpc_api; do
   "${HIPCC}" "${flags[@]}" -c "${here}/${src}.hip" -o "${obj}/${src}.o" &
   pids+=($!)
 done

@@ -23,6 +23,8 @@ int linearize_any(const dmpc_problem* p, const dmpc_problem_data* d, const doubl
 int dyn_param_grads_any(const dmpc_problem* p, const dmpc_problem_data* d, const double* x, const double* u,
                         const double* lam, const double* xt, const double* ut, const double* lamt, double* g,
                         hipStream_t st);
+int simulate_any(const dmpc_problem* p, const dmpc_problem_data* d, const double* controls, double* states,
+                 hipStream_t st);
 }  // namespace dmpc
 
 using namespace dmpc;
@@ -337,6 +339,14 @@ int dmpc_dynamics_param_grads(const dmpc_problem* prob, const dmpc_problem_data*
     return DMPC_ERR_ARG;
   return dyn_param_grads_any(prob, data, states, controls, dual, adj_states, adj_controls, adj_dual, grads,
                              reinterpret_cast<hipStream_t>(stream));
+}
+
+int dmpc_simulate(const dmpc_problem* prob, const dmpc_problem_data* data, const double* controls, double* states,
+                  void* stream) {
+  int rc = check_problem(prob);
+  if (rc) return rc;
+  if (!data || !data->x0 || !data->dyn_params || !controls || !states) return DMPC_ERR_ARG;
+  return simulate_any(prob, data, controls, states, reinterpret_cast<hipStream_t>(stream));
 }
 
 int dmpc_linearize(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,

@@ -158,7 +158,9 @@ def cpp_model_source(state_dot: Callable, nx: int, nu: int, n_params: int, name:
         return nm
 
     results = [ref(o) for o in out]
-    body = [f"    const T P{j} = sc<T>(p[{j}]);" for j in range(n_params)] + lines
+    uses_constants = any("sc<T>(S(" in ln for ln in lines + results)
+    body = ["    using S = scalar_t<T>;"] if uses_constants else []
+    body += [f"    const T P{j} = sc<T>(p[{j}]);" for j in range(n_params)] + lines
     body += [f"    xd[{i}] = {r};" for i, r in enumerate(results)]
     return "\n".join([
         f"// generated by diffmpc_amd.dynamics.custom_dynamics for the model '{name}'",
@@ -166,12 +168,14 @@ def cpp_model_source(state_dot: Callable, nx: int, nu: int, n_params: int, name:
         f"  static constexpr int NX = {nx}, NU = {nu}, NPARAM = {n_params};",
         "  template <class T, class PA>",
         "  __device__ static void f(const T* x, const T* u, PA p, T* xd) {",
-        "    using S = scalar_t<T>;",
         *body,
         "  }",
         "};",
         "",
     ])
+
+
+_STAMP = "library_sources.sha256"
 
 
 def _sources_digest() -> str:
@@ -193,7 +197,19 @@ def build_custom_library(source: str, out_dir: str) -> str:
     lib = os.path.join(out_dir, "libdmpc_custom.so")
     env = dict(os.environ, DMPC_CUSTOM_MODEL_FILE=os.path.abspath(header), DMPC_LIB_OUT=os.path.abspath(lib))
     subprocess.run(["bash", os.path.join(_CSRC, "build.sh")], env=env, check=True, stdout=subprocess.DEVNULL)
+    with open(os.path.join(out_dir, _STAMP), "w") as f:
+        f.write(_sources_digest())
     return lib
+
+
+def _is_fresh(out_dir: str, source: str) -> bool:
+    """The library in out_dir was built from `source` and the current library
+    sources."""
+    paths = [os.path.join(out_dir, n) for n in ("libdmpc_custom.so", "custom_model.h", _STAMP)]
+    if not all(os.path.exists(p) for p in paths):
+        return False
+    with open(paths[1]) as f, open(paths[2]) as g:
+        return f.read() == source and g.read() == _sources_digest()
 
 
 # ---------------------------------------------------------------------------
@@ -250,13 +266,11 @@ class CustomDynamics(Dynamics):
 
     def library_path(self, build: bool = True) -> str:
         """Path of the compiled library for this model (compiled on first use
-        when `build`)."""
+        when `build`, and again when the library sources have changed since)."""
         if self._lib_path is None:
             d = self.build_dir()
             lib = os.path.join(d, "libdmpc_custom.so")
-            stamp = os.path.join(d, "custom_model.h")
-            fresh = os.path.exists(lib) and os.path.exists(stamp) and open(stamp).read() == self.source
-            if not fresh:
+            if not _is_fresh(d, self.source):
                 if not build:
                     raise FileNotFoundError(f"{lib} is not built for this model")
                 build_custom_library(self.source, d)

@@ -168,6 +168,46 @@ class OptimalControlProblem:
         p.dt = float(torch.as_tensor(self.params["discretization_resolution"]).reshape(-1)[0])
         return p
 
+    def simulate(self, controls, params: Dict[str, Any] = None, scheme: int = None, out=None) -> torch.Tensor:
+        """Open-loop rollout of controls (B, N+1, nu) from params["initial_state"]
+        on the GPU (dmpc_simulate): x_0 = x0, x_{t+1} = f_d(x_t, u_t).
+
+        scheme: the integrator (0 Euler, 1 midpoint, 2 RK4; default the
+        problem's own).  The imitation-learning guess of
+        benchmarking/imitation-learning/utils.py:20-31 is scheme 0.
+        Ragged batches (params["horizons"]) stop each instance at its own
+        horizon and zero the knots past it.  Returns (B, N+1, nx) float64
+        (written into `out` when given); no gradient flows through it.
+        """
+        if params is None:
+            params = self.params
+        lib = _lib.load_for(self.dynamics)
+        us = controls.to(torch.float64).contiguous()
+        B, dev = us.shape[0], us.device
+        N, nx, nu = self.horizon, self.num_state_variables, self.num_control_variables
+        if us.shape[1:] != (N + 1, nu):
+            raise ValueError(f"controls must be (B, {N + 1}, {nu}), got {tuple(us.shape)}")
+        x0 = batched(params["initial_state"], B, 1, torch.float64, dev).contiguous()
+        dyn = self.dynamics.flat_params(params, B, dev).contiguous()
+        if out is None:
+            out = torch.empty(B, N + 1, nx, dtype=torch.float64, device=dev)
+        elif out.shape != (B, N + 1, nx) or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 (B, N+1, nx) tensor")
+        h = params.get("horizons")
+        if h is not None:
+            h = torch.as_tensor(h, device=dev).to(torch.int32).reshape(B).contiguous()
+            if not torch.cuda.is_current_stream_capturing():  # (the kernel clamps them to [1, N])
+                if int(h.min()) < 1 or int(h.max()) > N:
+                    raise ValueError(f"horizons must lie in [1, {N}]")
+        data = _lib.ProblemData(_lib.ptr(x0), None, None, None, None, None, None, _lib.ptr(dyn),
+                                _lib.ptr(h) if h is not None else None)
+        prob = self.c_problem(B)
+        if scheme is not None:
+            prob.scheme = int(scheme)
+        rc = lib.dmpc_simulate(prob, data, _lib.ptr(us), _lib.ptr(out), _lib.stream_ptr(dev))
+        _lib.check(rc, "dmpc_simulate")
+        return out
+
     def linearize(self, states, controls, params: Dict[str, Any] = None):
         """As_next, As, Bs, Cs at (states, controls) on the GPU (ocp.py:325-398).
 

@@ -18,6 +18,8 @@
  *   dmpc_pcg_recover    get_states_controls_from_kkt_mult...   diffmpc/solvers/pcg.py:290-348
  *   dmpc_dynamics_param_grads  _compute_mixed_derivatives + dL_dtheta for
  *                       dynamics parameters among the weights  diffmpc/solvers/sqp.py:605-660, 714-737
+ *   dmpc_simulate       rollout of demonstrated controls       benchmarking/imitation-learning/utils.py:20-31
+ *                       (and predict_next_state per step)      diffmpc/dynamics/integrators.py:20-71
  */
 #ifndef DMPC_H_
 #define DMPC_H_
@@ -171,6 +173,14 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
 int dmpc_dynamics_param_grads(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,
                               const double* controls, const double* dual, const double* adj_states,
                               const double* adj_controls, const double* adj_dual, double* grads, void* stream);
+
+/* Open-loop rollout of controls (B, N+1, nu) from data->x0 with the
+ * problem's integrator: states (B, N+1, nx), x_0 = x0, x_{t+1} = f_d(x_t, u_t)
+ * for t < N_b, float64 at the float32 dyn_params; knots past N_b are zero.
+ * data uses x0, dyn_params and horizons only.  The imitation-learning
+ * initial guess of utils.py:20-31 is this with scheme 0 (Euler). */
+int dmpc_simulate(const dmpc_problem* prob, const dmpc_problem_data* data, const double* controls, double* states,
+                  void* stream);
 
 /* f_d, A, B, C at (states, controls): fd (B,N,nx), A (B,N,nx,nx), Bm (B,N,nx,nu), C (B,N+1,nx). */
 int dmpc_linearize(const dmpc_problem* prob, const dmpc_problem_data* data, const double* states,

@@ -25,4 +25,9 @@ Parity pinning (what checks each function; see tests/test_oracle_*.py):
   sqp.sqp_backward      central finite differences of sqp_solve (FP64)
   sqp.linesearch        parity unpinned beyond the alpha=[1.0] no-op and a
                         hand-checked decrease rule on a 1-D case
+  rollout.*             the notebooks' printed losses and updated weights
+  sqp.dynamics_param_gradient  central FD of full solves (LTI, chain) and of the
+                        converged QP's sensitivities (nonlinear models)
+  imitation.*           explicit Euler steps; central FD of the loss of
+                        converged solves
 """
