@@ -64,12 +64,16 @@ class Rollout:
     """Batched closed-loop rollout of `num_sim_steps` MPC solves.
 
     solver, problem_params: the MPC problem (batched or shared parameters);
-    simulate(x, u) -> x+; reward(x+, u) -> (B,) reward of one step.
+    simulate(x, u) -> x+; reward(x+, u) -> (B,) reward of one step;
+    initial_guess(state) -> SQPSolution: the first solve's guess (default the
+    solver's straight line; the quadrotor notebook, for instance, normalises
+    the quaternions of that line and starts from hover thrust, cell 8).
     Calling returns (total reward (B,), states (T+1, B, nx), controls (T, B, nu)).
     """
 
     def __init__(self, solver, problem_params: Dict, simulate: Callable, reward: Callable, num_sim_steps: int,
-                 warm_start: bool = True, adjoint_warm_start: bool = True):
+                 warm_start: bool = True, adjoint_warm_start: bool = True,
+                 initial_guess: Optional[Callable] = None):
         self.solver = solver
         self.problem_params = problem_params
         self.simulate = simulate
@@ -77,8 +81,11 @@ class Rollout:
         self.num_sim_steps = num_sim_steps
         self.warm_start = warm_start
         self.adjoint_warm_start = adjoint_warm_start
+        self.initial_guess = initial_guess
 
     def _guess(self, state) -> SQPSolution:
+        if self.initial_guess is not None:
+            return self.initial_guess(state)
         return self.solver.initial_guess({**self.problem_params, "initial_state": state}, device=state.device)
 
     def __call__(self, x_init: torch.Tensor, weights: Optional[Dict] = None):

@@ -23,8 +23,10 @@ Parity pinning (what checks each function; see tests/test_oracle_*.py):
   qp.recover            stationarity H z + q + G^T lam = 0
   sqp.sqp_solve         LTI closed form (one iteration == exact QP), notebook value
   sqp.sqp_backward      central finite differences of sqp_solve (FP64)
-  sqp.linesearch        parity unpinned beyond the alpha=[1.0] no-op and a
-                        hand-checked decrease rule on a 1-D case
+  sqp.linesearch        the alpha=[1.0] no-op, a hand-checked decrease rule on
+                        a 1-D case, and the quadrotor notebook's recorded loss
+                        (1.5e-4: that run's solves stop unconverged at their
+                        iteration cap, so it pins no tighter)
   rollout.*             the notebooks' printed losses and updated weights
   sqp.dynamics_param_gradient  central FD of full solves (LTI, chain) and of the
                         converged QP's sensitivities (nonlinear models)

@@ -39,7 +39,9 @@ def main(ref):
         },
         "quadrotor": {
             "losses": [float(v) for v in re.findall(r"reward: (-?[0-9.]+)", qd)],
+            "w_x_after_step1": floats_after(qd, "weights_penalization_reference_state_trajectory"),
             "w_u_after_step1": floats_after(qd, "weights_penalization_control_squared"),
+            "w_f_after_step1": floats_after(qd, "weights_penalization_final_state"),
         },
     }
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "notebook_golden.json")

@@ -28,6 +28,53 @@ def notebook_spacecraft():
     return inertia, x_init, prob, params
 
 
+def notebook_quadrotor():
+    """notebooks/RL_quadrotor.ipynb cells 5, 7, 8, 12: B=1, horizon 20 (N=19),
+    10 RK4-simulated steps, quadrotor.yaml costs with mass 0.1 and inertia
+    [[.1,.01,.01],[.01,.1,.01],[.01,.01,.1]], <= 50 SQP iterations with the
+    default line search, PCG 1e-12.  Returns (x_init (1, 13), workload dict,
+    solver params, guess function, reward, reward gradient)."""
+    np.random.seed(0)  # generate_problem_data(batch_size=1, seed=0)
+    x = np.array([0., 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0])[None].repeat(1, 0)
+    x[:, :3] = -4.0 + np.random.rand(1, 3) * 8.0
+    x[:, 3:6] = -0.1 + np.random.rand(1, 3) * 0.2
+    delta = 0.5 * np.random.randn(4)
+    q = np.array([1.0, 0, 0, 0]) + delta / np.linalg.norm(delta)
+    x[:, 6:10] = q / np.linalg.norm(q)
+    N, nx, nu = 19, 13, 4
+    xr = np.array([0., 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0])
+    ur = np.array([0.981, 0, 0, 0])
+    rep = lambda v: np.asarray(v, np.float64)[None].copy()  # noqa: E731
+    inertia = [0.1, 0.01, 0.01, 0.01, 0.1, 0.01, 0.01, 0.01, 0.1]
+    prob = dict(horizon=N, dt=0.03, scheme=2, penalize_control_reference=True,
+                x0=x.copy(), x_final=rep(xr), x_ref=np.tile(xr, (1, N + 1, 1)), u_ref=np.tile(ur, (1, N + 1, 1)),
+                w_x=rep([1e-4, 1e-4, 1e-4, 1e-1, 1e-1, 1e-1, 1e-4, 1e-4, 1e-4, 1e-4, 1e-1, 1e-1, 1e-1]),
+                w_u=rep([1e1, 1e-1, 1e-1, 1e-1]),
+                w_f=rep([1e5, 1e5, 1e5, 1e3, 1e3, 1e3, 1e3, 1e3, 1e3, 1e3, 1e3, 1e3, 1e3]),
+                dyn=rep([0.1] + inertia))
+    params = W.solver_params(50, 1e-8, 1e-12, linesearch=True)
+
+    def guess(p):
+        """solver_initial_guess (cell 8): the straight line with normalised
+        quaternions, controls [1, 0, 0, 0], zero multipliers."""
+        t = np.arange(N + 1, dtype=np.float64)[None, :, None]
+        xs = p["x0"][:, None, :] * (N - t) / N + p["x_final"][:, None, :] * t / N + 1e-6
+        xs[:, :, 6:10] /= np.linalg.norm(xs[:, :, 6:10], axis=-1, keepdims=True)
+        us = np.zeros((xs.shape[0], N + 1, nu))
+        us[:, :, 0] = 1.0
+        return xs, us, np.zeros((xs.shape[0], N + 1, nx))
+
+    def reward(xn, u):
+        return -(np.sum((xn - xr)[:, :3] ** 2, 1) + 0.1 * np.sum((u - ur) ** 2, 1))
+
+    def reward_grad(xn, u):
+        gx = np.zeros_like(xn)
+        gx[:, :3] = -2.0 * (xn - xr)[:, :3]
+        return gx, -0.2 * (u - ur)
+
+    return x, prob, params, guess, reward, reward_grad
+
+
 def small_random_qp(T=3, nx=4, nu=5, seed=0, B=2):
     """Random QP of the reference's PCG warm-start test (tests/test_solver_pcg.py:179-227), batched."""
     rng = np.random.default_rng(seed)

@@ -17,7 +17,7 @@ from diffmpc_amd import workloads as W
 from oracle import dynamics as odyn
 from oracle import qp as oqp
 from oracle import sqp as osqp
-from tests.helpers import notebook_spacecraft, oracle_problem, small_random_qp
+from tests.helpers import notebook_quadrotor, notebook_spacecraft, oracle_problem, small_random_qp
 
 pytestmark = pytest.mark.gpu
 GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "notebook_golden.json")))
@@ -551,6 +551,50 @@ def test_graph_captured_ragged_step_matches_eager_on_every_replay(cuda):
     for (se, ge), (sg, gg) in zip(eager2, got):
         assert torch.equal(se, sg) and torch.equal(ge, gg)
     assert not all(torch.equal(a[0], b[0]) for a, b in zip(eager, eager2))
+
+
+def test_notebook_quadrotor_rollout_and_gradient(cuda):
+    """notebooks/RL_quadrotor.ipynb cells 7, 8, 12 on the GPU path through
+    diffmpc_amd.rollout.Rollout: ten RK4-simulated steps of line-searched
+    solves (all at the 50-iteration cap, unconverged), the summed reward and
+    one gradient step (lr = 1e-4) of the three weight vectors.  Bars as for
+    the oracle (see tests/test_oracle_sqp.py): loss 1.5e-4 relative,
+    control-weight updates within 2 % of the recorded ones."""
+    from diffmpc_amd.rollout import Rollout, euler_simulator
+
+    say("[notebook] quadrotor rollout")
+    x, prob, sp, guess_np, _, _ = notebook_quadrotor()
+    solver, params, _ = W.make_solver("quadrotor", prob, sp, device=cuda)
+    keys = ["weights_penalization_reference_state_trajectory", "weights_penalization_control_squared",
+            "weights_penalization_final_state"]
+    for k in keys:
+        params[k] = params[k][0]
+    xr = torch.tensor(prob["x_ref"][0, 0], device=cuda)
+    ur = torch.tensor(prob["u_ref"][0, 0], device=cuda)
+
+    def reward(xn, u):
+        return -(((xn - xr)[:, :3] ** 2).sum(1) + 0.1 * ((u - ur) ** 2).sum(1))
+
+    def guess(state):
+        g = solver.initial_guess({**params, "initial_state": state}, device=cuda)
+        xs, us, _ = guess_np({"x0": _np(state), "x_final": _np(params["final_state"])})
+        return g._replace(states=torch.tensor(xs, device=cuda), controls=torch.tensor(us, device=cuda))
+
+    sim = euler_simulator(solver.program.dynamics, params, float(prob["dt"]), scheme=2)
+    ro = Rollout(solver, params, sim, reward, 10, initial_guess=guess)
+    w = {k: params[k].clone().requires_grad_(True) for k in keys}
+    total, _, _ = ro(torch.tensor(x, device=cuda), w)
+    grads = torch.autograd.grad(total.sum(), [w[k] for k in keys])
+    torch.cuda.synchronize()
+    ref = GOLDEN["quadrotor"]
+    loss = float(total.detach().sum())
+    say(f"  loss {loss!r} reference {ref['losses'][0]!r}")
+    assert abs(loss - ref["losses"][0]) <= 1.5e-4 * abs(ref["losses"][0])
+    w0 = _np(params["weights_penalization_control_squared"])
+    step = np.maximum(1e-5, w0 + 1e-4 * np.nan_to_num(_np(grads[1]))) - w0
+    want = np.array(ref["w_u_after_step1"]) - w0
+    say(f"  w_u update {step} reference {want}")
+    assert np.all(np.abs(step - want) <= 0.02 * np.abs(want))
 
 
 def _rl_setup(cuda, batch, horizon, steps, adjoint_warm_start=True):
