@@ -68,8 +68,8 @@ def main():
         print(json.dumps({"copy_bytes": n * 4, "check": float(dst[-1])}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: instances are independent and this build drives a single GPU "
-              "per process; run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: instances are independent and one process drives one GPU; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

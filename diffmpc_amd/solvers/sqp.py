@@ -92,7 +92,7 @@ def _c_solver_params(params: Dict[str, Any]) -> "_lib.SolverParams":
     sp.pcg_max_iter = int(params["pcg"]["max_iter"])
     sp.pcg_tol_epsilon = float(params["pcg"]["tol_epsilon"])
     sp.warm_start_backward = int(bool(params["warm_start_backward"]))
-    # fused-mode tuning (keys of this build; 0 = library default)
+    # fused-mode tuning (keys with no reference counterpart; 0 = library default)
     sp.blocks_per_group = int(params.get("blocks_per_group", 0))
     sp.fused_variant = {"auto": 0, "register": 1, "streamed": 2}[params.get("fused_variant", "auto")]
     return sp
