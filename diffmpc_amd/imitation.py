@@ -165,6 +165,8 @@ class ImitationLearning:
         self.demo_controls = torch.as_tensor(demo_controls, dtype=torch.float64, device=self.device).contiguous()
         self.params = _device_params(self.problem.params, self.device)
         self.params["initial_state"] = self.x0
+        # the shared references and weights, expanded to the batch once
+        self.params = solver.batch_params(self.params, self.x0.shape[0], self.device)
         self.true_weights = torch.as_tensor(self.problem.params[weight_key], dtype=torch.float64,
                                             device=self.device)
         self._graph = None

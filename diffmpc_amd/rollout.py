@@ -90,6 +90,12 @@ class Rollout:
 
     def __call__(self, x_init: torch.Tensor, weights: Optional[Dict] = None):
         weights = {} if weights is None else weights
+        # expand shared parameters and weights to the batch once per rollout
+        # rather than once per solve
+        B, dev = x_init.shape[0], x_init.device
+        problem_params = self.solver.batch_params(
+            {k: v for k, v in self.problem_params.items() if k != "initial_state"}, B, dev)
+        weights = self.solver.batch_params(weights, B, dev)
         state = x_init
         first = self._guess(x_init.detach())
         guess = first
@@ -98,7 +104,7 @@ class Rollout:
         for _ in range(self.num_sim_steps):
             if not self.adjoint_warm_start:
                 guess = guess._replace(dual_backward_guess=torch.zeros_like(guess.dual))
-            params = {**self.problem_params, "initial_state": state.detach()}
+            params = {**problem_params, "initial_state": state.detach()}
             sol = self.solver.solve(guess, params, weights)
             u0 = sol.controls[:, 0]
             state = self.simulate(state, u0)

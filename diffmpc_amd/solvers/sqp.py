@@ -36,6 +36,10 @@ from .pcg import (PCGOptimalControl, QPCostParameters, QPDynamicsParameters, QPP
 DEFAULT_SOLVER_PARAMS = load_solver_params("sqp.yaml")
 
 # weights keys the backward differentiates (closed-form mixed derivatives)
+# problem_params key of the model's flat float32 parameter block, (B, n_params),
+# as batch_params precomputes it
+FLAT_DYNAMICS_KEY = "_flat_dynamics_parameters"
+
 WEIGHT_KEYS = {
     "weights_penalization_reference_state_trajectory": "w_x",
     "weights_penalization_control_squared": "w_u",
@@ -136,19 +140,30 @@ class _SQPSolveFn(torch.autograd.Function):
         data_t = [t.contiguous() for t in (x0, x_final, x_ref, u_ref, w_x, w_u, w_f)] + [dyn.contiguous()]
         data_t.append(horizons)
         data = _lib.ProblemData(*[_lib.ptr(t) for t in data_t])
-        x = guess["states"].to(torch.float64).clone().contiguous()
-        u = guess["controls"].to(torch.float64).clone().contiguous()
-        lam = guess["dual"].to(torch.float64).clone().contiguous()
-        status = torch.zeros(B, dtype=torch.int32, device=dev)
-        num_iter = torch.zeros(B, dtype=torch.int32, device=dev)
-        conv_err = guess["convergence_error"].to(torch.float64).clone().contiguous()
-        conv_errs = guess["convergence_errors"].to(torch.float64).clone().contiguous()
-        ls_alphas = guess["linesearch_alphas"].to(torch.float64).clone().contiguous()
+        # in/out arrays: fresh copies of the guess (the guess may be a previous
+        # solution that autograd still holds), made by one multi-tensor copy
+        dual_bwd_in = guess.get("dual_backward_guess")
+        src = [guess["states"], guess["controls"], guess["dual"], guess["convergence_error"],
+               guess["convergence_errors"], guess["linesearch_alphas"]]
+        if dual_bwd_in is not None:
+            src.append(dual_bwd_in)
+        src = [t.to(torch.float64) for t in src]
+        dst = [torch.empty(t.shape, dtype=torch.float64, device=dev) for t in src]
+        torch._foreach_copy_(dst, src)
+        x, u, lam, conv_err, conv_errs, ls_alphas = dst[:6]
+        dual_bwd_out = (dst[6] if dual_bwd_in is not None
+                        else torch.zeros(B, N + 1, nx, dtype=torch.float64, device=dev))
         pcg_iters = guess["num_pcg_iters"].to(torch.int32).clone().contiguous()
-        AB = torch.zeros(B, N, nx, nx + nu, dtype=torch.float32, device=dev)
-        fd = torch.zeros(B, N, nx, dtype=torch.float32, device=dev)
-        qinv = torch.zeros(B, N + 1, nx, dtype=torch.float64, device=dev)
-        rinv = torch.zeros(B, nu, dtype=torch.float64, device=dev)
+        # written by every solve (reset and finish kernels)
+        status = torch.empty(B, dtype=torch.int32, device=dev)
+        num_iter = torch.empty(B, dtype=torch.int32, device=dev)
+        # the saved linearization is written for every knot of a full-horizon
+        # batch; ragged batches leave the padding untouched, so it starts at zero
+        alloc = torch.empty if horizons is None else torch.zeros
+        AB = alloc(B, N, nx, nx + nu, dtype=torch.float32, device=dev)
+        fd = alloc(B, N, nx, dtype=torch.float32, device=dev)
+        qinv = alloc(B, N + 1, nx, dtype=torch.float64, device=dev)
+        rinv = alloc(B, nu, dtype=torch.float64, device=dev)
         sol = _lib.Solution(_lib.ptr(x), _lib.ptr(u), _lib.ptr(lam), _lib.ptr(status), _lib.ptr(num_iter),
                             _lib.ptr(conv_err), _lib.ptr(conv_errs), _lib.ptr(ls_alphas), _lib.ptr(pcg_iters))
         prob = prog.c_problem(B)
@@ -160,9 +175,6 @@ class _SQPSolveFn(torch.autograd.Function):
                            _lib.ptr(work) if nwork else None, nwork)
         rc = lib.dmpc_sqp_solve(prob, sp, data, sol, saved, mode, _lib.stream_ptr(dev))
         _lib.check(rc, "dmpc_sqp_solve")
-        dual_bwd_in = guess.get("dual_backward_guess")
-        dual_bwd_out = (dual_bwd_in.to(torch.float64).clone() if dual_bwd_in is not None
-                        else torch.zeros(B, N + 1, nx, dtype=torch.float64, device=dev)).contiguous()
         ctx.solver = solver
         ctx.prob_args = (prob, sp, mode)
         ctx.data_t = data_t
@@ -196,7 +208,7 @@ class _SQPSolveFn(torch.autograd.Function):
         g = {k: torch.empty(s, dtype=torch.float64, device=dev) for k, s in (
             ("w_x", (B, nx)), ("w_u", (B, nu)), ("w_f", (B, nx)), ("x0", (B, nx)), ("x_ref", (B, N + 1, nx)),
             ("u_ref", (B, N + 1, nu)), ("x_final", (B, nx)))}
-        iters = torch.zeros(B, dtype=torch.int32, device=dev)
+        iters = torch.empty(B, dtype=torch.int32, device=dev)  # written by the backward
         want_dyn = ctx.needs_input_grad[9]
         if want_dyn:  # the adjoint primal solution feeds the dynamics-parameter contraction
             adj_x = torch.empty(B, N + 1, nx, dtype=torch.float64, device=dev)
@@ -299,6 +311,33 @@ class SQPSolver:
             self.initial_guess_dual(batch, dev),
         )
 
+    # base (unbatched) ranks of the per-instance arrays the kernels read
+    _BATCH_NDIM = {
+        "initial_state": 1, "final_state": 1, "reference_state_trajectory": 2, "reference_control_trajectory": 2,
+        "weights_penalization_reference_state_trajectory": 1, "weights_penalization_control_squared": 1,
+        "weights_penalization_final_state": 1,
+    }
+
+    def batch_params(self, params: Dict[str, Any], batch: int, device=None) -> Dict[str, Any]:
+        """params with the per-instance arrays the kernels read expanded to
+        contiguous (batch, ...) device tensors (differentiably: gradients of a
+        shared value still sum over the batch) and, when params holds the
+        model's parameters, their flat float32 block under
+        FLAT_DYNAMICS_KEY.  Solving many times with the same parameters
+        (closed-loop rollouts, training loops), pass the result instead of the
+        shared values: each solve then reads them without first copying them
+        out to the batch.  A solve whose weights override a dynamics
+        parameter ignores the flat block and flattens the merged values."""
+        out = dict(params)
+        for k, nd in self._BATCH_NDIM.items():
+            if k in params and params[k] is not None:
+                out[k] = batched(params[k], batch, nd, torch.float64, device).contiguous()
+        dyn = self.program.dynamics
+        out.pop(FLAT_DYNAMICS_KEY, None)
+        if dyn.param_keys and all(k in params for k in dyn.param_keys):
+            out[FLAT_DYNAMICS_KEY] = dyn.flat_params(params, batch, device).contiguous()
+        return out
+
     def make_params_with_weights(self, weights, problem_params=None):
         new_params = copy.copy(self.problem_params if problem_params is None else problem_params)
         for key, value in weights.items():
@@ -378,8 +417,14 @@ class SQPSolver:
         wf = b("weights_penalization_final_state", 1)
         # dynamics parameters passed as weights keep their graph: the solve's
         # gradient with respect to the flat parameter block flows back to them
-        dyn = prog.dynamics.flat_params({k: (v.detach() if isinstance(v, torch.Tensor) and k not in weights else v)
-                                         for k, v in params.items()}, B, device)
+        flat = params.get(FLAT_DYNAMICS_KEY)
+        if (isinstance(flat, torch.Tensor) and not (dyn_keys & set(weights))
+                and flat.shape == (B, prog.dynamics.num_params) and flat.dtype == torch.float32
+                and flat.device == torch.device(device) and flat.is_contiguous()):
+            dyn = flat  # from batch_params (its graph, if any, is kept)
+        else:
+            dyn = prog.dynamics.flat_params({k: (v.detach() if isinstance(v, torch.Tensor) and k not in weights
+                                                 else v) for k, v in params.items()}, B, device)
         horizons = _ragged_horizons(params.get("horizons"), B, N, device)
         out = _SQPSolveFn.apply(self, g, x0, xf, xref, uref, wx, wu, wf, dyn, horizons)
         (x, u, lam, status, num_iter, conv_err, conv_errs, ls_alphas, pcg_iters, AB, fd, qinv, rinv,

@@ -123,3 +123,41 @@ def test_ragged_dynamics_parameter_gradients_match_oracle_per_instance(cuda, var
         e = _rel_rows(got[b:b + 1], want)
         say(f"  b={b} N={n}: rel {e:.2e}")
         assert e <= 1e-4
+
+
+def test_batch_params_solves_match_shared_parameter_solves(cuda):
+    """SQPSolver.batch_params (shared values expanded once, the model's flat
+    parameter block precomputed) gives the same solutions and gradients as
+    passing the shared values to every solve, with and without a dynamics
+    parameter among the weights."""
+    say("[batch-params] quadrotor")
+    name, prob, sp = CASES["quadrotor"]()
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    B = prob["x0"].shape[0]
+    key = "weights_penalization_reference_state_trajectory"
+    base = {k: v for k, v in params.items() if k != key}
+    shared_w = params[key][0]
+    bp = solver.batch_params({**base, key: shared_w}, B, cuda)
+    assert "_flat_dynamics_parameters" in bp
+
+    def run(p, with_mass, batched):
+        guess.dual_backward_guess.zero_()  # each backward writes its adjoint warm start back into the guess
+        w = shared_w.clone().requires_grad_(True)
+        m = params["mass"].clone().requires_grad_(True)
+        weights = {key: w}
+        if batched:
+            weights = solver.batch_params(weights, B, cuda)
+        if with_mass:
+            weights["mass"] = m
+        sol = solver.solve(guess, p, weights)
+        loss = (sol.states - params["reference_state_trajectory"]).square().sum()
+        leaves = [w, m] if with_mass else [w]
+        return sol.states.detach(), torch.autograd.grad(loss, leaves)
+
+    for with_mass in (False, True):
+        x1, g1 = run({**base, key: shared_w}, with_mass, False)
+        x2, g2 = run(bp, with_mass, True)
+        torch.cuda.synchronize()
+        assert torch.equal(x1, x2)
+        for a, b in zip(g1, g2):
+            assert torch.allclose(a, b, rtol=1e-12, atol=0.0)

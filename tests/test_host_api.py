@@ -111,3 +111,20 @@ def test_cost_and_constraints_match_oracle():
               w_f=p["weights_penalization_final_state"].numpy()[None], dyn=p["inertia_vector"].numpy()[None])
     assert np.isclose(c, osqp.cost(op, x[None], u[None])[0], rtol=1e-13)
     assert np.allclose(g, osqp.equality_constraints(op, x[None], u[None])[0], atol=1e-13)
+
+
+def test_batch_params_expands_once_and_keeps_gradients():
+    p = load_problem_params("quadrotor.yaml")
+    solver = SQPSolver(OptimalControlProblem(QuadrotorDynamics(), p), load_solver_params("sqp.yaml"))
+    w = p["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
+    out = solver.batch_params({**p, "weights_penalization_reference_state_trajectory": w}, 3)
+    assert out["reference_state_trajectory"].shape == (3, 26, 13) and out["reference_state_trajectory"].is_contiguous()
+    assert out["final_state"].shape == (3, 13) and out["initial_state"].shape == (3, 13)
+    flat = out["_flat_dynamics_parameters"]
+    assert flat.shape == (3, 10) and flat.dtype == torch.float32
+    assert torch.equal(flat, QuadrotorDynamics().flat_params(p, 3).contiguous())
+    out["weights_penalization_reference_state_trajectory"].sum().backward()
+    assert torch.equal(w.grad, torch.full_like(w, 3.0))  # the expand's gradient sums over the batch
+    # a weights dict without the model parameters gets no flat block
+    ow = solver.batch_params({"weights_penalization_control_squared": p["weights_penalization_control_squared"]}, 3)
+    assert "_flat_dynamics_parameters" not in ow and ow["weights_penalization_control_squared"].shape == (3, 4)
