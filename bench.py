@@ -23,8 +23,8 @@ METRIC = "SQP solves/s & fwd+bwd grads/s, quadrotor T=50 B=1024, fused vs plain 
 MI355X_FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per qp_iteration_kernel launch at the default workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scale factors
-# from a 256 MiB device copy): 99.4 MB read + 77.3 MB written
-QP_TRAFFIC_BYTES = 176.7e6
+# from a 256 MiB device copy): 102.8 MB read + 77.4 MB written
+QP_TRAFFIC_BYTES = 180.2e6
 QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
 
 
@@ -172,7 +172,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 dynamics/linearization, f64 Schur/PCG",
+        "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG",
         "data": "synthetic",
         "config": {"workload": f"quadrotor nx=13 nu=4 T={N} B={B}, 5 SQP iterations, PCG eps 1e-6",
                    "global_batch": B, "horizon": N, "parallelism": "single GPU"},
@@ -282,7 +282,7 @@ def bench_ragged(args):
         "metric": "ragged mixed batch fwd+bwd grads/s (8192 instances, N ~ U{10..200}, 3 SQP iterations)",
         "value": B / dt, "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 dynamics/linearization, f64 Schur/PCG", "data": "synthetic",
+        "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG", "data": "synthetic",
         "config": {"workload": "ragged LTI/spacecraft7/quadrotor mix, one hipGraph per step", "global_batch": B},
         "groups": info,
     }), flush=True)
@@ -332,7 +332,7 @@ def bench_chain(args):
         "metric": "chain fwd+bwd grads/s (nx=48 nu=12 N=256 B=512), fused vs plain GPU",
         "value": B / res["fused"], "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
         "ms_per_step": res["fused"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 dynamics/linearization, f64 Schur/PCG", "data": "synthetic",
+        "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG", "data": "synthetic",
         "config": {"workload": "12-mass chain nx=48 nu=12 N=256 B=512, 1 SQP iteration, PCG eps 1e-8",
                    "global_batch": B},
         "plain": {"grads_per_s": B / res["plain"], "ms_fwd_bwd": res["plain"] * 1e3},
@@ -397,7 +397,7 @@ def bench_rollout(args):
         "value": solves / res["fwd_bwd"], "unit": "grads/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": res["fwd_bwd"] * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": (RL_REFERENCE_MS["fwd_bwd"] / (res["fwd_bwd"] * 1e3)) if B == 64 else None,
-        "dtype": "f32 dynamics/linearization, f64 Schur/PCG", "data": "synthetic (the reference's seeded data)",
+        "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG", "data": "synthetic (the reference's seeded data)",
         "config": {"workload": f"RL rollout LTI 8/4 N=39, 50 steps, PCG 1e-12, B={B}", "global_batch": B},
         "fwd": {"ms": res["fwd"] * 1e3, "solves_per_s": solves / res["fwd"],
                 "vs_reference_jax_gpu": (RL_REFERENCE_MS["fwd"] / (res["fwd"] * 1e3)) if B == 64 else None},
@@ -452,7 +452,7 @@ def bench_dynparams(args):
         "metric": "quadrotor fwd+bwd grads/s incl. d/d(mass, inertia) (T=50 B=1024)",
         "value": B / t_dyn, "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": t_dyn * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 dynamics/linearization, f64 Schur/PCG, f64 hyper-dual parameter contraction",
+        "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG, f64 hyper-dual parameter contraction",
         "data": "synthetic",
         "config": {"workload": f"quadrotor nx=13 nu=4 T={args.horizon} B={B}, 5 SQP iterations, grads w.r.t. "
                                "w_x, w_u, x0, mass, inertia", "global_batch": B},
@@ -505,7 +505,7 @@ def bench_imitation(args):
         "value": dt * 1e3, "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": (IL_REFERENCE_MS / (dt * 1e3)) if B == 32 else None,
-        "dtype": "f32 dynamics/linearization, f64 Schur/PCG", "data": "synthetic (expert demonstrations solved here "
+        "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG", "data": "synthetic (expert demonstrations solved here "
                                                                      "from the reference's seeded initial states)",
         "config": {"workload": f"cart-pole imitation learning N=100 B={B}, <=5 SQP iterations, tol 0.1, PCG 1e-14, "
                                "line search; one hipGraph per step", "global_batch": B},

@@ -55,9 +55,9 @@ struct LdsPlan {
     const size_t NB = N + 1, NBP = N + 3;
     return NB * NX /*xb*/ + NB * NU /*ub*/ + NBP * NX /*lamS*/ + NB * NX /*qinv*/ + NB * NX /*qv*/ +
            NB * NU /*rv*/ + NU /*rinv*/ + NX /*x0*/ + 3 * 16 /*red*/ + 4 * NBP * NX /*RT Y YQ YX*/ +
-           NB * NX * NX /*LF*/ + NB * NX /*ID*/;
+           NB * NX * NX /*LF*/ + NB * NX /*ID*/ + (size_t)N * NX /*fd*/;
   }
-  static __host__ __device__ size_t floats(int N, int P) { return (size_t)N * NX + P; }
+  static __host__ __device__ size_t floats(int N, int P) { return (size_t)P; }
   // doubles(N) plus `ls` doubles of line-search scratch
   static __host__ size_t bytes(int N, int P, int ls = 0) {
     return (doubles(N) + ls) * 8 + floats(N, P) * 4 + 16;
@@ -68,8 +68,9 @@ struct Lds {
   double *xb, *ub, *lamS, *qinv, *qv, *rv, *rinv, *x0, *redA, *redB, *redC;
   double *RT, *Y, *YQ, *YX;
   double *LF, *ID;  // Cholesky factor rows of theta_t and 1 / L_kk
+  double* fd;       // f_d(x_t, u_t), t < N
   double* LS;       // line-search scratch (see linesearch_doubles)
-  float *fd, *prm;
+  float* prm;
 };
 
 template <int NX, int NU>
@@ -94,10 +95,9 @@ __device__ Lds carve(void* base, int N, int ls = 0) {
   L.YX = d; d += NBP * NX;
   L.LF = d; d += NB * NX * NX;
   L.ID = d; d += NB * NX;
+  L.fd = d; d += N * NX;
   L.LS = d; d += ls;
-  float* f = reinterpret_cast<float*>(d);
-  L.fd = f; f += N * NX;
-  L.prm = f;
+  L.prm = reinterpret_cast<float*>(d);
   return L;
 }
 
@@ -449,7 +449,7 @@ __device__ __forceinline__ void gamma_rows(const Regs<NX, NU, K>& R, const Lds& 
       } else {
         const int tp = t - 1;
         g = -R.qi[s] * L.qv[t * NX + lane];
-        if (WITH_C) g -= (double)L.fd[tp * NX + lane];
+        if (WITH_C) g -= L.fd[tp * NX + lane];
 #pragma unroll
         for (int k = 0; k < NX; ++k) {
           double v = L.qinv[tp * NX + k] * L.qv[tp * NX + k];
@@ -574,14 +574,36 @@ __global__ void sqp_reset_kernel(int B, int32_t* num_iter) {
   if (b < B) num_iter[b] = 0;
 }
 
-// f_d(x_t, u_t) and column j of [A_t B_t], one thread per (b, t, j)
-// (optimal_control_problem.py:360-383 with value_and_jacfwd).
+// Column j of [A_t B_t], one thread per (b, t, j), then f_d(x_t, u_t), one
+// thread per (b, t) after them (optimal_control_problem.py:360-383 with
+// value_and_jacfwd).  The Jacobian is taken in float32 dual numbers; the
+// value f_d, which fixes the constraint residual C_{t+1} = -f_d + A x_t + B u_t
+// and so the point the SQP iteration converges to, is evaluated in float64 at
+// the float64 iterate (a float32 value leaves the iteration hovering ~1e-8
+// away from it).  The value threads fill waves of their own.
+template <class Model>
+__host__ __device__ constexpr long long linearize_threads(int B, int N) {
+  return (long long)B * N * (Model::NX + Model::NU + 1);
+}
+
 template <class Model>
 __global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   const int N = a.N;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)a.B * N * NZ) return;
+  const long long njac = (long long)a.B * N * NZ;
+  if (idx >= njac) {
+    const long long v = idx - njac;
+    if (v >= (long long)a.B * N) return;
+    const int t = v % N;
+    const int b = v / N;
+    if (!sqp_active(a, b) || t >= horizon_of(a.horizons, N, b)) return;
+    float prm[Model::NPARAM];
+#pragma unroll
+    for (int k = 0; k < Model::NPARAM; ++k) prm[k] = a.dyn[(size_t)b * a.P + k];
+    fd_value<Model>(a.x, a.u, prm, a.dt64, a.scheme, N, b, t, a.fd + ((size_t)b * N + t) * NX);
+    return;
+  }
   const int j = idx % NZ;
   const int t = (idx / NZ) % N;
   const int b = idx / ((long long)NZ * N);
@@ -598,10 +620,6 @@ __global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
   float* At = a.AB + ((size_t)b * N + t) * NX * NZ;
 #pragma unroll
   for (int i = 0; i < NX; ++i) At[i * NZ + j] = col[i];
-  if (j == 0) {
-#pragma unroll
-    for (int i = 0; i < NX; ++i) a.fd[((size_t)b * N + t) * NX + i] = fv[i];
-  }
 }
 
 // One SQP iteration of one instance per workgroup: Schur/preconditioner
@@ -873,7 +891,6 @@ Launch plan(int N, int P, int ls = 0, int forced_K = 0) {
 
 template <class Model, int G, int K, int MAXT>
 int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
-  constexpr int NZ = Model::NX + Model::NU;
   static bool attr = false;
   if (!attr) {
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&qp_iteration_kernel<Model, G, K, MAXT>),
@@ -882,7 +899,7 @@ int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
     attr = true;
   }
   hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter);
-  const long long nlin = (long long)a.B * a.N * NZ;
+  const long long nlin = linearize_threads<Model>(a.B, a.N);
   for (int it = 0; it < a.maxit; ++it) {
     prof_begin(PROF_LINEARIZE, st);
     hipLaunchKernelGGL(linearize_kernel<Model>, dim3((unsigned)((nlin + 255) / 256)), dim3(256), 0, st, a);
@@ -992,8 +1009,7 @@ int stream_launch_reset(const SolveArgs& a, hipStream_t st) {
 
 template <class Model>
 int launch_linearize(const SolveArgs& a, hipStream_t st) {
-  constexpr int NZ = Model::NX + Model::NU;
-  const long long nlin = (long long)a.B * a.N * NZ;
+  const long long nlin = linearize_threads<Model>(a.B, a.N);
   prof_begin(PROF_LINEARIZE, st);
   hipLaunchKernelGGL(linearize_kernel<Model>, dim3((unsigned)((nlin + 255) / 256)), dim3(256), 0, st, a);
   prof_end(PROF_LINEARIZE, st);

@@ -38,7 +38,7 @@ struct SolveArgs {
   int32_t* pcg_iters;
   // last linearization point, kept for the backward
   float* AB;      // (B, N, nx, nx+nu): [A_t | B_t]
-  float* fd;      // (B, N, nx): f_d(x_t, u_t)
+  double* fd;     // (B, N, nx): f_d(x_t, u_t) in float64
   double* qinv;   // (B, N+1, nx) diagonal of Qinv
   double* rinv;   // (B, nu) diagonal of Rinv (the same for every t)
   // workspace of the streamed variant: work_stride bytes per instance

@@ -348,6 +348,22 @@ __device__ void step(const T* x, const T* u, PA p, scalar_t<T> dt, int scheme, T
   for (int i = 0; i < NX; ++i) out[i] = x[i] + (acc[i] + k[i]) * (dt / S(6));
 }
 
+// f_d(x_t, u_t) of instance b in float64 at the float64 iterate (trajectories
+// of N + 1 knots per instance), written to out[0..NX).
+template <class Model>
+__device__ void fd_value(const double* x, const double* u, const float* p, double dt, int scheme, int N, int b, int t,
+                         double* out) {
+  constexpr int NX = Model::NX, NU = Model::NU;
+  double xs[NX], us[NU], f[NX];
+#pragma unroll
+  for (int i = 0; i < NX; ++i) xs[i] = x[((size_t)b * (N + 1) + t) * NX + i];
+#pragma unroll
+  for (int k = 0; k < NU; ++k) us[k] = u[((size_t)b * (N + 1) + t) * NU + k];
+  step<Model, double>(xs, us, p, dt, scheme, f);
+#pragma unroll
+  for (int i = 0; i < NX; ++i) out[i] = f[i];
+}
+
 // Value and column j of the Jacobian d f_d / d[x; u] at (x, u) in float.
 // Mirrors value_and_jacfwd over next_state (optimal_control_problem.py:360-383)
 // with one forward-mode tangent e_j.

@@ -27,7 +27,7 @@ inline int nblk(long long n, int t = 256) { return (int)((n + t - 1) / t); }
 // ---------------------------------------------------------------------------
 // linearization: one thread per (b, t, j), column j of [A_t B_t] at (x_t, u_t)
 template <class Model>
-__global__ void lin_kernel(int B, int N, int P, float dt, int scheme, const float* dyn, const double* x,
+__global__ void lin_kernel(int B, int N, int P, float dt, double dt64, int scheme, const float* dyn, const double* x,
                            const double* u, double* fd, double* As, double* Bs) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -47,8 +47,8 @@ __global__ void lin_kernel(int B, int N, int P, float dt, int scheme, const floa
   } else {
     for (int i = 0; i < NX; ++i) Bs[(bt * NX + i) * NU + (j - NX)] = col[i];
   }
-  if (j == 0)
-    for (int i = 0; i < NX; ++i) fd[bt * NX + i] = fv[i];
+  // the value in float64 (the Jacobian in float32), as the fused linearization
+  if (j == 0) fd_value<Model>(x, u, prm, dt64, scheme, N, b, t, fd + bt * NX);
 }
 
 // C_0 = x0; C_{t+1} = -f_d + A_t x_t + B_t u_t; As_next = -I
@@ -622,7 +622,7 @@ __global__ void finish_kernel(int B, int N, int nx, int nu, double tol, double* 
 
 // saved linearization for the backward: AB (float), diag Qinv, diag Rinv
 __global__ void save_kernel(int B, int N, int nx, int nu, const double* As, const double* Bs, const double* Qinv,
-                            const double* Rinv, const double* fd, const int* active, float* AB, float* fdo,
+                            const double* Rinv, const double* fd, const int* active, float* AB, double* fdo,
                             double* qinv, double* rinv) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int nz = nx + nu;
@@ -638,7 +638,7 @@ __global__ void save_kernel(int B, int N, int nx, int nu, const double* As, cons
     if (active[blk / (N + 1)]) qinv[idx] = Qinv[blk * nx * nx + i * nx + i];
   }
   if (idx < (long long)B * N * nx) {
-    if (active[idx / ((long long)N * nx)]) fdo[idx] = (float)fd[idx];
+    if (active[idx / ((long long)N * nx)]) fdo[idx] = fd[idx];
   }
   if (idx < (long long)B * nu) {
     const int b = idx / nu, i = idx % nu;
@@ -906,7 +906,7 @@ int plain_forward_t(const SolveArgs& a, hipStream_t st) {
     if (!host_any || sqp_it >= a.maxit) break;
     // linearize (optimal_control_problem.py:325-398)
     hipLaunchKernelGGL(lin_kernel<Model>, dim3(nblk((long long)B * N * (nx + nu))), dim3(256), 0, st, B, N, a.P, a.dt,
-                       a.scheme, a.dyn, a.x, a.u, q.fd, q.As, q.Bs);
+                       a.dt64, a.scheme, a.dyn, a.x, a.u, q.fd, q.As, q.Bs);
     hipLaunchKernelGGL(c_kernel, dim3(nblk(nst)), dim3(256), 0, st, B, N, nx, nu, a.x0, a.x, a.u, q.fd, q.As, q.Bs,
                        q.Cs, q.As_next);
     rc = assemble(q, B, N, nx, nu, st);
@@ -1064,7 +1064,7 @@ int linearize_t(const dmpc_problem* p, const dmpc_problem_data* d, const double*
   constexpr int nx = Model::NX, nu = Model::NU;
   const int B = p->batch, N = p->horizon;
   hipLaunchKernelGGL(lin_kernel<Model>, dim3(nblk((long long)B * N * (nx + nu))), dim3(256), 0, st, B, N, p->n_params,
-                     (float)p->dt, p->scheme, d->dyn_params, x, u, fd, A, Bm);
+                     (float)p->dt, p->dt, p->scheme, d->dyn_params, x, u, fd, A, Bm);
   void* base = nullptr;
   int rc = get_scratch((size_t)B * N * nx * nx * 8 + 256, &base);
   if (rc) return rc;

@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(512) stream_qp_kernel(SolveArgs a) {
   const double* xb = a.x + (size_t)b * NBs * NX;
   const double* ub = a.u + (size_t)b * NBs * NU;
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
-  const float* fdb = a.fd + (size_t)b * Np * NX;
+  const double* fdb = a.fd + (size_t)b * Np * NX;
   const double* x0 = a.x0 + (size_t)b * NX;
   stream_init_vectors<NX>(w, Np, N, a.lam + (size_t)b * NBs * NX);
   // gamma = C + G H^-1 q (pcg.py:245-288), C = [x0; -f_d + A xbar + B ubar]
@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(512) stream_qp_kernel(SolveArgs a) {
       g = c.qinv(0, lane) * c.q(0, lane) + x0[lane];
     } else {
       const int tp = t - 1;
-      g = -c.qinv(t, lane) * c.q(t, lane) - (double)fdb[tp * NX + lane];
+      g = -c.qinv(t, lane) * c.q(t, lane) - fdb[tp * NX + lane];
       const float* Ar = ABb + (size_t)tp * NX * NZ + lane * NZ;
       for (int k = 0; k < NX; ++k) g += (double)Ar[k] * (c.qinv(tp, k) * c.q(tp, k) + xb[tp * NX + k]);
       for (int k = 0; k < NU; ++k) g += (double)Ar[NX + k] * (c.rinv(k) * c.r(tp, k) + ub[tp * NU + k]);

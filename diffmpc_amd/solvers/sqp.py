@@ -60,7 +60,7 @@ class SavedLinearization(NamedTuple):
     """QP data of the last linearization point (sqp.py:487-492)."""
 
     AB: torch.Tensor  # (B, N, nx, nx+nu) float32: [A_t | B_t]
-    fd: torch.Tensor  # (B, N, nx) float32: f_d(x_t, u_t)
+    fd: torch.Tensor  # (B, N, nx) float64: f_d(x_t, u_t)
     qinv: torch.Tensor  # (B, N+1, nx) float64: diag Qinv_t
     rinv: torch.Tensor  # (B, nu) float64: diag Rinv
 
@@ -161,7 +161,7 @@ class _SQPSolveFn(torch.autograd.Function):
         # batch; ragged batches leave the padding untouched, so it starts at zero
         alloc = torch.empty if horizons is None else torch.zeros
         AB = alloc(B, N, nx, nx + nu, dtype=torch.float32, device=dev)
-        fd = alloc(B, N, nx, dtype=torch.float32, device=dev)
+        fd = alloc(B, N, nx, dtype=torch.float64, device=dev)
         qinv = alloc(B, N + 1, nx, dtype=torch.float64, device=dev)
         rinv = alloc(B, nu, dtype=torch.float64, device=dev)
         sol = _lib.Solution(_lib.ptr(x), _lib.ptr(u), _lib.ptr(lam), _lib.ptr(status), _lib.ptr(num_iter),

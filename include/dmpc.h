@@ -110,7 +110,7 @@ typedef struct dmpc_solution { /* SQPSolution fields, sqp.py:38-57 */
 
 typedef struct dmpc_saved { /* QP data of the last linearization point (sqp.py:487-492) */
   float* AB;    /* (B, N, nx, nx+nu): [A_t | B_t] */
-  float* fd;    /* (B, N, nx): f_d(x_t, u_t) */
+  double* fd;   /* (B, N, nx): f_d(x_t, u_t), evaluated in float64 */
   double* qinv; /* (B, N+1, nx): diagonal of Qinv_t */
   double* rinv; /* (B, nu): diagonal of Rinv_t (the same for every t) */
   /* Device workspace of dmpc_workspace_bytes() bytes (NULL when that is 0):

@@ -1,0 +1,191 @@
+"""The reference's own solver tests, run through this package's host API on
+the GPU, each also checked against the float64 oracle:
+
+* tests/test_optimal_control.py:30-41  spacecraft.yaml as loaded, PCG 1e-15
+* tests/test_solver_sqp.py:30-100       spacecraft, horizons 10/20, batch 1/2,
+                                         line search on/off, adjoint warm start on/off
+* tests/test_solver_sqp.py:103-121      warm vs cold adjoint start
+* tests/test_differentiable.py:21-92    gradient of ||x*|| + ||u*|| w.r.t. the
+                                         state and control weights vs central differences
+
+The problem set-ups follow the reference files (spacecraft.yaml, seeded
+numpy draws); trajectories within 1e-5 and gradients within 1e-4 of the
+oracle, besides the reference's own assertions."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from diffmpc_amd import OptimalControlProblem, SpacecraftDynamics, SQPSolver, load_problem_params, load_solver_params
+from diffmpc_amd.solvers.sqp import SolverReturnStatus
+from oracle import dynamics as odyn
+from oracle import sqp as osqp
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _oracle_problem(p, x0):
+    """The oracle problem of a spacecraft.yaml-style parameter dict for
+    initial states x0 (B, 3)."""
+    x0 = np.atleast_2d(np.asarray(x0, np.float64))
+    B, N = x0.shape[0], int(p["horizon"])
+    a = lambda k: np.asarray(_np(torch.as_tensor(p[k])), np.float64)  # noqa: E731
+    rep = lambda v: np.repeat(v.reshape(1, -1), B, 0)  # noqa: E731
+    return dict(model=odyn.make_model("spacecraft3"), horizon=N, dt=float(p["discretization_resolution"]),
+                scheme=int(p["discretization_scheme"]), penalize_control_reference=bool(p["penalize_control_reference"]),
+                x0=x0, x_final=rep(a("final_state")), x_ref=np.repeat(a("reference_state_trajectory")[None], B, 0),
+                u_ref=np.repeat(a("reference_control_trajectory")[None], B, 0),
+                w_x=rep(a("weights_penalization_reference_state_trajectory")),
+                w_u=rep(a("weights_penalization_control_squared")),
+                w_f=rep(a("weights_penalization_final_state")),
+                dyn=rep(a("inertia_vector")).astype(np.float32).astype(np.float64))
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def test_spacecraft_with_sqp_solver(cuda):
+    """tests/test_optimal_control.py:30-41."""
+    say("[reference tests] spacecraft.yaml with the SQP solver")
+    problem_params = load_problem_params("spacecraft.yaml")
+    solver_parameters = load_solver_params("sqp.yaml")
+    solver_parameters["pcg"]["tol_epsilon"] = 1e-15
+    problem = OptimalControlProblem(dynamics=SpacecraftDynamics(), params=problem_params)
+    solver = SQPSolver(program=problem, params=solver_parameters)
+    sol = solver.solve(solver.initial_guess(device=cuda), problem.params)
+    torch.cuda.synchronize()
+    assert float(sol.convergence_error) < 2e-5
+    assert SolverReturnStatus(int(sol.status)) is SolverReturnStatus.SUCCESS
+    ref = osqp.sqp_solve(_oracle_problem(problem_params, _np(problem_params["initial_state"])), solver_parameters)
+    assert _rel(_np(sol.states), ref["states"][0]) <= 1e-5
+    assert _rel(_np(sol.controls), ref["controls"][0]) <= 1e-5
+
+
+def _sqp_solve_spacecraft(cuda, num_batch, horizon, use_linesearch, warm_start_backward):
+    """sqp_solve_spacecraft of tests/test_solver_sqp.py:30-80: inertia and
+    initial states from np.random.seed(0), zero references, no terminal
+    weight, 20 SQP iterations to 1e-6, PCG 1e-12, alphas [1.0]; solves
+    initial_states[0] from the straight-line guess."""
+    np.random.seed(0)
+    inertia_vector = 1.0 + np.random.rand(3) * 9.0
+    initial_states = -0.1 + np.random.rand(num_batch, 3) * 0.2
+    problem_params = load_problem_params("spacecraft.yaml")
+    N = horizon - 1
+    problem_params["horizon"] = N
+    problem_params["inertia_vector"] = torch.tensor(inertia_vector)
+    problem_params["reference_state_trajectory"] = torch.zeros(N + 1, 3, dtype=torch.float64)
+    problem_params["reference_control_trajectory"] = torch.zeros(N + 1, 3, dtype=torch.float64)
+    problem_params["weights_penalization_final_state"] = torch.zeros(3, dtype=torch.float64)
+    solver_params = load_solver_params("sqp.yaml")
+    solver_params["tol_convergence"] = 1.0e-6
+    solver_params["num_scp_iteration_max"] = 20
+    solver_params["pcg"]["tol_epsilon"] = 1.0e-12
+    solver_params["linesearch"] = use_linesearch
+    solver_params["warm_start_backward"] = warm_start_backward
+    solver_params["linesearch_alphas"] = [1.0]
+    dynamics = SpacecraftDynamics()
+    problem = OptimalControlProblem(dynamics=dynamics, params=problem_params)
+    solver = SQPSolver(program=problem, params=solver_params)
+    sqp_guess = solver.initial_guess()
+    weights = {k: problem_params[k] for k in ("weights_penalization_reference_state_trajectory",
+                                              "weights_penalization_control_squared")}
+    x0 = torch.tensor(initial_states[0])
+    guess = solver.initial_guess({**problem_params, "initial_state": x0}, device=cuda)
+    solution = solver.solve(guess, {**problem_params, "initial_state": x0}, weights)
+    torch.cuda.synchronize()
+    return solver, sqp_guess, solution, dynamics, problem_params, solver_params, initial_states[0]
+
+
+@pytest.mark.parametrize("num_batch", [1, 2])
+@pytest.mark.parametrize("horizon", [10, 20])
+@pytest.mark.parametrize("use_linesearch", [True, False])
+@pytest.mark.parametrize("warm_start_backward", [True, False])
+def test_sqp_solve(cuda, num_batch, horizon, use_linesearch, warm_start_backward):
+    """tests/test_solver_sqp.py:83-100 (plus the oracle's trajectories)."""
+    say(f"[reference tests] test_sqp_solve B={num_batch} T={horizon} ls={use_linesearch} ws={warm_start_backward}")
+    solver, sqp_guess, solution, dynamics, p, sp, x0 = _sqp_solve_spacecraft(
+        cuda, num_batch, horizon, use_linesearch, warm_start_backward)
+    assert solver.name == "SQPSolver"
+    assert solver.pcg is not None
+    assert sqp_guess.states.shape == (horizon, dynamics.num_states)
+    assert sqp_guess.controls.shape == (horizon, dynamics.num_controls)
+    assert sqp_guess.dual.shape == (horizon, dynamics.num_states)
+    assert int(solution.status) == SolverReturnStatus.SUCCESS
+    assert np.isfinite(float(solution.convergence_error))
+    ref = osqp.sqp_solve(_oracle_problem(p, x0), sp)
+    assert _rel(_np(solution.states), ref["states"][0]) <= 1e-5
+    assert _rel(_np(solution.controls), ref["controls"][0]) <= 1e-5
+
+
+def test_sqp_solve_warmstart(cuda):
+    """tests/test_solver_sqp.py:103-121."""
+    say("[reference tests] test_sqp_solve_warmstart")
+    it = {}
+    for warm_start_backward in (True, False):
+        *_, solution, _, _, _, _ = _sqp_solve_spacecraft(cuda, 1, 20, True, warm_start_backward)
+        assert int(solution.status) == SolverReturnStatus.SUCCESS
+        assert np.isfinite(float(solution.convergence_error))
+        it["warm" if warm_start_backward else "cold"] = int(solution.num_iter)
+    assert it["warm"] <= it["cold"]
+
+
+def test_sqp_solve_differentiable(cuda):
+    """tests/test_differentiable.py:21-92: horizon 10, x0 from np.random.seed(0),
+    30 SQP iterations to 1e-12, PCG 1e-24; the gradient of ||x*|| + ||u*||
+    with respect to the shared state and control weights, by autograd through
+    the implicit backward, against central differences of the oracle's
+    objective (|fd - grad| <= 1e-3 as there, and 1e-4 relative)."""
+    say("[reference tests] test_sqp_solve_differentiable")
+    np.random.seed(0)
+    horizon, N = 10, 9
+    problem_params = load_problem_params("spacecraft.yaml")
+    problem_params["initial_state"] = torch.tensor(-0.1 + np.random.rand(3) * 0.2)
+    problem_params["horizon"] = N
+    problem_params["reference_state_trajectory"] = torch.zeros(N + 1, 3, dtype=torch.float64)
+    problem_params["reference_control_trajectory"] = torch.zeros(N + 1, 3, dtype=torch.float64)
+    problem_params["weights_penalization_final_state"] = torch.zeros(3, dtype=torch.float64)
+    solver_params = load_solver_params("sqp.yaml")
+    solver_params["tol_convergence"] = 1.0e-12
+    solver_params["num_scp_iteration_max"] = 30
+    solver_params["pcg"]["tol_epsilon"] = 1.0e-24
+    solver_params["linesearch"] = False
+    solver_params["warm_start_backward"] = True
+    problem = OptimalControlProblem(dynamics=SpacecraftDynamics(), params=problem_params)
+    solver = SQPSolver(program=problem, params=solver_params)
+    keys = ("weights_penalization_reference_state_trajectory", "weights_penalization_control_squared")
+    weights = {k: problem_params[k].clone().to(cuda).requires_grad_(True) for k in keys}
+    sol = solver.solve(solver.initial_guess(device=cuda), problem_params, weights)
+    objective = torch.linalg.norm(sol.states) + torch.linalg.norm(sol.controls)
+    grads = torch.autograd.grad(objective, [weights[k] for k in keys])
+    torch.cuda.synchronize()
+
+    base = _oracle_problem(problem_params, _np(problem_params["initial_state"]))
+
+    def f(p):
+        s = osqp.sqp_solve(p, solver_params)
+        return np.linalg.norm(s["states"][0]) + np.linalg.norm(s["controls"][0])
+
+    h = 1e-6
+    for key, g in zip(("w_x", "w_u"), grads):
+        fd = np.zeros(3)
+        for i in range(3):
+            hi, lo = dict(base), dict(base)
+            hi[key], lo[key] = base[key].copy(), base[key].copy()
+            hi[key][0, i] += h
+            lo[key][0, i] -= h
+            fd[i] = (f(hi) - f(lo)) / (2 * h)
+        say(f"  {key}: grad {_np(g)} fd {fd}")
+        assert np.allclose(fd - _np(g), 0.0, atol=1e-3, rtol=1e-3)
+        assert _rel(_np(g), fd) <= 1e-4
