@@ -128,3 +128,29 @@ def test_batch_params_expands_once_and_keeps_gradients():
     # a weights dict without the model parameters gets no flat block
     ow = solver.batch_params({"weights_penalization_control_squared": p["weights_penalization_control_squared"]}, 3)
     assert "_flat_dynamics_parameters" not in ow and ow["weights_penalization_control_squared"].shape == (3, 4)
+
+
+def test_jax_utils_counterparts():
+    """value_and_jacfwd/jacrev against a closed-form Jacobian, and the PSD
+    projection against the oracle's (diffmpc/utils/jax_utils.py:12-95;
+    the reference's tests/test_utils.py:81-93 checks the projected spectrum)."""
+    from diffmpc_amd.utils import (project_matrix_onto_positive_semidefinite_cone, value_and_jacfwd,
+                                   value_and_jacrev)
+
+    A = torch.tensor([[1.0, 2.0, 0.5], [0.0, -1.0, 3.0]], dtype=torch.float64)
+    f = lambda x: A @ x + torch.sin(x[:2])  # noqa: E731
+    x = torch.tensor([0.3, -0.7, 1.1], dtype=torch.float64)
+    want = A.clone()
+    want[0, 0] += torch.cos(x[0])
+    want[1, 1] += torch.cos(x[1])
+    for fn in (value_and_jacfwd, value_and_jacrev):
+        y, J = fn(f, x)
+        assert torch.allclose(y, f(x)) and J.shape == (2, 3) and torch.allclose(J, want, atol=1e-14)
+    rng = np.random.default_rng(3)
+    for n in (2, 5, 13):
+        M = rng.normal(size=(n, n))
+        M = 0.5 * (M + M.T)
+        P = project_matrix_onto_positive_semidefinite_cone(torch.tensor(M), 1e-3)
+        assert np.allclose(P.numpy(), osqp.project_psd(M, 1e-3), atol=1e-12)
+        assert torch.linalg.eigvalsh(P).min() >= 1e-3 - 1e-12
+        assert torch.equal(P, P.T)
