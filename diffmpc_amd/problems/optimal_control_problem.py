@@ -139,6 +139,64 @@ class OptimalControlProblem:
         c = c + ((wx + wf) * (states[..., N, :] - xf) ** 2).sum(-1)
         return c
 
+    def step_cost(self, state, control, params: Dict[str, Any]) -> torch.Tensor:
+        """sum w (z - ref)^2 of one step over [x; u] (ocp.py:153-185); params
+        hold that step's references.  Leading axes broadcast."""
+        xr = _as_tensor(params["reference_state_trajectory"], state.device)
+        ur = _as_tensor(params["reference_control_trajectory"], state.device)
+        if not self.params["penalize_control_reference"]:
+            ur = torch.zeros_like(ur)
+        wx = _as_tensor(params["weights_penalization_reference_state_trajectory"], state.device)
+        wu = _as_tensor(params["weights_penalization_control_squared"], state.device)
+        return (wx * (state - xr) ** 2).sum(-1) + (wu * (control - ur) ** 2).sum(-1)
+
+    def final_cost(self, state, params: Dict[str, Any]) -> torch.Tensor:
+        """(w_x + w_f) (x_N - x_final)^2 summed (ocp.py:187-206)."""
+        wx = _as_tensor(params["weights_penalization_reference_state_trajectory"], state.device)
+        wf = _as_tensor(params["weights_penalization_final_state"], state.device)
+        xf = _as_tensor(params["final_state"], state.device)
+        return ((wx + wf) * (state - xf) ** 2).sum(-1)
+
+    def get_cost_linearized_matrices(self, states, controls, params: Dict[str, Any] = None):
+        """(QN, qN, Q, q, R, r) of the quadratic model of the cost at
+        (states, controls) (ocp.py:249-323): c = 0.5 x'Qx + q'x + 0.5 u'Ru + r'u
+        + const.  The reference takes them by autodiff of step_cost and
+        final_cost; those are quadratic with diagonal weights, so here
+        Q = 2 diag(w_x), q_t = grad - Q x_t = -2 w_x ref_t, R = 2 diag(w_u),
+        r_t = -2 w_u uref_t (0 unless the control reference is penalised),
+        QN = 2 diag(w_x + w_f), qN = -2 (w_x + w_f) x_final.  Leading batch axes
+        are kept; Q (..., N, nx, nx), q (..., N, nx), R (..., N, nu, nu),
+        r (..., N, nu)."""
+        if params is None:
+            params = self.params
+        N = self.horizon
+        dev = states.device
+        wx = _as_tensor(params["weights_penalization_reference_state_trajectory"], dev)
+        wu = _as_tensor(params["weights_penalization_control_squared"], dev)
+        wf = _as_tensor(params["weights_penalization_final_state"], dev)
+        xr = _as_tensor(params["reference_state_trajectory"], dev)[..., :N, :]
+        ur = _as_tensor(params["reference_control_trajectory"], dev)[..., :N, :]
+        xf = _as_tensor(params["final_state"], dev)
+        lead = states.shape[:-2]
+        wxt = wx.unsqueeze(-2).expand(lead + (N, wx.shape[-1]))
+        wut = wu.unsqueeze(-2).expand(lead + (N, wu.shape[-1]))
+        Q = torch.diag_embed(2.0 * wxt)
+        q = (-2.0 * wxt * xr).expand(lead + (N, wx.shape[-1]))
+        R = torch.diag_embed(2.0 * wut)
+        r = (-2.0 * wut * ur if self.params["penalize_control_reference"] else torch.zeros_like(wut))
+        wN = (wx + wf).expand(lead + (wx.shape[-1],))
+        QN = torch.diag_embed(2.0 * wN)
+        qN = -2.0 * wN * xf
+        return QN, qN, Q, q, R, r
+
+    def get_dynamics_linearized_matrices_with_states_controls(self, states, controls, params: Dict[str, Any] = None):
+        """(As_next, As, Bs, Cs) at (states, controls) on the GPU (ocp.py:325-398);
+        states (N+1, nx) as the reference passes them, or batched."""
+        if states.dim() == 2:
+            out = self.linearize(states.unsqueeze(0), controls.unsqueeze(0), params)
+            return tuple(t[0] for t in out)
+        return self.linearize(states, controls, params)
+
     def device_problem(self, params: Dict[str, Any], batch: int, device) -> Dict[str, torch.Tensor]:
         """Batched, contiguous device tensors of the C ABI's dmpc_problem_data."""
         N, nx, nu = self.horizon, self.num_state_variables, self.num_control_variables

@@ -30,8 +30,11 @@ from .. import _lib
 from ..dynamics.base_dynamics import batched
 from ..problems.optimal_control_problem import OptimalControlProblem
 from ..utils.load_params import load_solver_params
-from .pcg import (PCGOptimalControl, QPCostParameters, QPDynamicsParameters, QPParameters,
-                  SchurComplementMatrices)
+from ..utils.torch_utils import project_matrix_onto_positive_semidefinite_cone
+from .pcg import (PCGOptimalControl, QPCostParameters, QPDynamicsCostPremultipliedMatrices, QPDynamicsParameters,
+                  QPParameters, QPSolution, SchurComplementMatrices)
+
+PSD_MIN_EIGENVALUE = 1e-9  # sqp.py:191-196
 
 DEFAULT_SOLVER_PARAMS = load_solver_params("sqp.yaml")
 
@@ -436,6 +439,61 @@ class SQPSolver:
             conv_errs, ls_alphas, pcg_iters, dual_bwd_out = conv_errs[0], ls_alphas[0], pcg_iters[0], dual_bwd_out[0]
         return SQPSolution(x, u, lam, saved, None, None, status, num_iter, conv_err, conv_errs, ls_alphas,
                            pcg_iters, dual_bwd_out)
+
+    # ---- the pieces of one SQP iteration under the reference's names
+    # (sqp.py:160-299, 413-424, 562-603).  The solve itself does not call
+    # them: it runs the fused kernels.  They take one problem as the
+    # reference does ((N+1, nx) trajectories) or a batch.
+
+    def get_cost_terms(self, states, controls, problem_params: Dict[str, Any] = None):
+        """(Qs (N+1, nx, nx), qs (N+1, nx), Rs (N, nu, nu), rs (N, nu)) of the
+        quadratic cost model, Hessians projected onto eigenvalues >= 1e-9
+        (sqp.py:160-197)."""
+        QN, qN, Q, q, R, r = self.program.get_cost_linearized_matrices(
+            states, controls, self.problem_params if problem_params is None else problem_params)
+        Qs = torch.cat([Q, QN.unsqueeze(-3)], dim=-3)
+        qs = torch.cat([q, qN.unsqueeze(-2)], dim=-2)
+        return (project_matrix_onto_positive_semidefinite_cone(Qs, PSD_MIN_EIGENVALUE), qs,
+                project_matrix_onto_positive_semidefinite_cone(R, PSD_MIN_EIGENVALUE), r)
+
+    def get_dynamics_constraints_terms(self, states, controls, problem_params: Dict[str, Any] = None):
+        """QPDynamicsParameters (As_next, As, Bs, Cs) of the dynamics linearized
+        at (states, controls) (sqp.py:199-228), on the GPU."""
+        return QPDynamicsParameters(*self.program.get_dynamics_linearized_matrices_with_states_controls(
+            states, controls, self.problem_params if problem_params is None else problem_params))
+
+    def compute_qp_parameters(self, states, controls, problem_params: Dict[str, Any] = None):
+        """(QPParameters, QPDynamicsCostPremultipliedMatrices,
+        SchurComplementMatrices) at (states, controls) (sqp.py:230-260)."""
+        Qs, qs, Rs, rs = self.get_cost_terms(states, controls, problem_params)
+        dyn = self.get_dynamics_constraints_terms(states, controls, problem_params)
+        Qinv = torch.linalg.inv(Qs)
+        Rinv = torch.linalg.inv(Rs)
+        qp = QPParameters(dyn, QPCostParameters(Qinv, qs, Rinv, rs))
+        pre = QPDynamicsCostPremultipliedMatrices(dyn.As @ Qinv[..., :-1, :, :], dyn.As_next @ Qinv[..., 1:, :, :],
+                                                  dyn.Bs @ Rinv)
+        return qp, pre, self.pcg.compute_S_Phiinv(qp)
+
+    def solve_one_scp_iteration(self, qp_solution: QPSolution, problem_params: Dict[str, Any] = None):
+        """One QP of the SQP iteration at qp_solution (sqp.py:262-299): returns
+        (QPSolution, QPParameters, premultiplied matrices, Schur matrices,
+        PCG iteration count), through the PCG layer's entry points."""
+        qp, pre, schur = self.compute_qp_parameters(qp_solution.states, qp_solution.controls, problem_params)
+        gammas = self.pcg.compute_gamma(pre, qp.dynamics.Cs, qp.cost.Qinv[..., 0, :, :], qp.cost.qvec, qp.cost.rvec)
+        sol, dbg = self.pcg.solve_KKT_Schur(qp, schur, gammas, qp_solution.kkt_multipliers)
+        return sol, qp, pre, schur, dbg.num_iterations
+
+    @staticmethod
+    def state_control_difference(states, controls, new_states, new_controls):
+        """max(max |new_states - states|, max |new_controls - controls|) over
+        each problem's trajectory (sqp.py:413-424)."""
+        return torch.maximum((new_states - states).abs().amax(dim=(-2, -1)),
+                             (new_controls - controls).abs().amax(dim=(-2, -1)))
+
+    def get_differentiable_solve_function(self):
+        """solve(initial_guess, problem_params, weights), differentiable in the
+        weights (sqp.py:562-603)."""
+        return self.solve
 
     def qp_parameters(self, solution: SQPSolution) -> QPParameters:
         """Dense QP blocks of the last linearization point of a batched solution

@@ -189,3 +189,132 @@ def test_sqp_solve_differentiable(cuda):
         say(f"  {key}: grad {_np(g)} fd {fd}")
         assert np.allclose(fd - _np(g), 0.0, atol=1e-3, rtol=1e-3)
         assert _rel(_np(g), fd) <= 1e-4
+
+
+# ---- tests/test_solver_pcg.py, one problem at a time as there ---------------
+from diffmpc_amd.solvers.pcg import (PCGOptimalControl, QPCostParameters, QPDynamicsParameters,  # noqa: E402
+                                     QPParameters)
+
+
+def _t(a, cuda):
+    return torch.as_tensor(np.asarray(a, np.float64), device=cuda)
+
+
+def test_pcg_constructor_and_zero_shapes():
+    """tests/test_solver_pcg.py:17-56 (no GPU work)."""
+    solver_params = {"max_iter": 50, "tol_epsilon": 1e-5}
+    oc = PCGOptimalControl(10, 4, 2, solver_params)
+    assert (oc.name, oc.horizon, oc.num_states, oc.num_controls, oc.params) == (
+        "PCGOptimalControl", 10, 4, 2, solver_params)
+    qp = PCGOptimalControl(5, 3, 2).zero_qp_parameters()
+    assert qp.dynamics.As_next.shape == (5, 3, 3) and qp.dynamics.Bs.shape == (5, 3, 2)
+    assert qp.dynamics.Cs.shape == (6, 3) and qp.cost.Qinv.shape == (6, 3, 3) and qp.cost.rvec.shape == (5, 2)
+    oc = PCGOptimalControl(4, 2, 1)
+    assert oc.zero_dynamics_cost_premultiplied_matrices().Bs_x_Rinv.shape == (4, 2, 1)
+    assert oc.zero_schur_complement_matrices().S.shape == (5, 2, 6)
+
+
+def _identity_qp(T, nx, nu, cuda, As=None, As_next=None, Qinv=None):
+    eye = np.eye(nx)
+    return QPParameters(
+        QPDynamicsParameters(As_next=_t(np.broadcast_to(eye if As_next is None else As_next, (T, nx, nx)), cuda),
+                             As=_t(np.broadcast_to(eye if As is None else As, (T, nx, nx)), cuda),
+                             Bs=_t(np.ones((T, nx, nu)), cuda), Cs=_t(np.ones((T, nx)), cuda)),
+        QPCostParameters(Qinv=_t(np.tile(eye if Qinv is None else Qinv, (T + 1, 1, 1)), cuda),
+                         qvec=_t(np.zeros((T + 1, nx)), cuda), Rinv=_t(np.tile(np.eye(nu), (T, 1, 1)), cuda),
+                         rvec=_t(np.zeros((T, nu)), cuda)))
+
+
+@pytest.mark.parametrize("T", [2, 5])
+@pytest.mark.parametrize("nx", [1, 5])
+@pytest.mark.parametrize("nu", [1, 5])
+def test_pcg_layer_small_problems(cuda, T, nx, nu):
+    """tests/test_solver_pcg.py:58-177: S/Phiinv of identity dynamics, gamma of
+    zeros, recovery from zero multipliers (with the test's (T+1, 1) cost
+    vectors, which broadcast), and the trivial KKT solve."""
+    say(f"[reference tests] pcg layer T={T} nx={nx} nu={nu}")
+    oc = PCGOptimalControl(T, nx, nu, {"max_iter": 10, "tol_epsilon": 1e-8})
+    qp = _identity_qp(T, nx, nu, cuda)
+    schur = oc.compute_S_Phiinv(qp)
+    assert schur.S.shape == (T + 1, nx, 3 * nx) and schur.preconditioner_Phiinv.shape[0] == T + 1
+    assert not torch.isnan(schur.S).any()
+    dyn_cost = oc.zero_dynamics_cost_premultiplied_matrices(device=cuda)
+    gamma = oc.compute_gamma(dyn_cost, _t(np.zeros((T + 1, nx)), cuda), _t(np.eye(nx), cuda),
+                             _t(np.zeros((T + 1, nx)), cuda), _t(np.zeros((T, nu)), cuda))
+    assert gamma.shape == (T + 1, nx) and torch.allclose(gamma, torch.zeros_like(gamma))
+    z = oc.zero_qp_parameters(device=cuda)
+    qpz = QPParameters(z.dynamics, QPCostParameters(Qinv=_t(np.tile(np.eye(nx), (T + 1, 1, 1)), cuda),
+                                                    qvec=_t(np.zeros((T + 1, 1)), cuda),
+                                                    Rinv=_t(np.tile(np.eye(nu), (T, 1, 1)), cuda),
+                                                    rvec=_t(np.zeros((T, 1)), cuda)))
+    sol = oc.get_states_controls_from_kkt_multipliers(_t(np.zeros((T + 1, nx)), cuda), qpz)
+    assert sol.states.shape == (T + 1, nx) and sol.controls.shape == (T + 1, nu)
+    assert not sol.states.abs().any() and not sol.controls.abs().any()
+    if T <= 3 and nx <= 2 and nu <= 2:
+        gammas = oc.compute_gamma(dyn_cost, _t(np.ones((T + 1, nx)), cuda), _t(np.eye(nx), cuda),
+                                  _t(np.zeros((T + 1, nx)), cuda), _t(np.zeros((T, nu)), cuda))
+        sol, debug = oc.solve_KKT_Schur(qp, schur, gammas, _t(np.zeros((T + 1, nx)), cuda))
+        assert sol.states.shape == (T + 1, nx) and sol.controls.shape == (T + 1, nu)
+        assert int(debug.num_iterations) <= 10 and float(debug.convergence_eta) >= 0.0
+
+
+def test_pcg_layer_warmstart(cuda):
+    """tests/test_solver_pcg.py:179-227: a converged guess is kept and needs
+    fewer iterations."""
+    say("[reference tests] pcg warm start")
+    T, nx, nu = 3, 4, 5
+    rng = np.random.default_rng(0)
+    oc = PCGOptimalControl(T, nx, nu, {"max_iter": 20, "tol_epsilon": 1e-13})
+    qp = _identity_qp(T, nx, nu, cuda, As=np.diag(rng.random(nx)), As_next=np.diag(rng.random(nx)),
+                      Qinv=np.diag(rng.random(nx)))
+    schur = oc.compute_S_Phiinv(qp)
+    dyn_cost = oc.zero_dynamics_cost_premultiplied_matrices(device=cuda)
+    gammas = oc.compute_gamma(dyn_cost, _t(np.ones((T + 1, nx)), cuda), _t(np.eye(nx), cuda),
+                              _t(np.zeros((T + 1, nx)), cuda), _t(np.zeros((T, nu)), cuda))
+    sol, debug = oc.solve_KKT_Schur(qp, schur, gammas, _t(np.zeros((T + 1, nx)), cuda))
+    assert int(debug.num_iterations) <= 10 and float(debug.convergence_eta) >= 0.0
+    sol_w, debug_w = oc.solve_KKT_Schur(qp, schur, gammas, sol.kkt_multipliers)
+    assert torch.allclose(sol.states, sol_w.states) and torch.allclose(sol.controls, sol_w.controls)
+    assert torch.allclose(sol.kkt_multipliers, sol_w.kkt_multipliers)
+    assert int(debug_w.num_iterations) < int(debug.num_iterations)
+
+
+def test_one_sqp_iteration_through_the_layer_matches_oracle(cuda):
+    """SQPSolver.solve_one_scp_iteration (sqp.py:262-299: cost terms, GPU
+    linearization, Qinv/Rinv, premultiplied products, S/Phiinv, gamma, PCG)
+    on one quadrotor problem from quadrotor.yaml, against the oracle's QP of
+    the same iteration, and the same for a batch of two."""
+    say("[reference tests] one SQP iteration through the PCG layer")
+    from diffmpc_amd import QuadrotorDynamics
+    from diffmpc_amd.solvers.pcg import QPSolution
+
+    p = load_problem_params("quadrotor.yaml")
+    sp = load_solver_params("sqp.yaml")
+    # quadrotor.yaml's weights (1e-4 .. 1e5) make this QP ill-conditioned: at
+    # eps 1e-14 the two PCG runs (float32 and float64 A_t) stop ~1.5e-5 apart,
+    # so both solve it to the end
+    sp["pcg"]["tol_epsilon"] = 1e-24
+    sp["pcg"]["max_iter"] = 20000
+    solver = SQPSolver(OptimalControlProblem(QuadrotorDynamics(), p), params=sp)
+    g = solver.initial_guess(device=cuda)
+    rng = np.random.default_rng(2)
+    x = g.states + 0.05 * torch.tensor(rng.normal(size=tuple(g.states.shape)), device=cuda)
+    u = g.controls + 0.1 * torch.tensor(rng.normal(size=tuple(g.controls.shape)), device=cuda)
+    sol, qp, pre, schur, iters = solver.solve_one_scp_iteration(QPSolution(x, u, g.dual), p)
+    torch.cuda.synchronize()
+    rep = lambda k: np.asarray(_np(torch.as_tensor(p[k])), np.float64)[None]  # noqa: E731
+    op = dict(model=odyn.make_model("quadrotor"), horizon=int(p["horizon"]), dt=0.03, scheme=2,
+              penalize_control_reference=True, x0=rep("initial_state"), x_final=rep("final_state"),
+              x_ref=rep("reference_state_trajectory"), u_ref=rep("reference_control_trajectory"),
+              w_x=rep("weights_penalization_reference_state_trajectory"),
+              w_u=rep("weights_penalization_control_squared"), w_f=rep("weights_penalization_final_state"),
+              dyn=np.concatenate([rep("mass"), rep("inertia")], axis=1).astype(np.float32).astype(np.float64))
+    qpp = osqp.qp_parameters(op, _np(x)[None], _np(u)[None])
+    xo, uo, lo, *_ = osqp.solve_qp(qpp, np.zeros((1, op["horizon"] + 1, 13)), sp["pcg"])
+    say(f"  PCG iterations {int(iters)}; rel x {_rel(_np(sol.states), xo[0]):.2e} u {_rel(_np(sol.controls), uo[0]):.2e}")
+    assert _rel(_np(sol.states), xo[0]) <= 1e-5 and _rel(_np(sol.controls), uo[0]) <= 1e-5
+    assert _rel(_np(sol.kkt_multipliers), lo[0]) <= 1e-5
+    xb, ub = torch.stack([x, x]), torch.stack([u, u])
+    solb, *_ = solver.solve_one_scp_iteration(QPSolution(xb, ub, torch.stack([g.dual, g.dual])), p)
+    torch.cuda.synchronize()
+    assert torch.allclose(solb.states[1], sol.states, rtol=1e-12, atol=1e-14)

@@ -115,7 +115,7 @@ def test_cost_and_constraints_match_oracle():
 
 def test_batch_params_expands_once_and_keeps_gradients():
     p = load_problem_params("quadrotor.yaml")
-    solver = SQPSolver(OptimalControlProblem(QuadrotorDynamics(), p), load_solver_params("sqp.yaml"))
+    solver = SQPSolver(OptimalControlProblem(QuadrotorDynamics(), p), params=load_solver_params("sqp.yaml"))
     w = p["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
     out = solver.batch_params({**p, "weights_penalization_reference_state_trajectory": w}, 3)
     assert out["reference_state_trajectory"].shape == (3, 26, 13) and out["reference_state_trajectory"].is_contiguous()
@@ -154,3 +154,41 @@ def test_jax_utils_counterparts():
         assert np.allclose(P.numpy(), osqp.project_psd(M, 1e-3), atol=1e-12)
         assert torch.linalg.eigvalsh(P).min() >= 1e-3 - 1e-12
         assert torch.equal(P, P.T)
+
+
+def _quadrotor_oracle_problem(p, B=1):
+    rep = lambda k: np.repeat(p[k].numpy()[None], B, 0)  # noqa: E731
+    return dict(model=odyn.make_model("quadrotor"), horizon=int(p["horizon"]), dt=0.03, scheme=2,
+                penalize_control_reference=True, x0=rep("initial_state"), x_final=rep("final_state"),
+                x_ref=rep("reference_state_trajectory"), u_ref=rep("reference_control_trajectory"),
+                w_x=rep("weights_penalization_reference_state_trajectory"),
+                w_u=rep("weights_penalization_control_squared"), w_f=rep("weights_penalization_final_state"),
+                dyn=np.repeat(np.concatenate([p["mass"].numpy(), p["inertia"].numpy()])[None], B, 0))
+
+
+def test_cost_terms_match_oracle_and_the_cost():
+    """SQPSolver.get_cost_terms / OptimalControlProblem.get_cost_linearized_matrices
+    (sqp.py:160-197, ocp.py:249-323) against the oracle's cost_terms, with the
+    control reference penalised (quadrotor.yaml), one problem and a batch; and
+    step_cost + final_cost summing to cost (ocp.py:153-247)."""
+    p = load_problem_params("quadrotor.yaml")
+    prob = OptimalControlProblem(QuadrotorDynamics(), p)
+    solver = SQPSolver(prob, params=load_solver_params("sqp.yaml"))
+    N = prob.horizon
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(2, N + 1, 13))
+    u = rng.normal(size=(2, N + 1, 4))
+    op = _quadrotor_oracle_problem(p, 2)
+    Qw, qw, Rw, rw = osqp.cost_terms(op, x, u)
+    Qs, qs, Rs, rs = solver.get_cost_terms(torch.tensor(x), torch.tensor(u), p)
+    for got, want in ((Qs, Qw), (qs, qw), (Rs, Rw), (rs, rw)):
+        assert got.shape == want.shape and np.allclose(got.numpy(), want, rtol=1e-12, atol=1e-12)
+    Q1, q1, R1, r1 = solver.get_cost_terms(torch.tensor(x[1]), torch.tensor(u[1]), p)
+    assert np.allclose(Q1.numpy(), Qw[1]) and np.allclose(q1.numpy(), qw[1]) and np.allclose(r1.numpy(), rw[1])
+    xt, ut = torch.tensor(x[0]), torch.tensor(u[0])
+    steps = prob.step_cost(xt[:N], ut[:N], {**p, "reference_state_trajectory": p["reference_state_trajectory"][:N],
+                                            "reference_control_trajectory": p["reference_control_trajectory"][:N]})
+    total = steps.sum() + prob.final_cost(xt[N], p)
+    assert np.isclose(total.item(), prob.cost(xt, ut, p).item(), rtol=1e-13)
+    d = SQPSolver.state_control_difference(xt, ut, xt + 0.5, ut - 2.0)
+    assert d.item() == 2.0
