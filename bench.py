@@ -488,16 +488,19 @@ def bench_imitation(args):
     problem = il.cartpole_problem()
     solver = il.SQPSolver(program=problem, params=il.training_solver_params())
     tr = il.ImitationLearning(solver, x0, us, device=dev)
-    step = tr.graphed_step()
-    w = torch.tensor(il.INITIAL_WEIGHTS[0], dtype=torch.float64, device=dev)
+    g = tr.captured_step()  # one replay = one training step, chained through the graph's weight buffer
+    w0 = torch.tensor(il.INITIAL_WEIGHTS[0], dtype=torch.float64, device=dev)
+    g["w"].copy_(w0)
     for _ in range(args.warmup):
-        w, _ = step(w)
-    w = torch.tensor(il.INITIAL_WEIGHTS[0], dtype=torch.float64, device=dev)
+        g["graph"].replay()
+    g["w"].copy_(w0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    losses, model_losses = [], []
     for _ in range(args.steps):
-        m = tr.model_loss(w)
-        w, loss = step(w)
+        g["graph"].replay()
+        losses.append(g["loss"].clone())
+        model_losses.append(g["model_loss"].clone())
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     out = {
@@ -509,7 +512,7 @@ def bench_imitation(args):
                                                                      "from the reference's seeded initial states)",
         "config": {"workload": f"cart-pole imitation learning N=100 B={B}, <=5 SQP iterations, tol 0.1, PCG 1e-14, "
                                "line search; one hipGraph per step", "global_batch": B},
-        "imitation_loss": float(loss), "model_loss": float(m), "weights": w.tolist(),
+        "imitation_loss": float(losses[-1]), "model_loss": float(model_losses[-1]), "weights": g["w"].tolist(),
         "expert_collection_s": t_expert, "expert_max_convergence_error": float(expert.convergence_error.max()),
         "reference_ms_per_step": IL_REFERENCE_MS,
     }

@@ -24,7 +24,7 @@ loss, backward, update -- can be captured once and replayed as one hipGraph.
 Demonstrations are stored as ``.npz`` (x0, ctrl_traj).
 """
 import time
-from typing import Callable, Dict, Optional, Sequence
+from typing import Any, Callable, Dict, Optional, Sequence
 
 import numpy as np
 import torch
@@ -204,57 +204,88 @@ class ImitationLearning:
         """|w - w_true| (train_imitation_diffmpc.py:83-85)."""
         return torch.linalg.norm(torch.as_tensor(w, dtype=torch.float64, device=self.device) - self.true_weights)
 
+    def captured_step(self) -> Dict[str, Any]:
+        """The hipGraph of one training step, created on first use: it reads
+        the weights in `w`, writes the model loss of those weights, the
+        step's imitation loss and the updated weights, and finally stores the
+        updated weights back into `w`, so that consecutive replays are
+        consecutive training steps with no host work in between."""
+        if self._graph is None:
+            nx = self.true_weights.numel()
+            w = self.true_weights.clone().reshape(nx)
+            w_out = torch.empty_like(w)
+            loss_out = torch.empty((), dtype=torch.float64, device=self.device)
+            model_out = torch.empty((), dtype=torch.float64, device=self.device)
+
+            def fn():
+                model_out.copy_(self.model_loss(w))
+                w_new, loss = self.step(w)
+                loss_out.copy_(loss)
+                w_out.copy_(w_new)
+                w.copy_(w_new)
+                return w_out, loss_out, model_out
+
+            self._graph = dict(graph=GraphedStep(fn), w=w, w_out=w_out, loss=loss_out, model_loss=model_out)
+        return self._graph
+
     def graphed_step(self) -> Callable:
         """The step captured as one hipGraph.  Returns step(w) -> (new w, loss)
         with the same meaning as ``step``; the outputs are fresh tensors."""
-        if self._graph is None:
-            nx = self.true_weights.numel()
-            w_in = self.true_weights.clone().reshape(nx)
-            w_out = torch.empty_like(w_in)
-            loss_out = torch.empty((), dtype=torch.float64, device=self.device)
-
-            def fn():
-                w_new, loss = self.step(w_in)
-                w_out.copy_(w_new)
-                loss_out.copy_(loss)
-                return w_out, loss_out
-
-            self._graph = (GraphedStep(fn), w_in)
-        graph, w_in = self._graph
+        g = self.captured_step()
 
         def run(w):
-            w_in.copy_(torch.as_tensor(w, dtype=torch.float64, device=self.device).reshape(w_in.shape))
-            w_out, loss = graph.replay()
-            return w_out.clone(), loss.clone()
+            g["w"].copy_(torch.as_tensor(w, dtype=torch.float64, device=self.device).reshape(g["w"].shape))
+            g["graph"].replay()
+            return g["w_out"].clone(), g["loss"].clone()
 
         return run
 
     def train(self, initial_weights: Sequence = INITIAL_WEIGHTS, num_steps: int = NUM_STEPS, graphed: bool = True,
               log: Optional[Callable[[str], None]] = None) -> Dict[str, np.ndarray]:
         """The training loop of train_imitation_diffmpc.py:87-124.  For each
-        starting point: entry 0 holds the loss at the starting weights and
-        time 0, then one entry per step (model loss before the step, imitation
-        loss of the step, seconds since the run's start).  Returns
-        model_losses, imitation_losses, times as (runs, num_steps + 1) arrays
-        and the final weights (runs, nx)."""
-        step = self.graphed_step() if graphed else self.step
+        starting point: entry 0 holds the loss and model loss at the starting
+        weights and time 0, then one entry per step (model loss before the
+        step, imitation loss of the step, seconds since the run's start).
+        Returns model_losses, imitation_losses, times as (runs, num_steps + 1)
+        arrays and the final weights (runs, nx).  With `graphed`, each step is
+        one replay of the captured step (its first replay gives entry 0 and
+        entry 1, which the reference computes twice)."""
         all_model, all_imit, all_times, finals = [], [], [], []
         for i, w0 in enumerate(initial_weights):
             w = torch.as_tensor(w0, dtype=torch.float64, device=self.device)
-            imit = [step(w)[1]]
-            model = [self.model_loss(w)]
             times = [0.0]
-            torch.cuda.synchronize(self.device)
-            s = time.monotonic()
-            for k in range(num_steps):
-                model.append(self.model_loss(w))
-                w, loss = step(w)
-                imit.append(loss)
+            if graphed:
+                g = self.captured_step()
+                g["w"].copy_(w.reshape(g["w"].shape))
                 torch.cuda.synchronize(self.device)
-                times.append(time.monotonic() - s)
-                if log is not None:
-                    log(f"run {i + 1}/{len(initial_weights)}, step {k}/{num_steps}: IL loss {float(loss):.6g}, "
-                        f"model loss {float(model[-1]):.6g}, time {times[-1]:.3f}")
+                s = time.monotonic()
+                model, imit = [], []
+                for k in range(num_steps):
+                    g["graph"].replay()
+                    model.append(g["model_loss"].clone())
+                    imit.append(g["loss"].clone())
+                    torch.cuda.synchronize(self.device)
+                    times.append(time.monotonic() - s)
+                    if log is not None:
+                        log(f"run {i + 1}/{len(initial_weights)}, step {k}/{num_steps}: IL loss {float(imit[-1]):.6g}, "
+                            f"model loss {float(model[-1]):.6g}, time {times[-1]:.3f}")
+                model = [model[0]] + model
+                imit = [imit[0]] + imit
+                w = g["w"].clone()
+            else:
+                imit = [self.step(w)[1]]
+                model = [self.model_loss(w)]
+                torch.cuda.synchronize(self.device)
+                s = time.monotonic()
+                for k in range(num_steps):
+                    model.append(self.model_loss(w))
+                    w, loss = self.step(w)
+                    imit.append(loss)
+                    torch.cuda.synchronize(self.device)
+                    times.append(time.monotonic() - s)
+                    if log is not None:
+                        log(f"run {i + 1}/{len(initial_weights)}, step {k}/{num_steps}: IL loss {float(loss):.6g}, "
+                            f"model loss {float(model[-1]):.6g}, time {times[-1]:.3f}")
             all_model.append(torch.stack(model).cpu().numpy())
             all_imit.append(torch.stack(imit).cpu().numpy())
             all_times.append(np.array(times))
