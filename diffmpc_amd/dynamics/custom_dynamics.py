@@ -212,8 +212,52 @@ def _is_fresh(out_dir: str, source: str) -> bool:
         return f.read() == source and g.read() == _sources_digest()
 
 
+# in-package location of prebuilt custom libraries (one folder per model name)
+PREBUILT_ROOT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "_custom")
+
+
+class CustomBuild:
+    """A model compiled into its own build of the library (model id
+    MODEL_CUSTOM).  Needs `source` (the C++ definition of dmpc::CustomModel)
+    and `name`; `_build_dir` may pin the folder of the build."""
+
+    source: str = ""
+    name: str = "custom"
+    _build_dir: Optional[str] = None
+    _lib_path: Optional[str] = None
+
+    def prebuilt_dir(self) -> str:
+        """The in-package folder a prebuilt library of this model lives in."""
+        return os.path.join(PREBUILT_ROOT, self.name)
+
+    def build_dir(self) -> str:
+        """The pinned folder; else the in-package prebuilt folder when it holds
+        a current build of this source; else a cache folder keyed by the
+        source and the library sources."""
+        if self._build_dir is not None:
+            return self._build_dir
+        if _is_fresh(self.prebuilt_dir(), self.source):
+            return self.prebuilt_dir()
+        key = hashlib.sha256((self.source + _sources_digest()).encode()).hexdigest()[:16]
+        root = os.environ.get("DMPC_CUSTOM_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "diffmpc_amd"))
+        return os.path.join(root, f"{self.name}-{key}")
+
+    def library_path(self, build: bool = True) -> str:
+        """Path of the compiled library for this model (compiled on first use
+        when `build`, and again when the library sources have changed since)."""
+        if self._lib_path is None:
+            d = self.build_dir()
+            lib = os.path.join(d, "libdmpc_custom.so")
+            if not _is_fresh(d, self.source):
+                if not build:
+                    raise FileNotFoundError(f"{lib} is not built for this model")
+                build_custom_library(self.source, d)
+            self._lib_path = lib
+        return self._lib_path
+
+
 # ---------------------------------------------------------------------------
-class CustomDynamics(Dynamics):
+class CustomDynamics(CustomBuild, Dynamics):
     """A user-defined model (see the module docstring).
 
     state_dot(x, u, p, m): the model; parameters: (key, size) pairs giving
@@ -256,26 +300,6 @@ class CustomDynamics(Dynamics):
             out[k] = (o, n)
             o += n
         return out
-
-    def build_dir(self) -> str:
-        if self._build_dir is not None:
-            return self._build_dir
-        key = hashlib.sha256((self.source + _sources_digest()).encode()).hexdigest()[:16]
-        root = os.environ.get("DMPC_CUSTOM_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "diffmpc_amd"))
-        return os.path.join(root, f"{self.name}-{key}")
-
-    def library_path(self, build: bool = True) -> str:
-        """Path of the compiled library for this model (compiled on first use
-        when `build`, and again when the library sources have changed since)."""
-        if self._lib_path is None:
-            d = self.build_dir()
-            lib = os.path.join(d, "libdmpc_custom.so")
-            if not _is_fresh(d, self.source):
-                if not build:
-                    raise FileNotFoundError(f"{lib} is not built for this model")
-                build_custom_library(self.source, d)
-            self._lib_path = lib
-        return self._lib_path
 
     def _flat(self, params: Dict[str, Any], batch: int, dtype, device) -> torch.Tensor:
         cols = []

@@ -1,10 +1,14 @@
 """Linear dynamics x' = A x + B u + b (diffmpc/dynamics/linear_dynamics.py:29-70)."""
-from typing import Any, Dict
+from typing import Any, Dict, Optional
 
 import torch
 
 from .._lib import MODEL_LINEAR
 from .base_dynamics import Dynamics, batched
+from .custom_dynamics import MODEL_CUSTOM, CustomBuild
+
+# (nx, nu) the library is built for; other sizes run in a custom build of it
+BUILT_SIZES = ((4, 2), (8, 4))
 
 default_parameters: Dict[str, Any] = {
     "num_states": 4,
@@ -19,14 +23,31 @@ default_state_dot_parameters: Dict[str, Any] = {
 }
 
 
-class LinearDynamics(Dynamics):
-    """Linear dynamics class: state_dot = A @ state + B @ control + b."""
+class LinearDynamics(CustomBuild, Dynamics):
+    """Linear dynamics class: state_dot = A @ state + B @ control + b.
+
+    Any (nx, nu), as in the reference: the sizes in BUILT_SIZES run in the
+    library itself; any other size (nx <= 64) is the same model template
+    compiled into a custom build of the library (model id MODEL_CUSTOM,
+    compiled on first use and cached, or prebuilt in the package)."""
 
     model_id = MODEL_LINEAR
     param_keys = ("A", "B", "b")
 
     def __init__(self, parameters: Dict[str, Any] = default_parameters):
         super().__init__(parameters)
+        nx, nu = self.num_states, self.num_controls
+        if (nx, nu) not in BUILT_SIZES:
+            self.model_id = MODEL_CUSTOM
+            self.name = f"linear{nx}x{nu}"
+            self.source = (f"// the linear model with nx = {nx}, nu = {nu}\n"
+                           f"using CustomModel = LinearModel<{nx}, {nu}>;\n")
+
+    def library_path(self, build: bool = True) -> Optional[str]:
+        """None for the built sizes (the library itself), else the custom build."""
+        if self.model_id == MODEL_LINEAR:
+            return None
+        return super().library_path(build)
 
     @property
     def num_params(self) -> int:

@@ -13,9 +13,10 @@ that follow from the batch axis being explicit here instead of a caller's
   only (problem_params are stop-gradient, sqp.py:456-458); shared weights get
   the batch-summed gradient, like ``vmap(in_axes=None)``.
 
-The solve runs in libdmpc.so: mode "fused" (default) is one workgroup per
-instance per SQP iteration, "plain" is the per-launch op graph of the
-reference.  Solver parameters use the reference's keys plus the optional
+The solve runs in libdmpc.so: mode "fused" is one workgroup per instance per
+SQP iteration, "plain" is the per-launch op graph of the reference, and
+"auto" (the default) takes fused where the library builds it for the model
+and size, else plain (linear models of any size).  Solver parameters use the reference's keys plus the optional
 ``"mode"``, ``"fused_variant"`` ("auto", "register", "streamed") and
 ``"blocks_per_group"`` (0 = automatic, 1, 2, 4), which diffmpc_amd.autotune
 sets.  ``problem_params["horizons"]`` (B,) makes a ragged batch.
@@ -123,11 +124,21 @@ def _ragged_horizons(h, B, N, device):
     return h.to(device=device, dtype=torch.int32).contiguous()
 
 
-def _mode(params) -> int:
-    m = params.get("mode", "fused")
-    if m not in ("fused", "plain"):
-        raise ValueError(f"mode must be 'fused' or 'plain', got {m!r}")
-    return _lib.MODE_FUSED if m == "fused" else _lib.MODE_PLAIN
+def _mode(params, prob, lib, ragged: bool) -> int:
+    """The execution mode of a solve: params["mode"] "fused" or "plain", or
+    "auto" (the default): the fused kernels where the library builds them for
+    this model and size, else the per-launch kernels (which take any linear
+    model size but no ragged horizons)."""
+    m = params.get("mode", "auto")
+    if m not in ("auto", "fused", "plain"):
+        raise ValueError(f"mode must be 'auto', 'fused' or 'plain', got {m!r}")
+    if m != "auto":
+        return _lib.MODE_FUSED if m == "fused" else _lib.MODE_PLAIN
+    if lib.dmpc_supported(_lib.MODE_FUSED, prob.model, prob.nx, prob.nu, prob.horizon):
+        return _lib.MODE_FUSED
+    if not ragged and lib.dmpc_supported(_lib.MODE_PLAIN, prob.model, prob.nx, prob.nu, prob.horizon):
+        return _lib.MODE_PLAIN
+    return _lib.MODE_FUSED  # the solve reports the unsupported size
 
 
 class _SQPSolveFn(torch.autograd.Function):
@@ -171,7 +182,7 @@ class _SQPSolveFn(torch.autograd.Function):
                             _lib.ptr(conv_err), _lib.ptr(conv_errs), _lib.ptr(ls_alphas), _lib.ptr(pcg_iters))
         prob = prog.c_problem(B)
         sp = _c_solver_params(solver.params)
-        mode = _mode(solver.params)
+        mode = _mode(solver.params, prob, lib, horizons is not None)
         nwork = _lib.workspace_bytes(mode, prob, sp, lib)
         work = torch.empty(max(nwork, 1), dtype=torch.uint8, device=dev)
         saved = _lib.Saved(_lib.ptr(AB), _lib.ptr(fd), _lib.ptr(qinv), _lib.ptr(rinv),

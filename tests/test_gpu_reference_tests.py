@@ -11,6 +11,7 @@ the GPU, each also checked against the float64 oracle:
 The problem set-ups follow the reference files (spacecraft.yaml, seeded
 numpy draws); trajectories within 1e-5 and gradients within 1e-4 of the
 oracle, besides the reference's own assertions."""
+import os
 import sys
 
 import numpy as np
@@ -318,3 +319,45 @@ def test_one_sqp_iteration_through_the_layer_matches_oracle(cuda):
     solb, *_ = solver.solve_one_scp_iteration(QPSolution(xb, ub, torch.stack([g.dual, g.dual])), p)
     torch.cuda.synchronize()
     assert torch.allclose(solb.states[1], sol.states, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("mode", ["auto", "plain"])
+def test_linear_model_of_any_size(cuda, mode):
+    """A linear model of a size the library itself is not built for (nx=6,
+    nu=3) solves as the reference solves any LinearDynamics: LinearDynamics
+    runs it in a custom build of the library (prebuilt in the package by
+    the build step), in the default mode (fused) and in the per-launch mode;
+    checked against the oracle."""
+    say(f"[reference tests] linear nx=6 nu=3, mode {mode}")
+    from diffmpc_amd import LinearDynamics
+
+    rng = np.random.default_rng(9)
+    nx, nu, N = 6, 3, 15
+    A = 0.1 * rng.normal(size=(nx, nx))
+    Bm = rng.normal(size=(nx, nu))
+    p = dict(horizon=N, discretization_resolution=0.1, discretization_scheme=0, penalize_control_reference=False,
+             initial_state=torch.tensor(rng.normal(size=nx)), final_state=torch.zeros(nx, dtype=torch.float64),
+             reference_state_trajectory=torch.zeros(N + 1, nx, dtype=torch.float64),
+             reference_control_trajectory=torch.zeros(N + 1, nu, dtype=torch.float64),
+             weights_penalization_reference_state_trajectory=torch.ones(nx, dtype=torch.float64),
+             weights_penalization_control_squared=0.1 * torch.ones(nu, dtype=torch.float64),
+             weights_penalization_final_state=torch.zeros(nx, dtype=torch.float64),
+             A=torch.tensor(A), B=torch.tensor(Bm), b=torch.zeros(nx, dtype=torch.float64))
+    dyn = LinearDynamics({"num_states": nx, "num_controls": nu, "names_states": [f"x{i}" for i in range(nx)],
+                          "names_controls": [f"u{i}" for i in range(nu)]})
+    sp = load_solver_params("sqp.yaml")
+    sp["pcg"]["tol_epsilon"] = 1e-14
+    sp["mode"] = mode
+    if not os.path.exists(os.path.join(dyn.prebuilt_dir(), "libdmpc_custom.so")):
+        pytest.skip("linear6x3 custom library not built (run __graft_entry__.build())")
+    solver = SQPSolver(OptimalControlProblem(dyn, p), params=sp)
+    sol = solver.solve(solver.initial_guess(device=cuda), p)
+    torch.cuda.synchronize()
+    assert int(sol.status) == SolverReturnStatus.SUCCESS
+    op = dict(model=odyn.make_model("linear", nx, nu), horizon=N, dt=0.1, scheme=0, penalize_control_reference=False,
+              x0=_np(p["initial_state"])[None], x_final=np.zeros((1, nx)), x_ref=np.zeros((1, N + 1, nx)),
+              u_ref=np.zeros((1, N + 1, nu)), w_x=np.ones((1, nx)), w_u=0.1 * np.ones((1, nu)), w_f=np.zeros((1, nx)),
+              dyn=np.concatenate([A.ravel(), Bm.ravel(), np.zeros(nx)])[None].astype(np.float32).astype(np.float64))
+    ref = osqp.sqp_solve(op, sp)
+    assert _rel(_np(sol.states), ref["states"][0]) <= 1e-5
+    assert _rel(_np(sol.controls), ref["controls"][0]) <= 1e-5
