@@ -7,13 +7,13 @@
 // horizon (about 64 blocks of 13 states, 128 of 7) and excludes nx > 16.
 // Here a workgroup still owns one instance and lane i of a G-lane group
 // still owns row i of a block, but the groups walk over the blocks in
-// rounds (block t = round * NG + group) and every per-block quantity lives
-// in global memory (L2 / MALL resident in practice):
-//   * [A_t | B_t] from the linearization (row-major) and its transpose AT,
-//     so that both A_t^T v (columns) and A_{t-1} v (rows) read coalesced;
-//   * D_t = theta_t^-1 in float64, transposed (DT[t][j][i] = D_t[i][j]);
+// rounds (block t = round * NG + group) and the per-block matrices live in
+// global memory:
+//   * [A_t | B_t] from the linearization (row-major), plus its transpose for
+//     nx > 16 (compact_blocks below);
+//   * D_t = theta_t^-1 in float64 (packed lower triangle for nx <= 16);
 //   * the PCG vectors (each lane owns its entries; neighbours are read after
-//     a workgroup barrier).
+//     a workgroup barrier), in LDS when they fit, else in the workspace.
 // The arithmetic and its order are those of the register-resident kernels
 // (pcg.py:350-445 with the factored block-Jacobi-stair preconditioner), so
 // both variants agree to rounding.  Assembly (theta_t, its Cholesky factor
@@ -33,9 +33,23 @@ enum { V_LAM = 0, V_P, V_R, V_RT, V_Y, V_YQ, V_YX, V_AP, V_YXO, V_GB, V_GAM };
 
 __host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Packed lower triangle of a symmetric nx x nx block: entry (r, c), r >= c.
+__host__ __device__ constexpr int tri(int r, int c) { return r * (r + 1) / 2 + c; }
+
+// Block layout by size.  Compact (nx <= 16): the kernels read A_t and B_t in
+// both orientations from the caller's [A|B] rows (a lane reads a column for
+// A^T v, its own row for A v: a row is a few cache lines that serve all its
+// loads, and phase C's two A reads of a block share them) and D_t as a packed
+// lower triangle, which cuts the bytes per PCG iteration by about a third.
+// For larger blocks a lane's row spans many lines per load, so the workspace
+// holds the transposed copy AT = [A|B]^T (every load coalesced) and D_t in
+// full, transposed (DT[t][j][i] = D_t[i][j]).
+__host__ __device__ constexpr bool compact_blocks(int nx) { return nx <= 16; }
+
 // Per-instance workspace: 11 padded vectors (N+3 blocks, zero blocks at
-// both ends), DT (N+1, nx, nx), 1/L_kk (N+1, nx), the line-search step and
-// scratch, and AT (N, nx+nu, nx) float.  Each array starts on 256 bytes.
+// both ends), D_t = theta_t^-1 (N+1 blocks, packed or full), 1/L_kk (N+1,
+// nx), the line-search step and scratch, and (large blocks) AT (N, nx+nu, nx)
+// float.  Each array starts on 256 bytes.
 struct Work {
   double* v[kStreamVectors];
   double *DT, *ID, *DX, *DU, *LS;
@@ -50,7 +64,7 @@ __host__ __device__ inline size_t carve_work(char* base, int nx, int nu, int N, 
     off += vbytes;
   }
   if (w) w->DT = reinterpret_cast<double*>(base + off);
-  off += align256((size_t)(N + 1) * nx * nx * 8);
+  off += align256((size_t)(N + 1) * (compact_blocks(nx) ? tri(nx, 0) : nx * nx) * 8);
   if (w) w->ID = reinterpret_cast<double*>(base + off);
   off += align256((size_t)(N + 1) * nx * 8);
   if (w) w->DX = reinterpret_cast<double*>(base + off);
@@ -60,7 +74,7 @@ __host__ __device__ inline size_t carve_work(char* base, int nx, int nu, int N, 
   if (w) w->LS = reinterpret_cast<double*>(base + off);
   off += align256((size_t)ls_doubles * 8);
   if (w) w->AT = reinterpret_cast<float*>(base + off);
-  off += align256((size_t)N * (nx + nu) * nx * 4);
+  if (!compact_blocks(nx)) off += align256((size_t)N * (nx + nu) * nx * 4);
   return off;
 }
 
@@ -69,6 +83,20 @@ template <int NX>
 __device__ __forceinline__ double vget(const double* V, int tb, int lane) {
   return lane < NX ? V[(tb + 1) * NX + lane] : 0.0;
 }
+
+// The first NV padded vectors (V_LAM, V_P, V_R, V_RT, V_Y, V_YQ, V_YX, then
+// V_AP, V_YXO, V_GB, V_GAM) in LDS instead of the workspace: every PCG phase
+// reads and writes them, and the neighbour exchanges go through them.  NV is
+// a template constant so that each access compiles to LDS or global
+// instructions, not generic ones.
+template <int NV, int NX>
+__device__ __forceinline__ void vectors_in_lds(Work* w, double* lds, int Np) {
+  const size_t n = (size_t)(Np + 3) * NX;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) w->v[k] = lds + k * n;
+}
+
+__host__ __device__ inline size_t vector_lds_bytes(int nv, int nx, int Np) { return (size_t)nv * (Np + 3) * nx * 8; }
 
 // Diagonal cost model in closed form (sqp.py:160-197,
 // optimal_control_problem.py:249-323) for one instance.
@@ -98,14 +126,16 @@ __device__ __forceinline__ double gdot_mem(const AT* a, int stride, double v, in
   return gdot_chain<G, M>(r, v, lane, 0, 0.0);
 }
 
-// Group sum of b_k * v for the NU columns k of B_{t-1} (rows in AT), and the
-// B_{t-1} Rinv B_{t-1}^T v product (see bgram in the register variant).
+// Group sum of b_k * v for the NU columns k of B_{t-1} (brow: the lane's
+// entry of column 0, the next at stride `stride`), and the B_{t-1} Rinv
+// B_{t-1}^T v product (see bgram in the register variant).
 template <int G, int NX, int NU>
-__device__ __forceinline__ double bgram_mem(const float* brow, const CostView<NX, NU>& c, double v, double* quad) {
+__device__ __forceinline__ double bgram_mem(const float* brow, int stride, const CostView<NX, NU>& c, double v,
+                                            double* quad) {
   double g = 0.0, q = 0.0;
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
-    const double bk = brow[k * NX];
+    const double bk = brow[k * stride];
     const double sk = gsum<G>(bk * v);
     const double yk = c.rinv(k) * sk;
     g += bk * yk;
@@ -119,8 +149,9 @@ __device__ __forceinline__ double bgram_mem(const float* brow, const CostView<NX
 // Assembly: one workgroup of 64 threads per (instance, block t).  Thread i
 // owns row i.  theta_t (pcg.py:182-196 with As_next = -I), its Cholesky
 // factor (right-looking, the operation order of the register variant's
-// shuffle version) and D_t = theta_t^-1 by one column per thread, written
-// transposed to DT.  It also writes AT[t-1] = [A_{t-1} | B_{t-1}]^T.
+// shuffle version) and D_t = theta_t^-1 by one column per thread, written to
+// DT (packed: thread i writes rows j >= i of column i; full: transposed).
+// Large blocks also get AT[t-1] = [A_{t-1} | B_{t-1}]^T.
 struct AssembleArgs {
   int B, N;  // padded horizon
   const int32_t* horizons;
@@ -169,8 +200,10 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
     for (int e = i; e < NX * NZ; e += 64) {
       const float v = Ap[e];
       AP[e] = v;
-      const int r = e / NZ, k = e - r * NZ;
-      ATp[k * NX + r] = v;
+      if constexpr (!compact_blocks(NX)) {
+        const int r = e / NZ, k = e - r * NZ;
+        ATp[k * NX + r] = v;
+      }
     }
   }
   __syncthreads();
@@ -218,7 +251,8 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
   // column i of theta^-1 = L^-T L^-1 e_i (forward then back substitution;
   // the back substitution overwrites the forward result in place)
   if (i < NX) {
-    double* DTt = w.DT + (size_t)t * NX * NX;
+    constexpr bool kPacked = compact_blocks(NX);
+    double* DTt = w.DT + (size_t)t * (kPacked ? tri(NX, 0) : NX * NX);
     if constexpr (kColsInLds) {
       double* v = COL + i;  // v[j * 64]
       for (int j = 0; j < NX; ++j) {
@@ -231,7 +265,10 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
         for (int k = j + 1; k < NX; ++k) acc -= TH[k * (NX + 1) + j] * v[k * 64];
         v[j * 64] = acc * IDs[j];
       }
-      for (int j = 0; j < NX; ++j) DTt[j * NX + i] = v[j * 64];
+      for (int j = 0; j < NX; ++j) {
+        if (!kPacked) DTt[j * NX + i] = v[j * 64];
+        else if (j >= i) DTt[tri(j, i)] = v[j * 64];
+      }
     } else {
       double v[NX], d[NX];
 #pragma unroll
@@ -249,7 +286,10 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
         d[j] = acc * IDs[j];
       }
 #pragma unroll
-      for (int j = 0; j < NX; ++j) DTt[j * NX + i] = d[j];
+      for (int j = 0; j < NX; ++j) {
+        if (!kPacked) DTt[j * NX + i] = d[j];
+        else if (j >= i) DTt[tri(j, i)] = d[j];
+      }
     }
   }
 }
@@ -257,10 +297,33 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
 // D_t v: sum_j D_t[lane][j] v_j (v_j in lane j of the group), 0 when !ok.
 template <int G, int NX>
 __device__ __forceinline__ double dmul_mem(const Work& w, int t, double v, int lane, bool ok) {
-  const double* col = w.DT + (size_t)t * NX * NX + (lane < NX ? lane : 0);
-  const double acc = gdot_mem<G, NX>(col, NX, v, lane);
+  const int l = lane < NX ? lane : 0;
+  double acc;
+  if constexpr (compact_blocks(NX)) {
+    const double* D = w.DT + (size_t)t * tri(NX, 0);
+    const int lrow = tri(l, 0);
+    double r[NX];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) r[j] = D[l >= j ? lrow + j : tri(j, 0) + l];
+    acc = gdot_chain<G, NX>(r, v, lane, 0, 0.0);
+  } else {
+    acc = gdot_mem<G, NX>(w.DT + (size_t)t * NX * NX + l, NX, v, lane);
+  }
   return ok ? acc : 0.0;
 }
+
+// The lane's entry of column 0 of A_{t-1} (row `li`) and the stride to the
+// next column, in whichever copy the block layout reads; B_{t-1} follows at
+// NX strides.
+template <int NX, int NU>
+__device__ __forceinline__ const float* arow_mem(const float* ABb, const float* AT, int t, int li) {
+  constexpr int NZ = NX + NU;
+  const size_t tp = (size_t)(t >= 1 ? t - 1 : 0);
+  if constexpr (compact_blocks(NX)) return ABb + tp * NX * NZ + li * NZ;
+  else return AT + tp * NZ * NX + li;
+}
+template <int NX>
+__device__ __forceinline__ constexpr int arow_stride() { return compact_blocks(NX) ? 1 : NX; }
 
 // Per-instance setup shared by the forward and the backward kernels.
 template <int G, int NX, int NU>
@@ -302,7 +365,8 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   const int lane = K.lane, N = K.N;
   const int li = lane < NX ? lane : 0;
   auto acol = [&](int t) { return ABb + (size_t)(t < N ? t : 0) * NX * NZ + li; };        // stride NZ
-  auto arow = [&](int t) { return w.AT + (size_t)(t >= 1 ? t - 1 : 0) * NZ * NX + li; };  // stride NX
+  auto arow = [&](int t) { return arow_mem<NX, NU>(ABb, w.AT, t, li); };
+  constexpr int AS = arow_stride<NX>();
   auto has_a = [&](int t) { return t < N; };
   auto has_p = [&](int t) { return t >= 1; };
   // ---- r = gamma - S lam0 ----------------------------------------------
@@ -314,7 +378,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     const double pn = vget<NX>(LAM, t + 1, lane);
     const double ax = gdot_mem<G, NX>(acol(t), NZ, pn, lane);
     double q;
-    const double gb = bgram_mem<G, NX, NU>(arow(t) + NX * NX, c, pv, &q);
+    const double gb = bgram_mem<G, NX, NU>(arow(t) + NX * AS, AS, c, pv, &q);
     const double wx = ((t == 0) ? pv : -pv) + ((ok && has_a(t)) ? ax : 0.0);
     const double yxo = qi * wx;
     if (ok) {
@@ -330,7 +394,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     const int t = K.t(r);
     const bool ok = K.ok(r);
     const double u = vget<NX>(YX, t - 1, lane);
-    const double au = gdot_mem<G, NX>(arow(t), NX, u, lane);
+    const double au = gdot_mem<G, NX>(arow(t), AS, u, lane);
     const double g = (t == 0 ? 1.0 : -1.0) * vget<NX>(YXO, t, lane) + ((ok && has_p(t)) ? au : 0.0) +
                      vget<NX>(GB, t, lane);
     const double rv = ok ? vget<NX>(GAM, t, lane) + g : 0.0;
@@ -346,7 +410,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   for (int r = 0; r < K.rounds; ++r) {
     const int t = K.t(r);
     const bool ok = K.ok(r);
-    const double a1v = gdot_mem<G, NX>(arow(t), NX, vget<NX>(YQ, t - 1, lane), lane);
+    const double a1v = gdot_mem<G, NX>(arow(t), AS, vget<NX>(YQ, t - 1, lane), lane);
     const double a2v = gdot_mem<G, NX>(acol(t), NZ, vget<NX>(Y, t + 1, lane), lane);
     const double a1 = (ok && has_p(t)) ? a1v : 0.0, a2 = (ok && has_a(t)) ? a2v : 0.0;
     const double rv = vget<NX>(R, t, lane);
@@ -374,7 +438,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
       const double arn = gdot_mem<G, NX>(acol(t), NZ, rn, lane);
       const double apv = ok ? ((has_a(t) ? arn : 0.0) + beta * vget<NX>(AP, t, lane)) : 0.0;
       double q;
-      const double gb = bgram_mem<G, NX, NU>(arow(t) + NX * NX, c, pv, &q);
+      const double gb = bgram_mem<G, NX, NU>(arow(t) + NX * AS, AS, c, pv, &q);
       const double qi = ok ? c.qinv(t, lane) : 0.0;
       const double wx = ((t == 0) ? pv : -pv) + apv;
       const double yxo = qi * wx;
@@ -395,7 +459,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
       const int t = K.t(r);
       const bool ok = K.ok(r);
       const double u = vget<NX>(YX, t - 1, lane);
-      const double au = gdot_mem<G, NX>(arow(t), NX, u, lane);
+      const double au = gdot_mem<G, NX>(arow(t), AS, u, lane);
       const double g = (t == 0 ? 1.0 : -1.0) * vget<NX>(YXO, t, lane) + ((ok && has_p(t)) ? au : 0.0) +
                        vget<NX>(GB, t, lane);
       double rv = vget<NX>(R, t, lane);
@@ -416,7 +480,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     for (int r = 0; r < K.rounds; ++r) {
       const int t = K.t(r);
       const bool ok = K.ok(r);
-      const double a1v = gdot_mem<G, NX>(arow(t), NX, vget<NX>(YQ, t - 1, lane), lane);
+      const double a1v = gdot_mem<G, NX>(arow(t), AS, vget<NX>(YQ, t - 1, lane), lane);
       const double a2v = gdot_mem<G, NX>(acol(t), NZ, vget<NX>(Y, t + 1, lane), lane);
       const double a1 = (ok && has_p(t)) ? a1v : 0.0, a2 = (ok && has_a(t)) ? a2v : 0.0;
       const double rv = vget<NX>(R, t, lane);
@@ -451,11 +515,12 @@ __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r
   const double lt = vget<NX>(LAM, t, lane);
   const double ln = vget<NX>(LAM, t + 1, lane);
   const double ax = gdot_mem<G, NX>(ABb + (size_t)(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
-  const float* brow = w.AT + (size_t)(t >= 1 ? t - 1 : 0) * NZ * NX + NX * NX + li;
+  constexpr int AS = arow_stride<NX>();
+  const float* brow = arow_mem<NX, NU>(ABb, w.AT, t, li) + NX * AS;
   double bl = 0.0;
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
-    const double sk = gsum<G>((double)brow[k * NX] * lt);
+    const double sk = gsum<G>((double)brow[k * AS] * lt);
     bl = (lane == k) ? sk : bl;
   }
   *xn = 0.0;
@@ -483,10 +548,11 @@ __device__ void stream_init_vectors(const Work& w, int Np, int N, const double* 
 // One SQP iteration of one instance (streamed): gamma, PCG, recovery, line
 // search, step test and outputs.  The block matrices were assembled by
 // stream_assemble_kernel for this linearization.
-template <class Model, int G>
+template <class Model, int G, int NV>
 __global__ void __launch_bounds__(512) stream_qp_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
+  extern __shared__ double stream_vlds[];
   const int b = blockIdx.x;
   if (!sqp_active(a, b)) return;
   const int it = a.num_iter[b];
@@ -495,6 +561,7 @@ __global__ void __launch_bounds__(512) stream_qp_kernel(SolveArgs a) {
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, Np);
   Work w;
   carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, lsd, &w);
+  vectors_in_lds<NV, NX>(&w, stream_vlds, Np);
   Blocks<G, NX, NU> K;
   K.init(N);
   CostView<NX, NU> c{a.wx + (size_t)b * NX, a.wu + (size_t)b * NU, a.wf + (size_t)b * NX,
@@ -578,15 +645,17 @@ __global__ void __launch_bounds__(512) stream_qp_kernel(SolveArgs a) {
 // Implicit-differentiation backward (sqp.py:605-798), streamed: adjoint PCG
 // with q = -dl/dx, r = -dl/du[:-1], C = 0 on the saved linearization, then
 // the closed-form contraction of the register variant's backward.
-template <class Model, int G>
+template <class Model, int G, int NV>
 __global__ void __launch_bounds__(512) stream_bwd_kernel(BackwardArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
+  extern __shared__ double stream_vlds[];
   const int b = blockIdx.x;
   const int Np = a.N, NBs = Np + 1;
   const int N = horizon_of(a.horizons, Np, b), NB = N + 1;
   Work w;
   carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, 0, &w);
+  vectors_in_lds<NV, NX>(&w, stream_vlds, Np);
   Blocks<G, NX, NU> K;
   K.init(N);
   // cost view of the adjoint system: q~ = -dl/dx and r~ = -dl/du replace
@@ -628,11 +697,12 @@ __global__ void __launch_bounds__(512) stream_bwd_kernel(BackwardArgs a) {
     const double lt = vget<NX>(LAM, t, lane);
     const double ln = vget<NX>(LAM, t + 1, lane);
     const double ax = gdot_mem<G, NX>(ABb + (size_t)(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
-    const float* brow = w.AT + (size_t)(t >= 1 ? t - 1 : 0) * NZ * NX + NX * NX + li;
+    constexpr int AS = arow_stride<NX>();
+    const float* brow = arow_mem<NX, NU>(ABb, w.AT, t, li) + NX * AS;
     double bl = 0.0;
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
-      const double sk = gsum<G>((double)brow[k * NX] * lt);
+      const double sk = gsum<G>((double)brow[k * AS] * lt);
       bl = (lane == k) ? sk : bl;
     }
     if (!ok) continue;
@@ -724,12 +794,43 @@ int stream_launch_linearize(int model, const SolveArgs& a, hipStream_t st);
 int stream_launch_reset(const SolveArgs& a, hipStream_t st);
 int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st);
 
+// How many PCG vectors the streamed kernels keep in LDS for padded horizon N:
+// all of them, or the seven of the PCG phases, when they fit next to the
+// kernels' static LDS; else none (the workspace holds them).  The first call
+// per kernel raises its dynamic-LDS limit.
+constexpr int kHotVectors = 7;
+constexpr size_t kStreamLdsLimit = 160 * 1024 - 1024;
+
+template <class Model, int G>
+int lds_vectors(int N, bool forward) {
+  constexpr int NX = Model::NX;
+  static bool attr[2] = {false, false};
+  int nv = 0;
+  // the recovery writes the (N+1) x nu controls into V_AP, which may then
+  // run into the following vectors: all eleven in LDS only when nu <= nx
+  if (Model::NU <= NX && vector_lds_bytes(kStreamVectors, NX, N) <= kStreamLdsLimit) nv = kStreamVectors;
+  else if (vector_lds_bytes(kHotVectors, NX, N) <= kStreamLdsLimit) nv = kHotVectors;
+  if (nv && !attr[forward]) {
+    const void* k1 = forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kStreamVectors>)
+                             : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kStreamVectors>);
+    const void* k2 = forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors>)
+                             : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kHotVectors>);
+    if (hipFuncSetAttribute(k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess ||
+        hipFuncSetAttribute(k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess)
+      return 0;
+    attr[forward] = true;
+  }
+  return nv;
+}
+
 template <class Model, int G>
 int stream_forward_t(int model, const SolveArgs& a, hipStream_t st) {
   constexpr int NX = Model::NX, NU = Model::NU;
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
   const AssembleArgs aa = assemble_args<Model>(a, lsd);
   const int threads = stream_threads(a.N, G);
+  const int nv = lds_vectors<Model, G>(a.N, true);
+  const size_t lds = vector_lds_bytes(nv, NX, a.N);
   int rc = stream_launch_reset(a, st);
   if (rc) return rc;
   for (int it = 0; it < a.maxit; ++it) {
@@ -738,7 +839,12 @@ int stream_forward_t(int model, const SolveArgs& a, hipStream_t st) {
     prof_begin(PROF_QP_ITERATION, st);
     hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0,
                        st, aa);
-    hipLaunchKernelGGL((stream_qp_kernel<Model, G>), dim3(a.B), dim3(threads), 0, st, a);
+    if (nv == kStreamVectors)
+      hipLaunchKernelGGL((stream_qp_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(threads), lds, st, a);
+    else if (nv == kHotVectors)
+      hipLaunchKernelGGL((stream_qp_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(threads), lds, st, a);
+    else
+      hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0>), dim3(a.B), dim3(threads), 0, st, a);
     prof_end(PROF_QP_ITERATION, st);
   }
   rc = stream_launch_finish(model, a, st);
@@ -767,7 +873,15 @@ int stream_backward_t(const BackwardArgs& a, hipStream_t st) {
   prof_begin(PROF_BACKWARD, st);
   hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0, st,
                      aa);
-  hipLaunchKernelGGL((stream_bwd_kernel<Model, G>), dim3(a.B), dim3(stream_threads(a.N, G)), 0, st, a);
+  const int nv = lds_vectors<Model, G>(a.N, false);
+  const size_t lds = vector_lds_bytes(nv, NX, a.N);
+  if (nv == kStreamVectors)
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(stream_threads(a.N, G)), lds, st,
+                       a);
+  else if (nv == kHotVectors)
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(stream_threads(a.N, G)), lds, st, a);
+  else
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0>), dim3(a.B), dim3(stream_threads(a.N, G)), 0, st, a);
   prof_end(PROF_BACKWARD, st);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
 }

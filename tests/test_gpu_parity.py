@@ -453,14 +453,17 @@ def test_ragged_batch_matches_oracle_per_instance(cuda, variant):
             assert rel(g0[b : b + 1], g["x0"]) <= 1e-4
 
 
-def test_long_horizon_runs_streamed_and_matches_oracle(cuda):
-    """Quadrotor with N = 150: too long for the register-resident kernels, so
-    the automatic choice is the streamed variant."""
-    say("[long] quadrotor N=150")
+@pytest.mark.parametrize("horizon", [150, 240])
+def test_long_horizon_runs_streamed_and_matches_oracle(cuda, horizon):
+    """Quadrotor with N = 150 and 240: too long for the register-resident
+    kernels, so the automatic choice is the streamed variant; at 150 its seven
+    PCG-phase vectors sit in LDS, at 240 they do not fit and stay in the
+    workspace."""
+    say(f"[long] quadrotor N={horizon}")
     from diffmpc_amd import _lib
 
-    assert _lib.load().dmpc_supported(0, 1, 13, 4, 150) == 1
-    name, prob, sp = W.quadrotor(batch=2, seed=9, horizon=150, sqp_iters=3)
+    assert _lib.load().dmpc_supported(0, 1, 13, 4, horizon) == 1
+    name, prob, sp = W.quadrotor(batch=2, seed=9, horizon=horizon, sqp_iters=3)
     sol, gwx, _ = _solve_and_grad(name, prob, sp, cuda)
     op = oracle_problem(name, prob)
     ref = osqp.sqp_solve(op, sp)
