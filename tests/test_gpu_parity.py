@@ -473,12 +473,13 @@ def test_long_horizon_runs_streamed_and_matches_oracle(cuda, horizon):
     assert rel(_np(gwx), g["w_x"]) <= 1e-4
 
 
-@pytest.mark.parametrize("mode", ["fused", "plain"])
-def test_chain_matches_oracle(cuda, mode):
+@pytest.mark.parametrize("mode,horizon", [("fused", 24), ("plain", 24), ("fused", 64)])
+def test_chain_matches_oracle(cuda, mode, horizon):
     """48-state spring-damper chain (linear, one SQP iteration is the QP
-    solution): fused runs the streamed variant (64-lane groups)."""
-    say(f"[chain] mode={mode}")
-    name, prob, sp = W.chain(batch=2, seed=4, horizon=24)
+    solution): fused runs the streamed variant (64-lane groups), with its
+    PCG vectors in LDS at N = 24 and in the workspace at N = 64."""
+    say(f"[chain] mode={mode} N={horizon}")
+    name, prob, sp = W.chain(batch=2, seed=4, horizon=horizon)
     solver, params, guess = W.make_solver(name, prob, sp, device=cuda, mode=mode)
     wx = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
     sol = solver.solve(guess, params, {"weights_penalization_reference_state_trajectory": wx})
