@@ -114,8 +114,9 @@ typedef struct dmpc_saved { /* QP data of the last linearization point (sqp.py:4
   double* qinv; /* (B, N+1, nx): diagonal of Qinv_t */
   double* rinv; /* (B, nu): diagonal of Rinv_t (the same for every t) */
   /* Device workspace of dmpc_workspace_bytes() bytes (NULL when that is 0):
-   * the streamed fused variant keeps its vectors and preconditioner blocks
-   * there.  Contents need not survive between calls. */
+   * the streamed fused variant keeps its preconditioner blocks there, and
+   * the PCG vectors that do not fit in LDS.  Contents need not survive
+   * between calls. */
   void* work;
   uint64_t work_bytes;
 } dmpc_saved;
