@@ -338,6 +338,34 @@ def test_headline_workload_full_batch_matches_oracle_on_sampled_instances(cuda):
         assert e <= 1e-4, short
 
 
+@pytest.mark.parametrize("batch,horizon", [(1024, 50), (512, 150)])
+def test_full_batch_results_do_not_depend_on_instance_order(cuda, batch, horizon):
+    """Size-independent property at full size: instances are independent, so
+    solving a permuted batch permutes every per-instance output bit for bit
+    (trajectories, multipliers, PCG counts, gradients).  N = 50 runs the
+    register-resident kernels, N = 150 the streamed variant."""
+    say(f"[permute] quadrotor B={batch} N={horizon}")
+    name, prob, sp = W.quadrotor(batch=batch, seed=1, horizon=horizon, sqp_iters=3)
+    perm = np.random.default_rng(0).permutation(batch)
+    permuted = {k: (v[perm] if isinstance(v, np.ndarray) and v.ndim and v.shape[0] == batch else v)
+                for k, v in prob.items()}
+    key = "weights_penalization_reference_state_trajectory"
+
+    def run(p):
+        solver, params, guess = W.make_solver(name, p, sp, device=cuda)
+        assert params[key].shape[0] == batch
+        wx = params[key].clone().requires_grad_(True)
+        sol = solver.solve(guess, params, {key: wx})
+        loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+        (g,) = torch.autograd.grad(loss, [wx])
+        torch.cuda.synchronize()
+        return [_np(sol.states), _np(sol.controls), _np(sol.dual), _np(sol.num_pcg_iters), _np(g)]
+
+    first, second = run(prob), run(permuted)
+    for a, b in zip(first, second):
+        assert np.array_equal(a[perm], b)
+
+
 def test_spacecraft_attitude_full_batch_gradients_match_oracle_on_sampled_instances(cuda):
     """Spacecraft attitude nx=7 nu=3 N=100, B=4096, 3 SQP iterations: gradients
     of 0.5 sum ||x* - x_ref||^2 w.r.t. log weights (per instance here, so that
