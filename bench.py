@@ -48,6 +48,9 @@ def main():
                     help="time only the fused path (for profiler runs); prints the fused figures")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB mebibytes device-to-device once (reference kernel for HBM counters)")
+    ap.add_argument("--blocks-per-group", type=int, default=0, metavar="K",
+                    help="force K time blocks per lane group in the register-resident kernels "
+                         "(1, 2 or 4; 0 = automatic) for every solve of the run")
     ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain", "rollout", "dynparams", "imitation"],
                     default="quadrotor",
                     help="quadrotor: the headline line (default); ragged: mixed batch of 8192 with horizons "
@@ -78,6 +81,8 @@ def main():
     from diffmpc_amd import _lib
     from diffmpc_amd import workloads as W
 
+    if args.blocks_per_group:
+        _lib.check(_lib.load().dmpc_set_blocks_per_group(args.blocks_per_group), "dmpc_set_blocks_per_group")
     if args.workload == "ragged":
         return bench_ragged(args)
     if args.workload == "chain":
