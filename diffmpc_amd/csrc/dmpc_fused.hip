@@ -990,8 +990,12 @@ bool use_stream(int model, int nx, int nu, int N, int lsd, int variant) {
 
 int g_fused_variant = 0;
 
+// The fused kernels give each time block one lane group of G >= nx lanes,
+// lane i owning state row i; control rows are recovered by lanes i < nu of
+// the block's group and staged in the per-block vector buffers, so they
+// need nu <= nx.  Models with more controls than states run in the plain mode.
 bool fused_supported(int model, int nx, int nu, int N) {
-  if (N < 1) return false;
+  if (N < 1 || nu > nx) return false;
   return register_fits(model, nx, nu, N, 0) || stream_supported(model, nx, nu);
 }
 
@@ -1048,6 +1052,7 @@ int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st) {
 }
 
 int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st) {
+  if (nu > nx) return DMPC_ERR_UNSUPPORTED;  // see fused_supported
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
   if (use_stream(model, nx, nu, a.N, lsd, a.variant)) return stream_forward(model, nx, nu, a, st);
   if (!register_fits(model, nx, nu, a.N, lsd)) return DMPC_ERR_UNSUPPORTED;
@@ -1069,6 +1074,7 @@ int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st)
 }
 
 int fused_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t st) {
+  if (nu > nx) return DMPC_ERR_UNSUPPORTED;  // see fused_supported
   if (use_stream(model, nx, nu, a.N, 0, a.variant)) return stream_backward(model, nx, nu, a, st);
   if (!register_fits(model, nx, nu, a.N, 0)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {

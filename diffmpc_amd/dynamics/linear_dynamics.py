@@ -29,7 +29,11 @@ class LinearDynamics(CustomBuild, Dynamics):
     Any (nx, nu), as in the reference: the sizes in BUILT_SIZES run in the
     library itself; any other size (nx <= 64) is the same model template
     compiled into a custom build of the library (model id MODEL_CUSTOM,
-    compiled on first use and cached, or prebuilt in the package)."""
+    compiled on first use and cached, or prebuilt in the package).  Sizes
+    with more controls than states (nu > nx) run in the plain (per-launch)
+    mode only: the fused kernels recover control rows on the lanes of a
+    state block, so mode "auto" picks plain for them and mode "fused"
+    raises DmpcError (DMPC_ERR_UNSUPPORTED)."""
 
     model_id = MODEL_LINEAR
     param_keys = ("A", "B", "b")

@@ -124,6 +124,28 @@ def _ragged_horizons(h, B, N, device):
     return h.to(device=device, dtype=torch.int32).contiguous()
 
 
+def _check_shapes(B, N, nx, nu, maxit, n_params, guess, data):
+    """Every array the kernels index with fixed strides must have exactly the
+    shape of the problem: a guess, dual or history left over from a solver
+    with another horizon, size or iteration cap raises here instead of
+    reaching the device (where the reference raises a shape error)."""
+    want = {
+        "states": (B, N + 1, nx), "controls": (B, N + 1, nu), "dual": (B, N + 1, nx),
+        "convergence_error": (B,), "convergence_errors": (B, maxit), "linesearch_alphas": (B, maxit),
+        "num_pcg_iters": (B, maxit), "dual_backward_guess": (B, N + 1, nx),
+        "initial_state": (B, nx), "final_state": (B, nx), "reference_state_trajectory": (B, N + 1, nx),
+        "reference_control_trajectory": (B, N + 1, nu),
+        "weights_penalization_reference_state_trajectory": (B, nx),
+        "weights_penalization_control_squared": (B, nu), "weights_penalization_final_state": (B, nx),
+        "dynamics parameters": (B, n_params),
+    }
+    for src in (guess, data):
+        for k, t in src.items():
+            if t is not None and tuple(t.shape) != want[k]:
+                raise ValueError(f"{k} has shape {tuple(t.shape)}, expected {want[k]} "
+                                 f"(B={B}, N={N}, nx={nx}, nu={nu}, num_scp_iteration_max={maxit})")
+
+
 def _mode(params, prob, lib, ragged: bool) -> int:
     """The execution mode of a solve: params["mode"] "fused" or "plain", or
     "auto" (the default): the fused kernels where the library builds them for
@@ -395,7 +417,8 @@ class SQPSolver:
             if t.dim() == 0 or t.numel() == 0:
                 return torch.zeros(B, self.maxiter, dtype=dtype, device=device)
             t = t.reshape(-1, t.shape[-1]) if t.dim() >= 1 else t
-            return t.expand(B, self.maxiter).to(dtype) if t.shape[0] == 1 else t.to(dtype)
+            # one shared history row is expanded; any other shape is checked below
+            return t.expand(B, self.maxiter).to(dtype) if tuple(t.shape) == (1, self.maxiter) else t.to(dtype)
 
         cev = initial_guess.convergence_error
         if isinstance(cev, torch.Tensor) and cev.is_cuda:
@@ -440,6 +463,14 @@ class SQPSolver:
             dyn = prog.dynamics.flat_params({k: (v.detach() if isinstance(v, torch.Tensor) and k not in weights
                                                  else v) for k, v in params.items()}, B, device)
         horizons = _ragged_horizons(params.get("horizons"), B, N, device)
+        _check_shapes(B, N, nx, nu, self.maxiter, prog.dynamics.num_params, g,
+                      {"initial_state": x0, "final_state": xf, "reference_state_trajectory": xref,
+                       "reference_control_trajectory": uref,
+                       "weights_penalization_reference_state_trajectory": wx,
+                       "weights_penalization_control_squared": wu, "weights_penalization_final_state": wf,
+                       "dynamics parameters": dyn})
+        if torch.device(device).type != "cuda":
+            raise RuntimeError(f"SQPSolver.solve runs on the GPU (libdmpc.so); got device {device}")
         out = _SQPSolveFn.apply(self, g, x0, xf, xref, uref, wx, wu, wf, dyn, horizons)
         (x, u, lam, status, num_iter, conv_err, conv_errs, ls_alphas, pcg_iters, AB, fd, qinv, rinv,
          dual_bwd_out) = out

@@ -19,6 +19,7 @@ import pytest
 import torch
 
 from diffmpc_amd import OptimalControlProblem, SpacecraftDynamics, SQPSolver, load_problem_params, load_solver_params
+from diffmpc_amd import _lib
 from diffmpc_amd.solvers.sqp import SolverReturnStatus
 from oracle import dynamics as odyn
 from oracle import sqp as osqp
@@ -321,18 +322,20 @@ def test_one_sqp_iteration_through_the_layer_matches_oracle(cuda):
     assert torch.allclose(solb.states[1], sol.states, rtol=1e-12, atol=1e-14)
 
 
-@pytest.mark.parametrize("mode", ["auto", "plain"])
-def test_linear_model_of_any_size(cuda, mode):
-    """A linear model of a size the library itself is not built for (nx=6,
-    nu=3) solves as the reference solves any LinearDynamics: LinearDynamics
-    runs it in a custom build of the library (prebuilt in the package by
-    the build step), in the default mode (fused) and in the per-launch mode;
-    checked against the oracle."""
-    say(f"[reference tests] linear nx=6 nu=3, mode {mode}")
+@pytest.mark.parametrize("nx,nu,mode", [(6, 3, "auto"), (6, 3, "plain"), (2, 3, "auto"), (2, 3, "plain"),
+                                         (2, 3, "fused")])
+def test_linear_model_of_any_size(cuda, nx, nu, mode):
+    """A linear model of a size the library itself is not built for solves as
+    the reference solves any LinearDynamics: LinearDynamics runs it in a
+    custom build of the library (prebuilt in the package by the build step).
+    nx=6, nu=3 runs fused by default; nx=2, nu=3 (more controls than states)
+    runs in the per-launch mode, and asking for the fused mode raises.
+    Forward and backward (w_x, w_u, x0) are checked against the oracle."""
+    say(f"[reference tests] linear nx={nx} nu={nu}, mode {mode}")
     from diffmpc_amd import LinearDynamics
 
-    rng = np.random.default_rng(9)
-    nx, nu, N = 6, 3, 15
+    rng = np.random.default_rng(9 + nx)
+    N = 15
     A = 0.1 * rng.normal(size=(nx, nx))
     Bm = rng.normal(size=(nx, nu))
     p = dict(horizon=N, discretization_resolution=0.1, discretization_scheme=0, penalize_control_reference=False,
@@ -349,9 +352,18 @@ def test_linear_model_of_any_size(cuda, mode):
     sp["pcg"]["tol_epsilon"] = 1e-14
     sp["mode"] = mode
     if not os.path.exists(os.path.join(dyn.prebuilt_dir(), "libdmpc_custom.so")):
-        pytest.skip("linear6x3 custom library not built (run __graft_entry__.build())")
+        pytest.skip(f"linear{nx}x{nu} custom library not built (run __graft_entry__.build())")
     solver = SQPSolver(OptimalControlProblem(dyn, p), params=sp)
-    sol = solver.solve(solver.initial_guess(device=cuda), p)
+    wx = p["weights_penalization_reference_state_trajectory"].to(cuda).requires_grad_(True)
+    wu = p["weights_penalization_control_squared"].to(cuda).requires_grad_(True)
+    x0 = p["initial_state"].to(cuda).requires_grad_(True)
+    weights = {"weights_penalization_reference_state_trajectory": wx,
+               "weights_penalization_control_squared": wu, "initial_state": x0}
+    if mode == "fused" and nu > nx:
+        with pytest.raises(_lib.DmpcError):
+            solver.solve(solver.initial_guess(device=cuda), p, weights)
+        return
+    sol = solver.solve(solver.initial_guess(device=cuda), p, weights)
     torch.cuda.synchronize()
     assert int(sol.status) == SolverReturnStatus.SUCCESS
     op = dict(model=odyn.make_model("linear", nx, nu), horizon=N, dt=0.1, scheme=0, penalize_control_reference=False,
@@ -361,3 +373,10 @@ def test_linear_model_of_any_size(cuda, mode):
     ref = osqp.sqp_solve(op, sp)
     assert _rel(_np(sol.states), ref["states"][0]) <= 1e-5
     assert _rel(_np(sol.controls), ref["controls"][0]) <= 1e-5
+    # loss 1/2 |x|^2 + 1/2 |u|^2: every control row, including those past nx, feeds the gradient
+    loss = 0.5 * sol.states.square().sum() + 0.5 * sol.controls.square().sum()
+    gwx, gwu, gx0 = torch.autograd.grad(loss, [wx, wu, x0])
+    gref = osqp.sqp_backward(op, sp, ref, ref["states"], ref["controls"])
+    assert _rel(_np(gwx), gref["w_x"][0]) <= 1e-4
+    assert _rel(_np(gwu), gref["w_u"][0]) <= 1e-4
+    assert _rel(_np(gx0), gref["x0"][0]) <= 1e-4

@@ -192,3 +192,41 @@ def test_cost_terms_match_oracle_and_the_cost():
     assert np.isclose(total.item(), prob.cost(xt, ut, p).item(), rtol=1e-13)
     d = SQPSolver.state_control_difference(xt, ut, xt + 0.5, ut - 2.0)
     assert d.item() == 2.0
+
+
+@pytest.mark.parametrize("field,shape", [
+    ("dual", (1, 25, 13)), ("convergence_errors", (1, 7)), ("num_pcg_iters", (2, 5)),
+    ("dual_backward_guess", (1, 26, 12)), ("states", (1, 26, 12)),
+])
+def test_solve_rejects_guess_arrays_of_another_shape(field, shape):
+    """A guess, history or dual left over from a solver with another horizon,
+    size or iteration cap raises ValueError before anything reaches the
+    device (the kernels index them with fixed strides)."""
+    p = load_problem_params("quadrotor.yaml")
+    sp = load_solver_params("sqp.yaml")
+    sp["num_scp_iteration_max"] = 5
+    solver = SQPSolver(OptimalControlProblem(QuadrotorDynamics(), p), params=sp)
+    g = solver.initial_guess(device="cpu")
+    g = g._replace(**{field: torch.zeros(shape, dtype=torch.int32 if field == "num_pcg_iters" else torch.float64)})
+    with pytest.raises(ValueError, match=field):
+        solver.solve(g, p, device="cpu")
+
+
+@pytest.mark.parametrize("key,shape", [
+    ("reference_state_trajectory", (25, 13)), ("weights_penalization_control_squared", (3,)),
+    ("initial_state", (2, 13)), ("final_state", (12,)),
+])
+def test_solve_rejects_parameters_of_another_shape(key, shape):
+    p = load_problem_params("quadrotor.yaml")
+    solver = SQPSolver(OptimalControlProblem(QuadrotorDynamics(), p), params=load_solver_params("sqp.yaml"))
+    bad = dict(p)
+    bad[key] = torch.zeros(shape, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        solver.solve(solver.initial_guess(device="cpu"), bad, device="cpu")
+
+
+def test_solve_requires_a_gpu_device():
+    p = load_problem_params("quadrotor.yaml")
+    solver = SQPSolver(OptimalControlProblem(QuadrotorDynamics(), p), params=load_solver_params("sqp.yaml"))
+    with pytest.raises(RuntimeError, match="GPU"):
+        solver.solve(solver.initial_guess(device="cpu"), p, device="cpu")
