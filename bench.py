@@ -34,6 +34,114 @@ def pcg_flops(N, nx):
     return 4 * (3 * N + 1) * nx * nx + 10 * (N + 1) * nx
 
 
+def stream_block_bytes(nx, nu):
+    """Algorithmic HBM bytes of one PCG iteration per time block of the streamed
+    kernels: compact layout (nx <= 16) reads A three times and B once (float32)
+    and the packed D_t twice (float64); the full layout (nx > 16) reads A_t and
+    A_{t-1} twice each, B once (float32) and the full D_t twice (float64)."""
+    if nx <= 16:
+        return 12 * nx * nx + 4 * nx * nu + 8 * nx * (nx + 1)
+    return 16 * nx * nx + 4 * nx * nu + 16 * nx * nx
+
+
+def cpu_env():
+    """Host facts the CPU baseline is quoted with."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "MKL_NUM_THREADS", "OPENBLAS_NUM_THREADS",
+                                          "NUMEXPR_NUM_THREADS")}
+    return {"cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "cpu_model": model,
+            "thread_env": env}
+
+
+def _oracle_model(name, prob):
+    from oracle import dynamics as odyn
+
+    if name == "linear":
+        return odyn.make_model("linear", prob["x0"].shape[1], prob["u_ref"].shape[2])
+    return odyn.make_model(name)
+
+
+def _cpu_warm(_):
+    time.sleep(0.05)
+    return os.getpid()
+
+
+def _cpu_init():
+    import numpy  # noqa: F401
+
+    from diffmpc_amd import workloads  # noqa: F401
+    from oracle import sqp  # noqa: F401
+
+
+def cpu_chunk(job):
+    """One worker's share of a CPU baseline: the float64 oracle (one thread)
+    on instances [lo, hi) of a seeded workload, `reps` times; returns
+    (instances solved, seconds)."""
+    gen, kw, lo, hi, backward, reps = job
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from diffmpc_amd import workloads as W
+    from oracle import sqp as osqp
+
+    name, prob, sp = getattr(W, gen)(**kw)[:3]
+    sub = W.slice_problem(prob, slice(lo, hi))
+    sub["model"] = _oracle_model(name, prob)
+
+    def once():
+        ref = osqp.sqp_solve(sub, sp)
+        if backward:
+            osqp.sqp_backward(sub, sp, ref, ref["states"] - sub["x_ref"], np.zeros_like(ref["controls"]))
+
+    times = []
+    with threadpool_limits(limits=1):
+        if reps > 1:
+            once()  # warm-up
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            once()
+            times.append(time.perf_counter() - t0)
+    return hi - lo, times
+
+
+def cpu_baseline(gen, kw, backward, sample_1core, sample_all, unit, what, reps_1core=1, chunk_reps=1):
+    """The oracle timed on the host: one thread on `sample_1core` instances
+    (the median over `reps_1core` repeats after a warm-up when repeated),
+    then every core of this process's CPU share (at most 16 worker processes,
+    one thread each) on `sample_all` instances split evenly, by wall clock;
+    rates in instances per second."""
+    import multiprocessing as mp
+    import statistics
+
+    n1, times = cpu_chunk((gen, kw, 0, sample_1core, backward, reps_1core))
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    per = max(1, sample_all // workers)
+    jobs = [(gen, kw, i * per, (i + 1) * per, backward, chunk_reps) for i in range(workers)]
+    with mp.get_context("spawn").Pool(workers, initializer=_cpu_init) as pool:
+        pool.map(_cpu_warm, range(4 * workers))  # every worker started and imported
+        t0 = time.perf_counter()
+        res = pool.map(cpu_chunk, jobs)
+        t_all = time.perf_counter() - t0
+    n_all = sum(r[0] * len(r[1]) for r in res)
+    rep = f" x {reps_1core} repeats (median)" if reps_1core > 1 else ""
+    return {
+        "value": n1 / statistics.median(times), "unit": unit, "cores": 1, "kind": "port",
+        "sample": f"{what}: {sample_1core} instances{rep}, float64 numpy oracle, one thread",
+        "all_cores": {"value": n_all / t_all, "cores": workers,
+                      "sample": f"{n_all} instance solves on {workers} worker processes (one thread each), "
+                                "wall clock" + (" including one warm-up solve per worker" if chunk_reps > 1 else "")},
+        "host": cpu_env(),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -43,7 +151,7 @@ def main():
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--horizon", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=96, help="instances in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=192, help="instances in the one-thread CPU baseline sample")
     ap.add_argument("--fused-only", action="store_true",
                     help="time only the fused path (for profiler runs); prints the fused figures")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -51,9 +159,12 @@ def main():
     ap.add_argument("--blocks-per-group", type=int, default=0, metavar="K",
                     help="force K time blocks per lane group in the register-resident kernels "
                          "(1, 2 or 4; 0 = automatic) for every solve of the run")
-    ap.add_argument("--workload", choices=["quadrotor", "ragged", "chain", "rollout", "dynparams", "imitation"],
+    ap.add_argument("--workload", choices=["quadrotor", "config0", "spacecraft7", "ragged", "chain", "rollout",
+                                           "dynparams", "imitation"],
                     default="quadrotor",
-                    help="quadrotor: the headline line (default); ragged: mixed batch of 8192 with horizons "
+                    help="quadrotor: the headline line (default); config0: the single-instance LTI QP (nx=4 nu=2 "
+                         "T=20) with the CPU solves/s headline; spacecraft7: attitude MPC B=4096 N=100 forward + "
+                         "adjoint w.r.t. x0, log-weights and x_ref; ragged: mixed batch of 8192 with horizons "
                          "U{10..200} in one graph; chain: 48-state chain B=512 N=256; rollout: the reference's "
                          "RL benchmark (LTI 8/4, N=39, 50 closed-loop steps, B=--batch or 64); dynparams: the "
                          "headline workload with gradients w.r.t. mass and inertia as well; imitation: the "
@@ -83,6 +194,10 @@ def main():
 
     if args.blocks_per_group:
         _lib.check(_lib.load().dmpc_set_blocks_per_group(args.blocks_per_group), "dmpc_set_blocks_per_group")
+    if args.workload == "config0":
+        return bench_config0(args)
+    if args.workload == "spacecraft7":
+        return bench_spacecraft7(args)
     if args.workload == "ragged":
         return bench_ragged(args)
     if args.workload == "chain":
@@ -190,7 +305,7 @@ def main():
         "kernel_ms_per_step": {"qp_iteration": qp_ms / steps_prof, "backward": bw_ms / steps_prof,
                                "linearize": ln_ms / steps_prof},
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64_valu",
             "kernel": "qp_iteration_kernel (assembly + PCG + recovery, one SQP iteration of the batch)",
             "achieved": achieved,
             "peak": MI355X_FP64_PEAK_TFLOPS,
@@ -205,25 +320,163 @@ def main():
 
     # ---- CPU baseline: the float64 oracle on a bounded sample --------------
     if not args.no_cpu_baseline:
-        from threadpoolctl import threadpool_limits
+        out["cpu_baseline"] = cpu_baseline(
+            "quadrotor", dict(batch=args.batch, seed=1, horizon=args.horizon), True, args.cpu_sample, args.batch,
+            "grads/s", f"forward + backward of the headline instances (of {args.batch})")
+    print(json.dumps(out), flush=True)
 
-        from oracle import dynamics as odyn
-        from oracle import sqp as osqp
 
-        m = args.cpu_sample
-        sub = W.slice_problem(prob, slice(0, m))
-        sub["model"] = odyn.make_model(name)
-        with threadpool_limits(limits=1):
-            t0 = time.perf_counter()
-            ref = osqp.sqp_solve(sub, sp)
-            t1 = time.perf_counter()
-            osqp.sqp_backward(sub, sp, ref, ref["states"] - sub["x_ref"], np.zeros_like(ref["controls"]))
-            t2 = time.perf_counter()
-        out["cpu_baseline"] = {
-            "value": m / (t2 - t0), "unit": "grads/s", "cores": 1, "kind": "port",
-            "sample": f"{m} of the {B} instances, forward + backward, float64 numpy oracle, one thread",
-            "solves_per_s": m / (t1 - t0),
-        }
+def _timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        out = fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def _qp_roofline(lib_mod, fn, N, nx, sqp_iters, pcg_count):
+    """Run fn once with the library's launch events on; returns the QP-iteration
+    kernel's roofline object (FP64 VALU: algorithmic PCG flops per launch over
+    the mean launch time) and fn's result."""
+    import torch
+
+    lib_mod.profile_read(lib_mod.PROF_QP_ITERATION)
+    lib_mod.load().dmpc_profile_enable(1)
+    out = fn()
+    torch.cuda.synchronize()
+    lib_mod.load().dmpc_profile_enable(0)
+    ms, n = lib_mod.profile_read(lib_mod.PROF_QP_ITERATION)
+    avg = ms / max(n, 1)
+    flops = pcg_flops(N, nx) * pcg_count(out) / sqp_iters
+    achieved = flops / (avg * 1e-3) / 1e12
+    return {"bound": "fp64_valu", "kernel": "qp_iteration_kernel (assembly + PCG + recovery, one SQP iteration)",
+            "achieved": achieved, "peak": MI355X_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / MI355X_FP64_PEAK_TFLOPS, "traffic": None, "launch_avg_ms": avg,
+            "flops_per_launch": flops}, out
+
+
+def bench_config0(args):
+    """Config 0: the plain LTI MPC QP (nx=4, nu=2, T=20, A = I + 0.1 G,
+    Q = I, R = 0.1 I, Qf = 10 I, one SQP iteration, PCG 1e-10 / 200) -- the
+    reference's CPU path.  One instance on the GPU (forward solve, and
+    forward + backward, each captured as one hipGraph: launch-latency bound),
+    the same configuration batched (B = 4096 seeded instances), and the
+    float64 oracle on the host: one instance on one thread (median of 20
+    solves after a warm-up), and one instance per worker on every core."""
+    import torch
+
+    from diffmpc_amd import _lib
+    from diffmpc_amd import workloads as W
+    from diffmpc_amd.graph import GraphedStep
+
+    dev = torch.device("cuda:0")
+    res = {}
+    for B in (1, 4096):
+        name, prob, sp = W.lti(batch=B, seed=0)
+        solver, params, guess = W.make_solver(name, prob, sp, device=dev)
+
+        def fwd():
+            with torch.no_grad():
+                return solver.solve(guess, params)
+
+        def fwd_bwd():
+            guess.dual_backward_guess.zero_()
+            wx = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
+            x0 = params["initial_state"].clone().requires_grad_(True)
+            sol = solver.solve(guess, params, {"weights_penalization_reference_state_trajectory": wx,
+                                               "initial_state": x0})
+            return torch.autograd.grad(0.5 * sol.states.square().sum(), [wx, x0])
+
+        roof, sol = _qp_roofline(_lib, fwd, prob["horizon"], 4, 1, lambda o: int(o.num_pcg_iters.sum()))
+        steps = max(args.steps, 100) if B == 1 else args.steps
+        gf, gb = GraphedStep(fwd), GraphedStep(fwd_bwd)
+        t_f, _ = _timed(gf.replay, steps, args.warmup)
+        t_b, _ = _timed(gb.replay, steps, args.warmup)
+        res[B] = {"solves_per_s": B / t_f, "grads_per_s": B / t_b, "ms_fwd": t_f * 1e3, "ms_fwd_bwd": t_b * 1e3,
+                  "pcg_iterations": int(sol.num_pcg_iters.sum()), "roofline": roof}
+    out = {
+        "metric": "config 0: LTI MPC QP solves/s, one instance (nx=4 nu=2 T=20, 1 SQP iteration, PCG 1e-10)",
+        "value": res[1]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": max(args.steps, 100),
+        "warmup": args.warmup, "ms_per_step": res[1]["ms_fwd"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG", "data": "synthetic",
+        "config": {"workload": "LTI nx=4 nu=2 T=20, one instance, one hipGraph launch per solve", "global_batch": 1},
+        "single_instance": res[1], "batched_4096": res[4096],
+        "roofline": res[4096]["roofline"],
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline("lti", dict(batch=16, seed=0), False, 1, 16, "solves/s",
+                                           "one config-0 instance solved", reps_1core=20, chunk_reps=200)
+    print(json.dumps(out), flush=True)
+
+
+def bench_spacecraft7(args):
+    """Config 2: spacecraft attitude MPC, B = 4096, nx = 7, nu = 3, N = 100,
+    dt = 0.1, Euler, inertia ~ U(0.5, 2), 3 SQP iterations, PCG 1e-8; the
+    loss 0.5 sum ||x* - x_ref||^2 differentiated w.r.t. x0, the log-weights
+    theta_Q, theta_R (w = exp(theta), shared by the batch) and x_ref: fused vs
+    the per-launch path."""
+    import torch
+
+    from diffmpc_amd import _lib
+    from diffmpc_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    B = args.batch if args.batch != 1024 else 4096
+    name, prob, sp = W.spacecraft7(batch=B, seed=2)
+    N = prob["horizon"]
+    th_q0 = torch.tensor(prob["theta_q"], dtype=torch.float64, device=dev)
+    th_r0 = torch.tensor(prob["theta_r"], dtype=torch.float64, device=dev)
+
+    def make(mode):
+        solver, params, guess = W.make_solver(name, prob, sp, device=dev, mode=mode)
+
+        def fwd_bwd():
+            guess.dual_backward_guess.zero_()
+            th_q = th_q0.clone().requires_grad_(True)
+            th_r = th_r0.clone().requires_grad_(True)
+            x0 = params["initial_state"].clone().requires_grad_(True)
+            xr = params["reference_state_trajectory"].clone().requires_grad_(True)
+            sol = solver.solve(guess, params, {"weights_penalization_reference_state_trajectory": th_q.exp(),
+                                               "weights_penalization_control_squared": th_r.exp(),
+                                               "initial_state": x0, "reference_state_trajectory": xr})
+            loss = 0.5 * (sol.states - xr).square().sum()
+            return sol, torch.autograd.grad(loss, [x0, th_q, th_r, xr])
+
+        def fwd():
+            with torch.no_grad():
+                return solver.solve(guess, params)
+        return fwd, fwd_bwd, solver
+
+    fwd, fwd_bwd, solver = make("fused")
+    t_f, _ = _timed(fwd, args.steps, args.warmup)
+    t_b, _ = _timed(fwd_bwd, args.steps, args.warmup)
+    roof, (sol, _) = _qp_roofline(_lib, fwd_bwd, N, 7, sp["num_scp_iteration_max"],
+                                  lambda o: int(o[0].num_pcg_iters.sum()))
+    pfwd, pfwd_bwd, _ = make("plain")
+    t_pf, _ = _timed(pfwd, 1, 1)
+    t_pb, _ = _timed(pfwd_bwd, 1, 1)
+    out = {
+        "metric": "config 2: spacecraft attitude fwd+adjoint grads/s (nx=7 nu=3 N=100 B=4096), fused vs plain GPU",
+        "value": B / t_b, "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_b * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG", "data": "synthetic",
+        "config": {"workload": f"spacecraft attitude nx=7 nu=3 N={N} B={B}, 3 SQP iterations, PCG 1e-8, grads "
+                               "w.r.t. x0, log Q/R weights, x_ref", "global_batch": B},
+        "fused": {"solves_per_s": B / t_f, "grads_per_s": B / t_b, "ms_fwd": t_f * 1e3, "ms_fwd_bwd": t_b * 1e3},
+        "plain": {"solves_per_s": B / t_pf, "grads_per_s": B / t_pb, "ms_fwd": t_pf * 1e3, "ms_fwd_bwd": t_pb * 1e3},
+        "fused_vs_plain": {"solves": t_pf / t_f, "grads": t_pb / t_b},
+        "pcg_iterations_per_step": {"forward": int(sol.num_pcg_iters.sum()),
+                                    "backward": int(solver.last_backward_pcg_iters.sum())},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline("spacecraft7", dict(batch=B, seed=2), True, 64, 1024, "grads/s",
+                                           f"forward + backward of config-2 instances (of {B})")
     print(json.dumps(out), flush=True)
 
 
@@ -273,6 +526,32 @@ def bench_ragged(args):
             info[f"{name}/{part}"] = {"instances": int(len(hs)), "max_horizon": int(hs.max()),
                                       "launch_shape": list(best),
                                       "tuning_ms": {f"{k[0]}/{k[1]}": v for k, v in timings.items()}}
+    # per-group roofline of the forward QP kernels (one eager forward solve
+    # with the library's launch events): register-resident groups against
+    # the FP64 VALU peak (algorithmic PCG flops), streamed groups against HBM
+    # (algorithmic block bytes per PCG iteration)
+    lib = _lib.load()
+    for (solver, params, guess, _), (key, gi) in zip(groups, info.items()):
+        nx, nu = solver.program.num_state_variables, solver.program.num_control_variables
+        hs = params["horizons"].cpu().numpy()
+        _lib.profile_read(_lib.PROF_QP_ITERATION)
+        lib.dmpc_profile_enable(1)
+        with torch.no_grad():
+            sol = solver.solve(guess, params)
+        torch.cuda.synchronize()
+        lib.dmpc_profile_enable(0)
+        ms, n = _lib.profile_read(_lib.PROF_QP_ITERATION)
+        iters = sol.num_pcg_iters.sum(dim=1).cpu().numpy()
+        if gi["launch_shape"][0] == "streamed":
+            amount = float(((hs + 1) * iters).sum()) * stream_block_bytes(nx, nu)
+            gi["roofline"] = {"bound": "hbm", "achieved": amount / (ms * 1e-3) / 1e9, "peak": 8000.0,
+                              "unit": "GB/s", "kernel_ms": ms, "launches": n}
+        else:
+            amount = float(sum(pcg_flops(int(h), nx) * int(k) for h, k in zip(hs, iters)))
+            gi["roofline"] = {"bound": "fp64_valu", "achieved": amount / (ms * 1e-3) / 1e12,
+                              "peak": MI355X_FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "kernel_ms": ms, "launches": n}
+        gi["roofline"]["frac"] = gi["roofline"]["achieved"] / gi["roofline"]["peak"]
+    dom = max(info, key=lambda k: info[k]["roofline"]["kernel_ms"])
     graphed = GraphedStep(_fwd_bwd_step(groups))
     for _ in range(args.warmup):
         graphed.replay()
@@ -289,6 +568,10 @@ def bench_ragged(args):
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 Jacobians, f64 dynamics values/Schur/PCG", "data": "synthetic",
         "config": {"workload": "ragged LTI/spacecraft7/quadrotor mix, one hipGraph per step", "global_batch": B},
+        "roofline": dict(info[dom]["roofline"], group=dom,
+                         kernel=("stream assembly + stream_qp_kernel" if info[dom]["launch_shape"][0] == "streamed"
+                                 else "qp_iteration_kernel") + " (forward, all SQP iterations of the group)",
+                         traffic=None),
         "groups": info,
     }), flush=True)
 

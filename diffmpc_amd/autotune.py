@@ -7,9 +7,11 @@ streamed variant that keeps the blocks in a device workspace.  ``tune`` times
 every candidate the library accepts on the caller's own problem (forward
 plus backward, HIP events on the current stream), writes the fastest into the
 solver's params (``fused_variant``, ``blocks_per_group``) and caches it per
-(model, nx, nu, horizon, batch, line search) key.  No reference counterpart:
+problem shape (model, sizes, batch, ragged horizons, mode, line search, SQP
+and PCG settings).  No reference counterpart:
 XLA picks its own launch shapes.
 """
+import hashlib
 from typing import Dict, List, Tuple
 
 import torch
@@ -46,10 +48,16 @@ def register_horizon_limit(model: int, nx: int, nu: int, linesearch: bool = Fals
     return lo
 
 
-def _key(solver, B):
+def _key(solver, params, B):
+    """Everything the timings depend on: model and sizes, batch, the ragged
+    horizons (the exact mix, by digest), mode, line search, SQP and PCG caps."""
     prog, p = solver.program, solver.params
-    return (prog.dynamics.model_id, prog.num_state_variables, prog.num_control_variables, prog.horizon, B,
-            bool(p["linesearch"]), tuple(p["linesearch_alphas"]), p["num_scp_iteration_max"])
+    h = params.get("horizons") if isinstance(params, dict) else None
+    hkey = None if h is None else hashlib.sha1(torch.as_tensor(h).cpu().numpy().tobytes()).hexdigest()
+    return (prog.dynamics.model_id, prog.num_state_variables, prog.num_control_variables, prog.horizon, B, hkey,
+            p.get("mode", "auto"), bool(p["linesearch"]), tuple(p["linesearch_alphas"]),
+            p["num_scp_iteration_max"], float(p["tol_convergence"]), int(p["pcg"]["max_iter"]),
+            float(p["pcg"]["tol_epsilon"]))
 
 
 def _apply(solver, choice):
@@ -61,7 +69,7 @@ def tune(solver, params, guess, reps: int = 2, backward: bool = True, candidates
     (choice, {choice: ms per fwd(+bwd)}).  Cached choices are applied without
     timing (the timings dict is then empty)."""
     B = guess.states.shape[0] if guess.states.dim() == 3 else 1
-    key = _key(solver, B)
+    key = _key(solver, params, B)
     if key in _cache:
         _apply(solver, _cache[key])
         return _cache[key], {}
