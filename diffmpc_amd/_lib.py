@@ -18,6 +18,8 @@ DMPC_OK = 0
 DMPC_ERR_ARG = -1
 DMPC_ERR_UNSUPPORTED = -2
 DMPC_ERR_HIP = -3
+DMPC_ERR_TOO_LARGE = -4
+NOALLOC_SCRATCH_BYTES = 64 << 20  # DMPC_NOALLOC_SCRATCH_BYTES
 
 MODE_FUSED = 0
 MODE_PLAIN = 1
@@ -92,6 +94,9 @@ _SIGNATURES = {
     "dmpc_supported": (ctypes.c_int, [ctypes.c_int32] * 5),
     "dmpc_sqp_solve": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(SolverParams), ctypes.POINTER(ProblemData),
                                       ctypes.POINTER(Solution), ctypes.POINTER(Saved), ctypes.c_int32, ctypes.c_void_p]),
+    "dmpc_sqp_solve_noalloc": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(SolverParams),
+                                              ctypes.POINTER(ProblemData), ctypes.POINTER(Solution),
+                                              ctypes.POINTER(Saved), ctypes.c_void_p]),
     "dmpc_sqp_backward": (ctypes.c_int, [ctypes.POINTER(Problem), ctypes.POINTER(SolverParams), ctypes.POINTER(ProblemData),
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Saved), ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.POINTER(Grads), ctypes.c_int32, ctypes.c_void_p]),
@@ -186,7 +191,8 @@ def exported_symbols():
 def check(rc, what):
     if rc == DMPC_OK:
         return
-    names = {DMPC_ERR_ARG: "bad argument", DMPC_ERR_UNSUPPORTED: "unsupported configuration", DMPC_ERR_HIP: "HIP error"}
+    names = {DMPC_ERR_ARG: "bad argument", DMPC_ERR_UNSUPPORTED: "unsupported configuration", DMPC_ERR_HIP: "HIP error",
+             DMPC_ERR_TOO_LARGE: "workspace larger than the allocation-free entry point's scratch"}
     raise DmpcError(f"{what} failed: {names.get(rc, rc)} ({rc})")
 
 

@@ -1,6 +1,7 @@
 // C ABI of libdmpc.so (declared in include/dmpc.h): argument checks and
 // dispatch to the fused or per-launch implementation.
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -190,8 +191,45 @@ int dmpc_workspace_bytes(int32_t mode, const dmpc_problem* prob, const dmpc_solv
   return DMPC_OK;
 }
 
-int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
-                   dmpc_solution* sol, dmpc_saved* saved, int32_t mode, void* stream) {
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Library-owned scratch of dmpc_sqp_solve_noalloc: the streamed fused
+// variant's workspace (preconditioner blocks, PCG vectors, line-search
+// scratch), one copy per device, held until the process ends.  Host code
+// never passes `scratch` itself (its host-side name is not a device
+// address): the device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_ptr.  The kernels write every
+// word of the workspace they read (vectors are cleared, blocks assembled)
+// before reading it, so nothing needs clearing between calls.
+static __device__ __align__(256) unsigned char scratch[DMPC_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device, with that device current).
+int scratch_on_current_device(char** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return DMPC_ERR_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return DMPC_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return DMPC_ERR_HIP;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<char*>(g_scratch_ptr[dev]);
+  return DMPC_OK;
+}
+
+// The body of dmpc_sqp_solve and dmpc_sqp_solve_noalloc: with `noalloc` the
+// streamed variant's workspace is the device's `scratch` (saved->work is not
+// read) and only the fused mode is accepted (the plain mode allocates).
+int sqp_solve_impl(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
+                   dmpc_solution* sol, dmpc_saved* saved, int32_t mode, void* stream, bool noalloc) {
   int rc = check_problem(prob);
   if (rc) return rc;
   if (!params || !data || !sol || !saved) return DMPC_ERR_ARG;
@@ -254,16 +292,37 @@ int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, c
   if (mode == DMPC_MODE_FUSED) {
     a.work_stride = fused_work_stride(prob->model, prob->nx, prob->nu, prob->horizon, a.ls_eval, a.n_alphas, a.variant);
     if (a.work_stride) {
-      if (!saved->work || saved->work_bytes < (uint64_t)a.work_stride * (uint64_t)a.B) return DMPC_ERR_ARG;
-      a.work = static_cast<char*>(saved->work);
+      const uint64_t need = (uint64_t)a.work_stride * (uint64_t)a.B;
+      if (noalloc) {
+        if (need > (uint64_t)DMPC_NOALLOC_SCRATCH_BYTES) return DMPC_ERR_TOO_LARGE;
+        rc = scratch_on_current_device(&a.work);
+        if (rc) return rc;
+      } else {
+        if (!saved->work || saved->work_bytes < need) return DMPC_ERR_ARG;
+        a.work = static_cast<char*>(saved->work);
+      }
     }
     return fused_forward(prob->model, prob->nx, prob->nu, a, st);
   }
+  if (noalloc) return DMPC_ERR_UNSUPPORTED;  // the per-launch mode allocates its scratch
   if (mode == DMPC_MODE_PLAIN) {
     if (data->horizons) return DMPC_ERR_UNSUPPORTED;  // ragged batches: fused mode only
     return plain_forward(prob->model, prob->nx, prob->nu, a, st);
   }
   return DMPC_ERR_ARG;
+}
+}  // namespace
+
+extern "C" {
+
+int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
+                   dmpc_solution* sol, dmpc_saved* saved, int32_t mode, void* stream) {
+  return sqp_solve_impl(prob, params, data, sol, saved, mode, stream, false);
+}
+
+int dmpc_sqp_solve_noalloc(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
+                           dmpc_solution* sol, dmpc_saved* saved, void* stream) {
+  return sqp_solve_impl(prob, params, data, sol, saved, DMPC_MODE_FUSED, stream, true);
 }
 
 int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,

@@ -34,6 +34,10 @@ extern "C" {
 #define DMPC_ERR_ARG (-1)          /* null pointer, bad size or option */
 #define DMPC_ERR_UNSUPPORTED (-2)  /* model / size / option not built in this mode */
 #define DMPC_ERR_HIP (-3)          /* a HIP call failed */
+#define DMPC_ERR_TOO_LARGE (-4)    /* workspace beyond DMPC_NOALLOC_SCRATCH_BYTES (dmpc_sqp_solve_noalloc) */
+
+/* Size of the library-owned device scratch of dmpc_sqp_solve_noalloc, per device. */
+#define DMPC_NOALLOC_SCRATCH_BYTES (64u << 20)
 
 /* dynamics models; flat per-instance parameter layouts (float32, n_params) */
 #define DMPC_MODEL_LINEAR 0      /* A (nx*nx, row-major), B (nx*nu), b (nx)   linear_dynamics.py:46-70 */
@@ -152,6 +156,23 @@ int dmpc_supported(int32_t mode, int32_t model, int32_t nx, int32_t nu, int32_t 
 /* Batched SQP solve (sqp.py:436-560).  Solution arrays carry the initial guess in. */
 int dmpc_sqp_solve(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
                    dmpc_solution* sol, dmpc_saved* saved, int32_t mode, void* stream);
+
+/* dmpc_sqp_solve in the fused mode without any allocation, for callers that
+ * cannot allocate device memory (code under HIP graph capture, real-time
+ * control loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The caller passes inputs and outputs as for dmpc_sqp_solve
+ * (saved->work and saved->work_bytes are not read); the workspace the
+ * streamed variant needs (dmpc_workspace_bytes for DMPC_MODE_FUSED) comes
+ * from a library-owned device array of DMPC_NOALLOC_SCRATCH_BYTES per
+ * device, held until the process ends.  Returns DMPC_ERR_TOO_LARGE, before
+ * launching anything, when the problem needs more.  The calling thread's
+ * current device must be the stream's.  Every call on a device shares that
+ * device's array: calls that overlap in time on one device (other streams,
+ * other threads) race and must be serialised by the caller; the array keeps
+ * the last call's contents, which no call reads before writing.  Calls no
+ * hipMalloc/hipFree (with dmpc_profile_enable off). */
+int dmpc_sqp_solve_noalloc(const dmpc_problem* prob, const dmpc_solver_params* params, const dmpc_problem_data* data,
+                           dmpc_solution* sol, dmpc_saved* saved, void* stream);
 
 /* Implicit-differentiation backward (sqp.py:668-798): one adjoint PCG on the
  * saved linearization with q = -dl/dx, r = -dl/du[:-1], C = 0, then the
