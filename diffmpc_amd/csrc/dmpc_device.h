@@ -73,6 +73,38 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return gsum<16>(k < nw ? red[k] : 0.0);
 }
 
+// block_sum for workgroups of up to 1024 threads with `red` holding 64
+// doubles: each 16-lane row publishes its DPP row sum (no cross-row
+// combination inside the wave), and after the barrier lane k of every row
+// adds the partials k, k + 16, k + 32, k + 48 and the row sums those with
+// the butterfly.  Every thread reads the same partials in the same order, so
+// every thread gets the same bits.  Fewer instructions than block_sum.
+__device__ __forceinline__ double block_sum_rows(double v, double* red) {
+  v = gsum<16>(v);
+  if ((threadIdx.x & 15) == 0) red[threadIdx.x >> 4] = v;
+  __syncthreads();
+  const int nr = blockDim.x >> 4;
+  double s = 0.0;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int k = (threadIdx.x & 15) + 16 * m;
+    s += k < nr ? red[k] : 0.0;
+  }
+  return gsum<16>(s);
+}
+
+// a / b for the PCG step lengths: the hardware reciprocal refined by two
+// Newton steps and one residual correction of the quotient (within an ulp of
+// the correctly rounded quotient for normal operands; fewer instructions than
+// the IEEE division sequence).
+__device__ __forceinline__ double div_newton(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+
 __device__ __forceinline__ double block_max(double v, double* red) {
   v = wave_max(v);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -194,6 +226,44 @@ __device__ __forceinline__ double gdot16x13(const double* a, double v, double ac
         "v"(a[8]), "v"(a[9]), "v"(a[10]), "v"(a[11]), "v"(a[12]));
 #undef DMPC_FMAC_BCAST
   return acc;
+}
+
+// Two independent gdot16x13 chains in one instruction stream, terms of the
+// two chains alternating: acc0 += sum_j a0[j] v0_j, acc1 += sum_j a1[j] v1_j,
+// each chain summed in order j = 0..12 with one rounding per term (the same
+// arithmetic as two gdot16x13 calls).  A lane with two time blocks, or two
+// products of one block that do not depend on each other, then keeps two
+// FMAs in flight, which the dependent single chain cannot.  Terms j < 7 and
+// j >= 7 are two asm blocks so that the caller can load the second half of
+// its operands after the first half has been consumed.
+#define DMPC_FMAC_PAIR(acc, vv, op, j) "v_fmac_f64_dpp %" #acc ", %" #vv ", %" #op " row_newbcast:" #j " row_mask:0xf bank_mask:0xf\n"
+__device__ __forceinline__ void gdot16x13_pair_lo(const double* a0, double v0, const double* a1, double v1,
+                                                  double& acc0, double& acc1) {
+  asm("s_nop 4\n" DMPC_FMAC_PAIR(0, 2, 4, 0) DMPC_FMAC_PAIR(1, 3, 11, 0) DMPC_FMAC_PAIR(0, 2, 5, 1)
+          DMPC_FMAC_PAIR(1, 3, 12, 1) DMPC_FMAC_PAIR(0, 2, 6, 2) DMPC_FMAC_PAIR(1, 3, 13, 2)
+              DMPC_FMAC_PAIR(0, 2, 7, 3) DMPC_FMAC_PAIR(1, 3, 14, 3) DMPC_FMAC_PAIR(0, 2, 8, 4)
+                  DMPC_FMAC_PAIR(1, 3, 15, 4) DMPC_FMAC_PAIR(0, 2, 9, 5) DMPC_FMAC_PAIR(1, 3, 16, 5)
+                      DMPC_FMAC_PAIR(0, 2, 10, 6) DMPC_FMAC_PAIR(1, 3, 17, 6)
+      : "+&v"(acc0), "+&v"(acc1)
+      : "v"(v0), "v"(v1), "v"(a0[0]), "v"(a0[1]), "v"(a0[2]), "v"(a0[3]), "v"(a0[4]), "v"(a0[5]), "v"(a0[6]),
+        "v"(a1[0]), "v"(a1[1]), "v"(a1[2]), "v"(a1[3]), "v"(a1[4]), "v"(a1[5]), "v"(a1[6]));
+}
+// terms j = 7..12; a0, a1 point at element 7
+__device__ __forceinline__ void gdot16x13_pair_hi(const double* a0, double v0, const double* a1, double v1,
+                                                  double& acc0, double& acc1) {
+  asm("s_nop 4\n" DMPC_FMAC_PAIR(0, 2, 4, 7) DMPC_FMAC_PAIR(1, 3, 10, 7) DMPC_FMAC_PAIR(0, 2, 5, 8)
+          DMPC_FMAC_PAIR(1, 3, 11, 8) DMPC_FMAC_PAIR(0, 2, 6, 9) DMPC_FMAC_PAIR(1, 3, 12, 9)
+              DMPC_FMAC_PAIR(0, 2, 7, 10) DMPC_FMAC_PAIR(1, 3, 13, 10) DMPC_FMAC_PAIR(0, 2, 8, 11)
+                  DMPC_FMAC_PAIR(1, 3, 14, 11) DMPC_FMAC_PAIR(0, 2, 9, 12) DMPC_FMAC_PAIR(1, 3, 15, 12)
+      : "+&v"(acc0), "+&v"(acc1)
+      : "v"(v0), "v"(v1), "v"(a0[0]), "v"(a0[1]), "v"(a0[2]), "v"(a0[3]), "v"(a0[4]), "v"(a0[5]), "v"(a1[0]),
+        "v"(a1[1]), "v"(a1[2]), "v"(a1[3]), "v"(a1[4]), "v"(a1[5]));
+}
+#undef DMPC_FMAC_PAIR
+__device__ __forceinline__ void gdot16x13_pair(const double* a0, double v0, const double* a1, double v1,
+                                               double& acc0, double& acc1) {
+  gdot16x13_pair_lo(a0, v0, a1, v1, acc0, acc1);
+  gdot16x13_pair_hi(a0 + 7, v0, a1 + 7, v1, acc0, acc1);
 }
 
 // As gdot16x13 for float32 a[]: each term converts a[j] into a scratch pair

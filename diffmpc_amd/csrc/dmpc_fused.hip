@@ -54,7 +54,7 @@ struct LdsPlan {
   static __host__ __device__ size_t doubles(int N) {
     const size_t NB = N + 1, NBP = N + 3;
     return NB * NX /*xb*/ + NB * NU /*ub*/ + NBP * NX /*lamS*/ + NB * NX /*qinv*/ + NB * NX /*qv*/ +
-           NB * NU /*rv*/ + NU /*rinv*/ + NX /*x0*/ + 3 * 16 /*red*/ + 4 * NBP * NX /*RT Y YQ YX*/ +
+           NB * NU /*rv*/ + NU /*rinv*/ + NX /*x0*/ + 3 * 64 /*red*/ + 4 * NBP * NX /*RT Y YQ YX*/ +
            NB * NX * NX /*LF*/ + NB * NX /*ID*/ + (size_t)N * NX /*fd*/;
   }
   static __host__ __device__ size_t floats(int N, int P) { return (size_t)P; }
@@ -86,9 +86,9 @@ __device__ Lds carve(void* base, int N, int ls = 0) {
   L.rv = d; d += NB * NU;
   L.rinv = d; d += NU;
   L.x0 = d; d += NX;
-  L.redA = d; d += 16;
-  L.redB = d; d += 16;
-  L.redC = d; d += 16;
+  L.redA = d; d += 64;  // 64 doubles each: block_sum_rows (one partial per 16-lane row)
+  L.redB = d; d += 64;
+  L.redC = d; d += 64;
   L.RT = d; d += NBP * NX;
   L.Y = d; d += NBP * NX;
   L.YQ = d; d += NBP * NX;
@@ -117,9 +117,10 @@ __host__ __device__ constexpr bool quad_bterm() { return G == 16 && NU == 4; }
 // Per-thread register state of the K blocks it owns.
 template <int NX, int NU, int K>
 struct Regs {
-  // A/B entries are float32 values; with one block per thread group they are
-  // kept as float64 (no conversion in the PCG loop), with more as float32
-  using AT = typename std::conditional<K == 1, double, float>::type;
+  // A/B entries are float32 values; with one or two blocks per thread group
+  // they are kept as float64 (no conversion in the PCG loop), with four as
+  // float32
+  using AT = typename std::conditional<K <= 2, double, float>::type;
   AT acol[K][NX];   // column `lane` of A_t            (zero for t = N)
   AT arow[K][NX];   // row `lane` of A_{t-1}           (zero for t = 0)
   AT brow[K][NU];   // row `lane` of B_{t-1}           (zero for t = 0;
@@ -270,6 +271,81 @@ __device__ __forceinline__ double dmul(const Regs<NX, NU, K>& R, int s, const Ld
   return R.ok[s] ? acc : 0.0;
 }
 
+// Two chains of a thread run as one interleaved pair (gdot16x13_pair) when
+// both are 13-term float64 rows of 16-lane groups.
+template <int G, int NX, class AT>
+__host__ __device__ constexpr bool pairable() {
+  return G == 16 && NX == 13 && std::is_same<AT, double>::value;
+}
+
+// (oa, ob) = (gdot(ma, va), gdot(mb, vb)) for two products that do not
+// depend on each other; the same arithmetic either way.
+template <int G, int NX, class AT>
+__device__ __forceinline__ void gdot_two(const AT* ma, double va, const AT* mb, double vb, int lane, int salt,
+                                         double& oa, double& ob) {
+  if constexpr (pairable<G, NX, AT>()) {
+    oa = 0.0;
+    ob = 0.0;
+    gdot16x13_pair(ma, va, mb, vb, oa, ob);
+  } else {
+    oa = gdot<G, NX>(ma, va, lane, salt);
+    ob = gdot<G, NX>(mb, vb, lane, salt);
+  }
+}
+
+// out[s] = gdot(M[s], v[s]) for the K blocks of the thread (two blocks as
+// one interleaved pair).
+template <int G, int NX, int K, class AT>
+__device__ __forceinline__ void gdot_k(const AT (&M)[K][NX], const double (&v)[K], int lane, int salt,
+                                       double (&out)[K]) {
+  if constexpr (K == 2 && pairable<G, NX, AT>()) {
+    gdot_two<G, NX>(M[0], v[0], M[1], v[1], lane, salt, out[0], out[1]);
+  } else {
+#pragma unroll
+    for (int s = 0; s < K; ++s) out[s] = gdot<G, NX>(M[s], v[s], lane, salt);
+  }
+}
+
+// y[s] = D_{t_s} v[s] for the K blocks of the thread, zero on masked lanes;
+// two blocks with D in LDS read their columns in two halves and run the
+// products as one interleaved pair (the same arithmetic as dmul).
+template <int G, int NX, int NU, int K>
+__device__ __forceinline__ void dmul_k(const Regs<NX, NU, K>& R, const Lds& L, const double (&v)[K], int lane,
+                                       double (&y)[K]) {
+  if constexpr (K == 2 && d_in_lds<NX>() && G == 16 && NX == 13) {
+    const int li = lane < NX ? lane : 0;
+    const double* c0 = L.LF + (size_t)R.t[0] * NX * NX + li;
+    const double* c1 = L.LF + (size_t)R.t[1] * NX * NX + li;
+    double a0 = 0.0, a1 = 0.0;
+    {
+      double r0[7], r1[7];
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        r0[j] = c0[j * NX];
+        r1[j] = c1[j * NX];
+      }
+      gdot16x13_pair_lo(r0, v[0], r1, v[1], a0, a1);
+    }
+    {
+      // the second half is addressed through the first half's result, so its
+      // loads are not hoisted above the first half (register pressure)
+      const int o = after(7 * NX, a0);
+      double r0[6], r1[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        r0[j] = c0[o + j * NX];
+        r1[j] = c1[o + j * NX];
+      }
+      gdot16x13_pair_hi(r0, v[0], r1, v[1], a0, a1);
+    }
+    y[0] = R.ok[0] ? a0 : 0.0;
+    y[1] = R.ok[1] ? a1 : 0.0;
+  } else {
+#pragma unroll
+    for (int s = 0; s < K; ++s) y[s] = dmul<G>(R, s, L, v[s], lane);
+  }
+}
+
 // Entry `lane` of block tb of an LDS vector (padded layout), 0 outside the block.
 template <int NX>
 __device__ __forceinline__ double own(const double* V, int tb, int lane) {
@@ -360,7 +436,7 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
       L.RT[(t + 1) * NX + lane] = rt[s];
     }
   }
-  double eta = block_sum(part, L.redC);
+  double eta = block_sum_rows(part, L.redC);
   double beta = 0.0;
   int it = 0;
   while (true) {
@@ -370,13 +446,16 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
     // A_t^T p_{t+1} = A_t^T rt_{t+1} + beta A_t^T p_{t+1}(previous), so only
     // the neighbour's rt is read; the previous product stays in registers.
     double vpart = 0.0;
+    double rn[K], apn[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) rn[s] = own<NX>(L.RT, R.t[s] + 1, lane);
+    gdot_k<G, NX>(R.acol, rn, lane, it, apn);
 #pragma unroll
     for (int s = 0; s < K; ++s) {
       const int t = R.t[s];
       const double pv = rt[s] + beta * p[s];
       p[s] = pv;
-      const double rn = own<NX>(L.RT, t + 1, lane);
-      ap[s] = gdot<G, NX>(R.acol[s], rn, lane, it) + beta * ap[s];
+      ap[s] = apn[s] + beta * ap[s];
       double q;
       gb[s] = bgram<G>(R, s, L, pv, it, &q);
       const double wx = ((t == 0) ? pv : -pv) + ap[s];
@@ -393,41 +472,56 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
         }
       }
     }
-    const double v = -block_sum(vpart, L.redA);
-    const double alpha = eta / v;
+    const double v = -block_sum_rows(vpart, L.redA);
+    const double alpha = div_newton(eta, v);
     // ---- phase B: ups = S p; lam += alpha p; r -= alpha ups; y = D r -----
+    {
+      double u[K], au[K], y[K];
 #pragma unroll
-    for (int s = 0; s < K; ++s) {
-      const int t = R.t[s];
-      const double u = own<NX>(L.YX, t - 1, lane);
-      const double g = (t == 0 ? 1.0 : -1.0) * yxo[s] + gdot<G, NX>(R.arow[s], u, lane, it) + gb[s];
-      if (R.ok[s]) {
-        lam[s] += alpha * p[s];
-        r[s] += alpha * g;  // r -= alpha * ups, ups = -g
+      for (int s = 0; s < K; ++s) u[s] = own<NX>(L.YX, R.t[s] - 1, lane);
+      gdot_k<G, NX>(R.arow, u, lane, it, au);
+#pragma unroll
+      for (int s = 0; s < K; ++s) {
+        const double g = (R.t[s] == 0 ? 1.0 : -1.0) * yxo[s] + au[s] + gb[s];
+        if (R.ok[s]) {
+          lam[s] += alpha * p[s];
+          r[s] += alpha * g;  // r -= alpha * ups, ups = -g
+        }
       }
-      const double y = dmul<G>(R, s, L, r[s], lane);
-      if (R.ok[s]) {
-        L.Y[(t + 1) * NX + lane] = y;
-        L.YQ[(t + 1) * NX + lane] = R.qi[s] * y;
+      dmul_k<G>(R, L, r, lane, y);
+#pragma unroll
+      for (int s = 0; s < K; ++s) {
+        if (R.ok[s]) {
+          const int t = R.t[s];
+          L.Y[(t + 1) * NX + lane] = y[s];
+          L.YQ[(t + 1) * NX + lane] = R.qi[s] * y[s];
+        }
       }
     }
     __syncthreads();
     // ---- phase C: rt = Phi^-1 r; eta' = r.rt ------------------------------
     double epart = 0.0;
+    {
+      double z[K];
+#pragma unroll
+      for (int s = 0; s < K; ++s) {
+        const int t = R.t[s];
+        double a1, a2;
+        gdot_two<G, NX>(R.arow[s], own<NX>(L.YQ, t - 1, lane), R.acol[s], own<NX>(L.Y, t + 1, lane), lane, it, a1,
+                        a2);
+        z[s] = R.ok[s] ? r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2 : 0.0;
+      }
+      dmul_k<G>(R, L, z, lane, rt);
+    }
 #pragma unroll
     for (int s = 0; s < K; ++s) {
-      const int t = R.t[s];
-      const double a1 = gdot<G, NX>(R.arow[s], own<NX>(L.YQ, t - 1, lane), lane, it);
-      const double a2 = gdot<G, NX>(R.acol[s], own<NX>(L.Y, t + 1, lane), lane, it);
-      const double z = R.ok[s] ? r[s] - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * R.qi[s] * a2 : 0.0;
-      rt[s] = dmul<G>(R, s, L, z, lane);
       if (R.ok[s]) {
         epart += r[s] * rt[s];
-        L.RT[(t + 1) * NX + lane] = rt[s];
+        L.RT[(R.t[s] + 1) * NX + lane] = rt[s];
       }
     }
-    const double eta_new = block_sum(epart, L.redB);
-    beta = eta_new / eta;
+    const double eta_new = block_sum_rows(epart, L.redB);
+    beta = div_newton(eta_new, eta);
     eta = eta_new;
     ++it;
   }

