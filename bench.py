@@ -23,8 +23,8 @@ METRIC = "SQP solves/s & fwd+bwd grads/s, quadrotor T=50 B=1024, fused vs plain 
 MI355X_FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per qp_iteration_kernel launch at the default workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scale factors
-# from a 256 MiB device copy): 102.8 MB read + 77.4 MB written
-QP_TRAFFIC_BYTES = 180.2e6
+# from a 256 MiB device copy): 102.7 MB read + 77.3 MB written
+QP_TRAFFIC_BYTES = 180.0e6
 QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
 
 
