@@ -959,6 +959,18 @@ struct Launch {
 // compiler may give each thread 128 / 256 / 512 registers.
 inline int max_threads(int K) { return K == 1 ? 1024 : (K == 2 ? 512 : 256); }
 
+// Largest number of time blocks per lane group a model is dispatched with.
+// The attitude model stops at two: its four-block backward
+// (fused_backward_kernel<Spacecraft7Model, 8, 4, 256>, 256 VGPRs + AGPRs,
+// SGPRs spilled to VGPR lanes) once returned run-to-run varying dl/dw_x
+// when its adjoint outputs were stored to global memory inside the block
+// loop, and the cause was not found in its code or its assembly; the
+// variant is not built rather than kept on a workaround.
+template <class Model>
+constexpr int max_blocks_per_group() {
+  return std::is_same<Model, Spacecraft7Model>::value ? 2 : 4;
+}
+
 template <class Model, int G>
 Launch plan(int N, int P, int ls = 0, int forced_K = 0) {
   constexpr int NX = Model::NX, NU = Model::NU;
@@ -970,6 +982,7 @@ Launch plan(int N, int P, int ls = 0, int forced_K = 0) {
   const int cand[3] = {1, 2, 4};
   for (int c = 0; c < 3; ++c) {
     const int K = cand[c];
+    if (K > max_blocks_per_group<Model>()) continue;
     const int want = forced_K > 0 ? forced_K : g_forced_K;
     if (want > 0 && K != want) continue;
     const int groups = (NB + K - 1) / K;
@@ -1028,7 +1041,9 @@ int fwd_model(const SolveArgs& a, hipStream_t st) {
   switch (l.K) {
     case 1: return launch_fwd_k<Model, G, 1, 1024>(a, l, st);
     case 2: return launch_fwd_k<Model, G, 2, 512>(a, l, st);
-    case 4: return launch_fwd_k<Model, G, 4, 256>(a, l, st);
+    case 4:
+      if constexpr (max_blocks_per_group<Model>() >= 4) return launch_fwd_k<Model, G, 4, 256>(a, l, st);
+      return DMPC_ERR_UNSUPPORTED;
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }
@@ -1040,7 +1055,9 @@ int bwd_model(const BackwardArgs& a, hipStream_t st) {
   switch (l.K) {
     case 1: return launch_bwd_k<Model, G, 1, 1024>(a, l, st);
     case 2: return launch_bwd_k<Model, G, 2, 512>(a, l, st);
-    case 4: return launch_bwd_k<Model, G, 4, 256>(a, l, st);
+    case 4:
+      if constexpr (max_blocks_per_group<Model>() >= 4) return launch_bwd_k<Model, G, 4, 256>(a, l, st);
+      return DMPC_ERR_UNSUPPORTED;
     default: return DMPC_ERR_UNSUPPORTED;
   }
 }
