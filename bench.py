@@ -26,6 +26,10 @@ MI355X_FP64_PEAK_TFLOPS = 78.6
 # from a 256 MiB device copy): 102.7 MB read + 77.3 MB written
 QP_TRAFFIC_BYTES = 180.0e6
 QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
+# HBM bytes per forward SQP iteration of the chain workload (stream_assemble_kernel
+# + stream_qp_kernel, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0): 16.90 TB read + 0.77 TB written
+CHAIN_TRAFFIC_BYTES = 17.67e12
+CHAIN_TRAFFIC_SOURCE = "profiles/chain_nx48_B512_N256.md"
 
 
 def pcg_flops(N, nx):
@@ -589,7 +593,7 @@ def bench_chain(args):
     name, prob, sp = W.chain(batch=512, seed=4)
     N, nx, nu = prob["horizon"], 48, 12
     res = {}
-    for mode in ("fused", "plain"):
+    for mode in ("fused",) if args.fused_only else ("fused", "plain"):
         solver, params, guess = W.make_solver(name, prob, sp, device=dev, mode=mode)
         w = params["weights_penalization_reference_state_trajectory"].clone().requires_grad_(True)
         step = _fwd_bwd_step([(solver, params, guess, w)])
@@ -616,6 +620,10 @@ def bench_chain(args):
     launch_ms = qp_ms / max(qp_n, 1)
     achieved = per_block * (N + 1) * iters / (launch_ms * 1e-3) / 1e9
     B = prob["x0"].shape[0]
+    if args.fused_only:
+        print(json.dumps({"fused_ms_fwd_bwd": res["fused"] * 1e3, "qp_launch_avg_ms": launch_ms,
+                          "pcg_iterations_forward": iters, "achieved_gbs": achieved}), flush=True)
+        return
     print(json.dumps({
         "metric": "chain fwd+bwd grads/s (nx=48 nu=12 N=256 B=512), fused vs plain GPU",
         "value": B / res["fused"], "unit": "grads/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
@@ -628,7 +636,8 @@ def bench_chain(args):
         "pcg_iterations_forward": iters,
         "roofline": {"bound": "hbm", "kernel": "stream assembly + stream_qp_kernel (forward SQP iteration)",
                      "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
-                     "traffic": None, "launch_avg_ms": launch_ms},
+                     "traffic": CHAIN_TRAFFIC_BYTES if (N, B) == (256, 512) else None,
+                     "traffic_source": CHAIN_TRAFFIC_SOURCE, "launch_avg_ms": launch_ms},
     }), flush=True)
 
 
