@@ -30,6 +30,12 @@ QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
 # + stream_qp_kernel, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0): 16.90 TB read + 0.77 TB written
 CHAIN_TRAFFIC_BYTES = 17.67e12
 CHAIN_TRAFFIC_SOURCE = "profiles/chain_nx48_B512_N256.md"
+# HBM bytes per forward launch (stream_assemble_kernel + stream_qp_kernel, one SQP
+# iteration, mean over the three) of the ragged mix's dominant group, the streamed
+# quadrotor instances with N in 64..200: 267.2 GB read + 22.3 GB written
+RAGGED_TRAFFIC_BYTES = 289.4e9
+RAGGED_TRAFFIC_GROUP = "quadrotor/1"
+RAGGED_TRAFFIC_SOURCE = "profiles/ragged_mixed_B8192_T10-200.md"
 
 
 def pcg_flops(N, nx):
@@ -575,7 +581,9 @@ def bench_ragged(args):
         "roofline": dict(info[dom]["roofline"], group=dom,
                          kernel=("stream assembly + stream_qp_kernel" if info[dom]["launch_shape"][0] == "streamed"
                                  else "qp_iteration_kernel") + " (forward, all SQP iterations of the group)",
-                         traffic=None),
+                         traffic=RAGGED_TRAFFIC_BYTES if dom == RAGGED_TRAFFIC_GROUP else None,
+                         traffic_unit="bytes per forward launch (one SQP iteration)",
+                         traffic_source=RAGGED_TRAFFIC_SOURCE),
         "groups": info,
     }), flush=True)
 
