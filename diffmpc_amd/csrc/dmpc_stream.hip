@@ -84,6 +84,15 @@ __device__ __forceinline__ double vget(const double* V, int tb, int lane) {
   return lane < NX ? V[(tb + 1) * NX + lane] : 0.0;
 }
 
+// Threads per workgroup of the streamed kernels: up to 64 lane groups of 16
+// (1024 threads, 16 waves) for blocks of up to 16 states, whose PCG loop fits
+// in the 128 registers per lane that 16 waves leave (the line search and the
+// recovery spill a little, once per SQP iteration); 512 for 64-lane blocks.
+// More lane groups mean fewer rounds per PCG phase and more waves to cover
+// the latency of each round's block loads.
+template <class Model>
+__host__ __device__ constexpr int stream_max_threads() { return Model::NX <= 16 ? 1024 : 512; }
+
 // The first NV padded vectors (V_LAM, V_P, V_R, V_RT, V_Y, V_YQ, V_YX, then
 // V_AP, V_YXO, V_GB, V_GAM) in LDS instead of the workspace: every PCG phase
 // reads and writes them, and the neighbour exchanges go through them.  NV is
@@ -123,7 +132,7 @@ __device__ __forceinline__ double gdot_mem(const AT* a, int stride, double v, in
   AT r[M];
 #pragma unroll
   for (int j = 0; j < M; ++j) r[j] = a[j * stride];
-  return gdot_chain<G, M>(r, v, lane, 0, 0.0);
+  return gdot<G, M>(r, v, lane, 0);
 }
 
 // Group sum of b_k * v for the NU columns k of B_{t-1} (brow: the lane's
@@ -305,7 +314,7 @@ __device__ __forceinline__ double dmul_mem(const Work& w, int t, double v, int l
     double r[NX];
 #pragma unroll
     for (int j = 0; j < NX; ++j) r[j] = D[l >= j ? lrow + j : tri(j, 0) + l];
-    acc = gdot_chain<G, NX>(r, v, lane, 0, 0.0);
+    acc = gdot<G, NX>(r, v, lane, 0);
   } else {
     acc = gdot_mem<G, NX>(w.DT + (size_t)t * NX * NX + l, NX, v, lane);
   }
@@ -549,7 +558,7 @@ __device__ void stream_init_vectors(const Work& w, int Np, int N, const double* 
 // search, step test and outputs.  The block matrices were assembled by
 // stream_assemble_kernel for this linearization.
 template <class Model, int G, int NV>
-__global__ void __launch_bounds__(512) stream_qp_kernel(SolveArgs a) {
+__global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
   extern __shared__ double stream_vlds[];
@@ -646,7 +655,7 @@ __global__ void __launch_bounds__(512) stream_qp_kernel(SolveArgs a) {
 // with q = -dl/dx, r = -dl/du[:-1], C = 0 on the saved linearization, then
 // the closed-form contraction of the register variant's backward.
 template <class Model, int G, int NV>
-__global__ void __launch_bounds__(512) stream_bwd_kernel(BackwardArgs a) {
+__global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel(BackwardArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
   extern __shared__ double stream_vlds[];
@@ -756,10 +765,21 @@ __global__ void __launch_bounds__(512) stream_bwd_kernel(BackwardArgs a) {
 // ---------------------------------------------------------------------------
 // host side
 
-inline int stream_threads(int N, int G) {
+// Threads per workgroup: enough waves for one round over the N + 1 blocks,
+// capped at maxt.  Above 512 the rounds get longer (a round of 16 waves per
+// CU takes about 1.6 times as long as one of 8 waves, quadrotor N = 64..200),
+// so 512 threads are kept unless the wider workgroup cuts the rounds by more
+// than that: 512 up to N = 80 for 16-lane blocks, 1024 from N = 96.
+inline int stream_threads(int N, int G, int maxt) {
   const int want = (N + 1) * G;
-  const int t = ((want + 63) / 64) * 64;
-  return t < 64 ? 64 : (t > 512 ? 512 : t);
+  int t = ((want + 63) / 64) * 64;
+  t = t < 64 ? 64 : (t > maxt ? maxt : t);
+  if (t > 512) {
+    const int r512 = (N + 1 + 512 / G - 1) / (512 / G);
+    const int rt = (N + 1 + t / G - 1) / (t / G);
+    if (r512 * 10 <= rt * 16) t = 512;
+  }
+  return t;
 }
 
 template <class Model>
@@ -828,7 +848,7 @@ int stream_forward_t(int model, const SolveArgs& a, hipStream_t st) {
   constexpr int NX = Model::NX, NU = Model::NU;
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
   const AssembleArgs aa = assemble_args<Model>(a, lsd);
-  const int threads = stream_threads(a.N, G);
+  const int threads = stream_threads(a.N, G, stream_max_threads<Model>());
   const int nv = lds_vectors<Model, G>(a.N, true);
   const size_t lds = vector_lds_bytes(nv, NX, a.N);
   int rc = stream_launch_reset(a, st);
@@ -876,12 +896,12 @@ int stream_backward_t(const BackwardArgs& a, hipStream_t st) {
   const int nv = lds_vectors<Model, G>(a.N, false);
   const size_t lds = vector_lds_bytes(nv, NX, a.N);
   if (nv == kStreamVectors)
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(stream_threads(a.N, G)), lds, st,
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(stream_threads(a.N, G, stream_max_threads<Model>())), lds, st,
                        a);
   else if (nv == kHotVectors)
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(stream_threads(a.N, G)), lds, st, a);
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(stream_threads(a.N, G, stream_max_threads<Model>())), lds, st, a);
   else
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0>), dim3(a.B), dim3(stream_threads(a.N, G)), 0, st, a);
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0>), dim3(a.B), dim3(stream_threads(a.N, G, stream_max_threads<Model>())), 0, st, a);
   prof_end(PROF_BACKWARD, st);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
 }
