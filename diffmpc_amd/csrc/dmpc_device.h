@@ -286,6 +286,410 @@ __device__ __forceinline__ double gdot16x13f(const float* a, double v, double ac
   return acc;
 }
 
+// acc + sum_{j<M} a[j] * v_j for 8-lane groups, which sit two per 16-lane
+// row.  float64 DPP takes only row_newbcast (lane j of the row to all 16
+// lanes) and ignores bank masks, so the chain runs twice with EXEC narrowed
+// (ANDed with the caller's mask and restored in the same asm block): lanes
+// 0-7 of every row with row_newbcast:j, then lanes 8-15 with
+// row_newbcast:8+j.  One v_fmac_f64_dpp per term and lane half (two
+// instructions against five for gbcast's four 32-bit moves and an FMA); the
+// s_nop 4 after each EXEC write covers the SALU-EXEC-to-DPP wait states and
+// the first also the DPP source's read-after-write; SCC is clobbered.  float
+// a[] converts each term into a scratch pair first, as gdot16x13f does.  Same
+// terms, order and roundings as gdot_chain.
+constexpr unsigned long long kLowHalves = 0x00ff00ff00ff00ffull;  // lanes 0-7 of each 16-lane row
+
+template <int M>
+__device__ __forceinline__ double gdot8d_fused(const double* a, double v, double acc) {
+  unsigned long long save;
+  if constexpr (M == 1) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %4, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %4, %3 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save)
+        : "s"(kLowHalves), "v"(a[0]), "v"(v) : "scc");
+  } else if constexpr (M == 2) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %5, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %5, %4 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %5, %3 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %5, %4 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(v) : "scc");
+  } else if constexpr (M == 3) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %6, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %6, %4 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %6, %5 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %6, %3 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %6, %4 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %6, %5 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(v) : "scc");
+  } else if constexpr (M == 4) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %7, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %7, %4 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %7, %5 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %7, %6 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %7, %3 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %7, %4 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %7, %5 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %7, %6 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(v) : "scc");
+  } else if constexpr (M == 5) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %8, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %8, %4 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %8, %5 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %8, %6 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %8, %7 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %8, %3 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %8, %4 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %8, %5 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %8, %6 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %8, %7 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(v) : "scc");
+  } else if constexpr (M == 6) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %9, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %4 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %5 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %6 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %7 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %8 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %9, %3 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %4 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %5 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %6 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %7 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %9, %8 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(v) : "scc");
+  } else if constexpr (M == 7) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %10, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %4 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %5 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %6 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %7 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %8 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %9 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %10, %3 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %4 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %5 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %6 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %7 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %8 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %10, %9 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(v) : "scc");
+  } else if constexpr (M == 8) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %11, %3 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %4 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %5 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %6 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %7 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %8 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %9 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %10 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %2\n"
+        "s_nop 4\n"
+        "v_fmac_f64_dpp %0, %11, %3 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %4 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %5 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %6 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %7 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %8 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %9 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_fmac_f64_dpp %0, %11, %10 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]), "v"(v) : "scc");
+  }
+  return acc;
+}
+
+template <int M>
+__device__ __forceinline__ double gdot8f_fused(const float* a, double v, double acc) {
+  unsigned long long save;
+  double tmp;
+  if constexpr (M == 1) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %5, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %5, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save), "=&v"(tmp)
+        : "s"(kLowHalves), "v"(a[0]), "v"(v) : "scc");
+  } else if constexpr (M == 2) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %6, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %6, %2 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %6, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %6, %2 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save), "=&v"(tmp)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(v) : "scc");
+  } else if constexpr (M == 3) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %7, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %7, %2 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %7, %2 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %7, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %7, %2 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %7, %2 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save), "=&v"(tmp)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(v) : "scc");
+  } else if constexpr (M == 4) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %8, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %8, %2 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %8, %2 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %8, %2 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %8, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %8, %2 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %8, %2 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %8, %2 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save), "=&v"(tmp)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(v) : "scc");
+  } else if constexpr (M == 5) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %8\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %8\n"
+        "v_fmac_f64_dpp %0, %9, %2 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save), "=&v"(tmp)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(v) : "scc");
+  } else if constexpr (M == 6) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %8\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %9\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %8\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %9\n"
+        "v_fmac_f64_dpp %0, %10, %2 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save), "=&v"(tmp)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(v) : "scc");
+  } else if constexpr (M == 7) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %8\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %9\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %10\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %8\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %9\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %10\n"
+        "v_fmac_f64_dpp %0, %11, %2 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save), "=&v"(tmp)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(v) : "scc");
+  } else if constexpr (M == 8) {
+    asm volatile("s_mov_b64 %1, exec\n"
+        "s_and_b64 exec, exec, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %8\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %9\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %10\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %11\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+        "s_andn2_b64 exec, %1, %3\n"
+        "s_nop 4\n"
+        "v_cvt_f64_f32 %2, %4\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %5\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %6\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %7\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %8\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %9\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %10\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+        "v_cvt_f64_f32 %2, %11\n"
+        "v_fmac_f64_dpp %0, %12, %2 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+        "s_mov_b64 exec, %1\n"
+        "s_nop 4\n"
+        : "+v"(acc), "=&s"(save), "=&v"(tmp)
+        : "s"(kLowHalves), "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]), "v"(v) : "scc");
+  }
+  return acc;
+}
+
 // ---------------------------------------------------------------------------
 // B-term of a 16-lane group with four columns, by reduce-scatter and
 // all-gather instead of four group all-reduces.  Lane i of the group belongs
@@ -328,13 +732,19 @@ __device__ __forceinline__ double gdot(const AT* a, double v, int lane, int salt
     return gdot16x13(a, v, 0.0);
   } else if constexpr (G == 16 && M == 13 && std::is_same<AT, float>::value) {
     return gdot16x13f(a, v, 0.0);
+  } else if constexpr (G == 8 && M <= 8 && std::is_same<AT, double>::value) {
+    return gdot8d_fused<M>(a, v, 0.0);
+  } else if constexpr (G == 8 && M <= 8 && std::is_same<AT, float>::value) {
+    return gdot8f_fused<M>(a, v, 0.0);
   } else {
     return gdot_chain<G, M>(a, v, lane, salt, 0.0);
   }
 }
 template <int G, int M, int j = 0>
 __device__ __forceinline__ double gdotd(const double* a, double v, int lane, double acc = 0.0) {
-  if constexpr (j == M) {
+  if constexpr (j == 0 && G == 8 && M <= 8) {
+    return gdot8d_fused<M>(a, v, acc);
+  } else if constexpr (j == M) {
     return acc;
   } else {
     acc += a[j] * gbcast<G, j>(v, lane);
