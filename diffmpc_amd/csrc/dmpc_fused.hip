@@ -153,6 +153,49 @@ __device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, co
   }
 }
 
+// Row `lane` of theta_t = A_{t-1} Qinv_{t-1} A_{t-1}^T + B_{t-1} Rinv
+// B_{t-1}^T + Qinv_t for 16-lane groups of 13 states, from the lanes' own
+// rows of [A_{t-1} | B_{t-1}] (one load of the row per lane): term k of entry
+// j is A[j][k] * (A[lane][k] q_k), with A[j][k] read from lane j by DPP
+// row_newbcast:j inside the FMA.  k runs outermost, so each of the 13
+// accumulators receives the same products in the same order (A terms k =
+// 0..12, then the B terms) as the entry-by-entry form, which reads every
+// A[j][k] from memory instead (221 loads per lane and block).  Lanes >= nx
+// and blocks without a predecessor carry zero rows; masked lanes return the
+// identity row.
+template <int NU>
+__device__ __forceinline__ void theta_row16x13(const float* Ap, const double* qprev, const double* rinv, double qi,
+                                               bool has_prev, bool ok, int lane, double (&row)[13]) {
+  constexpr int NX = 13, NZ = NX + NU;
+  double acc[NX];
+#pragma unroll
+  for (int j = 0; j < NX; ++j) acc[j] = (j == lane) ? qi : 0.0;
+  const float* own = Ap + (size_t)(lane < NX ? lane : 0) * NZ;
+#define DMPC_TH(j) "v_fmac_f64_dpp %" #j ", %13, %14 row_newbcast:" #j " row_mask:0xf bank_mask:0xf\n"
+#define DMPC_TH_BLOCK(src, scale)                                                                            \
+  asm("s_nop 4\n" DMPC_TH(0) DMPC_TH(1) DMPC_TH(2) DMPC_TH(3) DMPC_TH(4) DMPC_TH(5) DMPC_TH(6) DMPC_TH(7)    \
+          DMPC_TH(8) DMPC_TH(9) DMPC_TH(10) DMPC_TH(11) DMPC_TH(12)                                          \
+      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]),     \
+        "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11]), "+v"(acc[12])                 \
+      : "v"(src), "v"(scale))
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    const double a = has_prev ? (double)own[k] : 0.0;
+    const double aq = a * qprev[k];
+    DMPC_TH_BLOCK(a, aq);
+  }
+#pragma unroll
+  for (int k = 0; k < NU; ++k) {
+    const double bk = has_prev ? (double)own[NX + k] : 0.0;
+    const double bq = bk * rinv[k];
+    DMPC_TH_BLOCK(bk, bq);
+  }
+#undef DMPC_TH_BLOCK
+#undef DMPC_TH
+#pragma unroll
+  for (int j = 0; j < NX; ++j) row[j] = ok ? acc[j] : ((j == lane) ? 1.0 : 0.0);
+}
+
 // theta_t row `lane`, its Cholesky factor and D_t = theta_t^-1 (row `lane`),
 // float64, inside the lane group of block t:
 //   theta_0 = Qinv_0,
@@ -172,7 +215,9 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* A
     const int tp = t >= 1 ? t - 1 : 0;
     const float* Ap = ABb + (size_t)tp * NX * NZ;  // [A_{t-1} | B_{t-1}], L1-cached broadcast reads
     double row[NX];
-    {
+    if constexpr (G == 16 && NX == 13) {
+      theta_row16x13<NU>(Ap, L.qinv + tp * NX, L.rinv, ok ? L.qinv[t * NX + lane] : 0.0, has_prev, ok, lane, row);
+    } else {
       double aq[NX], bq[NU];
 #pragma unroll
       for (int k = 0; k < NX; ++k) aq[k] = has_prev ? (double)Ap[lane * NZ + k] * L.qinv[tp * NX + k] : 0.0;
