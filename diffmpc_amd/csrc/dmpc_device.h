@@ -79,18 +79,35 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // adds the partials k, k + 16, k + 32, k + 48 and the row sums those with
 // the butterfly.  Every thread reads the same partials in the same order, so
 // every thread gets the same bits.  Fewer instructions than block_sum.
-__device__ __forceinline__ double block_sum_rows(double v, double* red) {
+// `red` must hold zeros past the workgroup's rows (the caller clears all 64
+// entries once; rows beyond blockDim.x / 16 are never written), so the four
+// partial reads need no EXEC mask.
+// The two halves are separate so that a caller can issue loads that only
+// need the barrier (neighbour vector entries) between them.
+__device__ __forceinline__ void block_sum_rows_publish(double v, double* red) {
   v = gsum<16>(v);
   if ((threadIdx.x & 15) == 0) red[threadIdx.x >> 4] = v;
   __syncthreads();
-  const int nr = blockDim.x >> 4;
+}
+__device__ __forceinline__ void block_sum_rows_load(const double* red, double (&q)[4]) {
+  const double* p = red + (threadIdx.x & 15);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) q[m] = p[16 * m];
+}
+__device__ __forceinline__ double block_sum_rows_sum(const double (&q)[4]) {
   double s = 0.0;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int k = (threadIdx.x & 15) + 16 * m;
-    s += k < nr ? red[k] : 0.0;
-  }
+  for (int m = 0; m < 4; ++m) s += q[m];
   return gsum<16>(s);
+}
+__device__ __forceinline__ double block_sum_rows_finish(const double* red) {
+  double q[4];
+  block_sum_rows_load(red, q);
+  return block_sum_rows_sum(q);
+}
+__device__ __forceinline__ double block_sum_rows(double v, double* red) {
+  block_sum_rows_publish(v, red);
+  return block_sum_rows_finish(red);
 }
 
 // a / b for the PCG step lengths: the hardware reciprocal refined by two

@@ -391,10 +391,14 @@ __device__ __forceinline__ void dmul_k(const Regs<NX, NU, K>& R, const Lds& L, c
   }
 }
 
-// Entry `lane` of block tb of an LDS vector (padded layout), 0 outside the block.
+// Entry `lane` of block tb of an LDS vector (padded layout).  Lanes >= nx
+// read entries of the next block (or, past the last padding block, the first
+// words of the next LDS array): every caller uses their values only as DPP
+// broadcast sources, which read lanes < nx, or multiplied by the lane's zero
+// matrix rows, so the read needs no EXEC mask and no address select.
 template <int NX>
 __device__ __forceinline__ double own(const double* V, int tb, int lane) {
-  return lane < NX ? V[(tb + 1) * NX + lane] : 0.0;
+  return V[(tb + 1) * NX + lane];
 }
 
 // B_{t-1} Rinv B_{t-1}^T v_t for block t from the lane's own entry v of v_t:
@@ -426,7 +430,9 @@ __device__ __forceinline__ double bgram(const Regs<NX, NU, K>& R, int s, const L
 // Block-Jacobi-preconditioned CG on S lam = gamma (pcg.py:350-445) with the
 // reference's stop rule: at least one iteration, then continue while
 // |eta| > eps and it <= max_iter - 1.  lam enters as the guess in L.lamS
-// (padded) and leaves in registers.  Returns the iteration count.
+// (padded), is updated there in place and is also returned in lam[] (the
+// caller synchronises before other lanes read L.lamS).  Returns the
+// iteration count.
 //
 // S = -G H^-1 G^T is applied matrix-free.  Row block t of G H^-1 G^T p is
 //   (t == 0 ? 1 : -1) Qinv_t w_t + A_{t-1} Qinv_{t-1} w_{t-1} + B_{t-1} Rinv B_{t-1}^T p_t
@@ -481,7 +487,17 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
       L.RT[(t + 1) * NX + lane] = rt[s];
     }
   }
-  double eta = block_sum_rows(part, L.redC);
+  // Each workgroup sum is split around its barrier: the loads of neighbour
+  // entries written before the barrier, and the products of those entries
+  // that do not need the sum (A_t^T rt_{t+1} here and at the end of phase C,
+  // A_{t-1} u_{t-1} in phase B), are issued between the barrier and the
+  // reading of the partials, so their LDS latency overlaps the sum's.
+  double rn[K], apn[K];
+  block_sum_rows_publish(part, L.redC);
+#pragma unroll
+  for (int s = 0; s < K; ++s) rn[s] = own<NX>(L.RT, R.t[s] + 1, lane);
+  gdot_k<G, NX>(R.acol, rn, lane, 0, apn);
+  double eta = block_sum_rows_finish(L.redC);
   double beta = 0.0;
   int it = 0;
   while (true) {
@@ -489,12 +505,9 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
     if (!cont) break;
     // ---- phase A: p = rt + beta p; w = G^T p; y = H^-1 w; v = -w.y --------
     // A_t^T p_{t+1} = A_t^T rt_{t+1} + beta A_t^T p_{t+1}(previous), so only
-    // the neighbour's rt is read; the previous product stays in registers.
+    // the neighbour's rt is read (apn, formed before beta was known); the
+    // previous product stays in registers.
     double vpart = 0.0;
-    double rn[K], apn[K];
-#pragma unroll
-    for (int s = 0; s < K; ++s) rn[s] = own<NX>(L.RT, R.t[s] + 1, lane);
-    gdot_k<G, NX>(R.acol, rn, lane, it, apn);
 #pragma unroll
     for (int s = 0; s < K; ++s) {
       const int t = R.t[s];
@@ -517,14 +530,15 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
         }
       }
     }
-    const double v = -block_sum_rows(vpart, L.redA);
-    const double alpha = div_newton(eta, v);
+    block_sum_rows_publish(vpart, L.redA);
     // ---- phase B: ups = S p; lam += alpha p; r -= alpha ups; y = D r -----
     {
       double u[K], au[K], y[K];
 #pragma unroll
       for (int s = 0; s < K; ++s) u[s] = own<NX>(L.YX, R.t[s] - 1, lane);
       gdot_k<G, NX>(R.arow, u, lane, it, au);
+      const double v = -block_sum_rows_finish(L.redA);
+      const double alpha = div_newton(eta, v);
 #pragma unroll
       for (int s = 0; s < K; ++s) {
         const double g = (R.t[s] == 0 ? 1.0 : -1.0) * yxo[s] + au[s] + gb[s];
@@ -565,11 +579,19 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
         L.RT[(R.t[s] + 1) * NX + lane] = rt[s];
       }
     }
-    const double eta_new = block_sum_rows(epart, L.redB);
+    block_sum_rows_publish(epart, L.redB);
+    // next iteration's A_t^T rt_{t+1} (unused if the loop ends here)
+#pragma unroll
+    for (int s = 0; s < K; ++s) rn[s] = own<NX>(L.RT, R.t[s] + 1, lane);
+    gdot_k<G, NX>(R.acol, rn, lane, it, apn);
+    const double eta_new = block_sum_rows_finish(L.redB);
     beta = div_newton(eta_new, eta);
     eta = eta_new;
     ++it;
   }
+#pragma unroll
+  for (int s = 0; s < K; ++s)
+    if (R.ok[s]) L.lamS[(R.t[s] + 1) * NX + lane] = lam[s];
   *eta_out = eta;
   return it;
 }
@@ -670,6 +692,10 @@ __device__ void zero_lds_vectors(const Lds& L, int N) {
   for (int k = threadIdx.x; k < NBP * NX; k += blockDim.x) {
     L.RT[k] = 0.0; L.Y[k] = 0.0; L.YQ[k] = 0.0; L.YX[k] = 0.0; L.lamS[k] = 0.0;
   }
+  // block_sum_rows reads all 64 partials of each array
+  for (int k = threadIdx.x; k < 64; k += blockDim.x) {
+    L.redA[k] = 0.0; L.redB[k] = 0.0; L.redC[k] = 0.0;
+  }
 }
 
 // Closed-form quadratic model of the diagonal cost (sqp.py:160-197,
@@ -711,6 +737,84 @@ __device__ void cost_terms(const Lds& L, int N, int NBs, int b, const double* wx
 __global__ void sqp_reset_kernel(int B, int32_t* num_iter) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < B) num_iter[b] = 0;
+}
+
+// Longest-first launch order for the next QP launch.  The hardware hands
+// workgroups to CUs in index order as CUs free up, so when several instances
+// take turns on a CU the launch ends with whichever long instances started
+// last.  The work of instance c is estimated as (N_c + 1) blocks times the
+// PCG iterations of its previous SQP iteration (times 1 in the first, where
+// only ragged horizons tell instances apart); finished instances get -1
+// (they return at once).  order lists the instances by decreasing estimate,
+// ties by index (a stable sort, so the result is deterministic).  Each
+// workgroup stages every key in LDS and ranks kOrderPerGroup instances, 16
+// threads per instance summing the comparisons of a sixteenth of the keys
+// each.
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderPerGroup = kOrderThreads / 16;
+constexpr int kOrderMaxBatch = 32768;  // keys staged in LDS: 128 KB
+
+__device__ __forceinline__ int order_key(const SolveArgs& a, int c) {
+  if (!sqp_active(a, c)) return -1;
+  const int ic = a.num_iter[c];
+  const long long n = horizon_of(a.horizons, a.N, c) + 1;
+  const long long k = ic < 1 ? n : (long long)a.pcg_iters[(size_t)c * a.maxit + ic - 1] * n;
+  return k > 0x7fffffffLL ? 0x7fffffff : (int)k;
+}
+
+__global__ void __launch_bounds__(kOrderThreads) order_kernel(SolveArgs a, int32_t* order) {
+  extern __shared__ int keys_[];
+  __shared__ int rank[kOrderPerGroup];
+  for (int c = threadIdx.x; c < a.B; c += blockDim.x) keys_[c] = order_key(a, c);
+  if (threadIdx.x < kOrderPerGroup) rank[threadIdx.x] = 0;
+  __syncthreads();
+  const int b = blockIdx.x * kOrderPerGroup + (threadIdx.x >> 4);
+  const int part = threadIdx.x & 15;
+  if (b < a.B) {
+    const int kb = keys_[b];
+    const int chunk = (a.B + 15) / 16;
+    const int lo = part * chunk, hi = min(a.B, lo + chunk);
+    int r = 0;
+    for (int c = lo; c < hi; ++c) {
+      const int kc = keys_[c];
+      r += (kc > kb) || (kc == kb && c < b);
+    }
+    atomicAdd(&rank[threadIdx.x >> 4], r);
+  }
+  __syncthreads();
+  if (part == 0 && b < a.B) order[rank[threadIdx.x >> 4]] = b;
+}
+
+// Whether ordering pays: more instances than CUs (then instances share CUs
+// in turn), and few enough for the keys to fit in LDS.
+int order_grid(int B) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 1 << 30;  // unknown: never order
+  }
+  if (B <= cus || B > kOrderMaxBatch) return 0;
+  return (B + kOrderPerGroup - 1) / kOrderPerGroup;
+}
+
+// Enqueues the order kernel for QP launch `it` into `order`, or returns
+// false when the identity order is kept (also for the first launch of a
+// batch with one horizon, whose keys are all equal).
+bool launch_order(const SolveArgs& a, int it, int32_t* order, hipStream_t st) {
+  if ((it < 1 && !a.horizons) || !order) return false;
+  const int grid = order_grid(a.B);
+  if (grid == 0) return false;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&order_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kOrderMaxBatch * (int)sizeof(int)) != hipSuccess)
+      return false;
+    attr = true;
+  }
+  hipLaunchKernelGGL(order_kernel, dim3(grid), dim3(kOrderThreads), (size_t)a.B * sizeof(int), st, a, order);
+  return true;
 }
 
 // Column j of [A_t B_t], one thread per (b, t, j), then f_d(x_t, u_t), one
@@ -768,7 +872,7 @@ template <class Model, int G, int K, int MAXT>
 __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   extern __shared__ double smem_[];
-  const int b = blockIdx.x;
+  const int b = a.order ? a.order[blockIdx.x] : blockIdx.x;
   if (!sqp_active(a, b)) return;
   const int it = a.num_iter[b];
   const int Np = a.N, NBs = Np + 1;                          // padded horizon, knot stride
@@ -797,10 +901,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   gamma_rows<true, G>(R, L, lane, gam);
   double eta;
   const int npcg = pcg<G>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta);
-#pragma unroll
-  for (int s = 0; s < K; ++s)
-    if (R.ok[s]) L.lamS[(R.t[s] + 1) * NX + lane] = lam[s];
-  __syncthreads();
+  __syncthreads();  // lam (in L.lamS) complete for the neighbour reads
   double xn[K], un[K];
   recover_rows<G>(R, L, N, lane, xn, un);
   double alpha = 1.0;
@@ -924,10 +1025,7 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   gamma_rows<false, G>(R, L, lane, gam);
   double eta;
   const int npcg = pcg<G>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta);
-#pragma unroll
-  for (int s = 0; s < K; ++s)
-    if (R.ok[s]) L.lamS[(R.t[s] + 1) * NX + lane] = lam[s];
-  __syncthreads();
+  __syncthreads();  // lam (in L.lamS) complete for the neighbour reads
   double xt[K], ut[K];
   recover_rows<G>(R, L, N, lane, xt, ut);
   // per-row contraction terms -> LDS (reusing the PCG vectors), then one
@@ -1056,8 +1154,11 @@ int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
     prof_begin(PROF_LINEARIZE, st);
     hipLaunchKernelGGL(linearize_kernel<Model>, dim3((unsigned)((nlin + 255) / 256)), dim3(256), 0, st, a);
     prof_end(PROF_LINEARIZE, st);
+    // the solution's status array is free until sqp_finish_kernel writes it
+    SolveArgs ai = a;
+    ai.order = launch_order(a, it, a.status, st) ? a.status : nullptr;
     prof_begin(PROF_QP_ITERATION, st);
-    hipLaunchKernelGGL((qp_iteration_kernel<Model, G, K, MAXT>), dim3(a.B), dim3(l.threads), l.lds, st, a);
+    hipLaunchKernelGGL((qp_iteration_kernel<Model, G, K, MAXT>), dim3(a.B), dim3(l.threads), l.lds, st, ai);
     prof_end(PROF_QP_ITERATION, st);
   }
   hipLaunchKernelGGL(sqp_finish_kernel, dim3(a.B), dim3(256), 0, st, a, Model::NX, Model::NU);
@@ -1162,6 +1263,11 @@ size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_al
 }
 
 // launches the streamed variant shares with the register-resident one
+const int32_t* stream_launch_order(const SolveArgs& a, int it, hipStream_t st) {
+  // the solution's status array is free until sqp_finish_kernel writes it
+  return launch_order(a, it, a.status, st) ? a.status : nullptr;
+}
+
 int stream_launch_reset(const SolveArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;

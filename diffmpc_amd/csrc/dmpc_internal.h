@@ -44,6 +44,9 @@ struct SolveArgs {
   // workspace of the streamed variant: work_stride bytes per instance
   char* work;
   size_t work_stride;
+  // launch order of the per-instance workgroups of one QP launch (workgroup
+  // i solves instance order[i]), or null for the identity
+  const int32_t* order;
 };
 
 struct BackwardArgs {

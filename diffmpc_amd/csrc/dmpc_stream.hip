@@ -562,7 +562,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
   extern __shared__ double stream_vlds[];
-  const int b = blockIdx.x;
+  const int b = a.order ? a.order[blockIdx.x] : blockIdx.x;
   if (!sqp_active(a, b)) return;
   const int it = a.num_iter[b];
   const int Np = a.N, NBs = Np + 1;
@@ -812,6 +812,7 @@ size_t stream_work_stride(int nx, int nu, int N, int ls_eval, int n_alphas) {
 // variant (dmpc_fused.hip)
 int stream_launch_linearize(int model, const SolveArgs& a, hipStream_t st);
 int stream_launch_reset(const SolveArgs& a, hipStream_t st);
+const int32_t* stream_launch_order(const SolveArgs& a, int it, hipStream_t st);
 int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st);
 
 // How many PCG vectors the streamed kernels keep in LDS for padded horizon N:
@@ -856,15 +857,17 @@ int stream_forward_t(int model, const SolveArgs& a, hipStream_t st) {
   for (int it = 0; it < a.maxit; ++it) {
     rc = stream_launch_linearize(model, a, st);
     if (rc) return rc;
+    SolveArgs ai = a;
+    ai.order = stream_launch_order(a, it, st);
     prof_begin(PROF_QP_ITERATION, st);
     hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0,
                        st, aa);
     if (nv == kStreamVectors)
-      hipLaunchKernelGGL((stream_qp_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(threads), lds, st, a);
+      hipLaunchKernelGGL((stream_qp_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(threads), lds, st, ai);
     else if (nv == kHotVectors)
-      hipLaunchKernelGGL((stream_qp_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(threads), lds, st, a);
+      hipLaunchKernelGGL((stream_qp_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(threads), lds, st, ai);
     else
-      hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0>), dim3(a.B), dim3(threads), 0, st, a);
+      hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0>), dim3(a.B), dim3(threads), 0, st, ai);
     prof_end(PROF_QP_ITERATION, st);
   }
   rc = stream_launch_finish(model, a, st);

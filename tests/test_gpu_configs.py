@@ -204,3 +204,36 @@ def test_psd_floor_branch_matches_oracle(cuda, variant):
             e = rel(_np(gk)[b], g[ok][b])
             say(f"    d/d{ok}: {e:.1e}")
             assert e <= 1e-3, (k, e)
+
+
+@pytest.mark.parametrize("variant,hmin,hmax", [("register", 20, 63), ("streamed", 64, 120)])
+def test_longest_first_launch_order_keeps_every_result(cuda, variant, hmin, hmax):
+    """With more instances than CUs each QP launch hands the instances to its
+    workgroups longest-first (estimated from the horizons, then from the
+    previous SQP iteration's PCG counts).  Instances are independent, so 600
+    ragged quadrotor instances solved as one batch equal, bit for bit, the
+    same instances solved in batches of 200 (at most one instance per CU,
+    launched in index order): trajectories, multipliers, PCG counts and the
+    gradient w.r.t. the state weights."""
+    say(f"[launch order] quadrotor {variant}, 600 instances, N ~ U{{{hmin}..{hmax}}}")
+    name, prob, sp = W.quadrotor(batch=600, seed=5, horizon=hmax, sqp_iters=3)
+    hs = np.random.default_rng(5).integers(hmin, hmax + 1, size=600).astype(np.int32)
+    key = "weights_penalization_reference_state_trajectory"
+
+    def run(idx):
+        sub = W.slice_problem(prob, idx)
+        solver, params, guess = W.make_solver(name, sub, sp, device=cuda, horizons=hs[idx])
+        wx = params[key].clone().requires_grad_(True)
+        with _variant(1 if variant == "register" else 2):
+            sol = solver.solve(guess, params, {key: wx})
+            loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+            (g,) = torch.autograd.grad(loss, [wx])
+        torch.cuda.synchronize()
+        return [_np(sol.states), _np(sol.controls), _np(sol.dual), _np(sol.num_pcg_iters), _np(g)]
+
+    whole = run(np.arange(600))
+    parts = [run(np.arange(k, k + 200)) for k in (0, 200, 400)]
+    for j, a in enumerate(whole):
+        b = np.concatenate([p[j] for p in parts])
+        assert np.array_equal(a, b), j
+    say("  identical")
