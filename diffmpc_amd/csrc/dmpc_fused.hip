@@ -141,14 +141,24 @@ __device__ __forceinline__ void load_ab(Regs<NX, NU, K>& R, const float* ABb, co
   for (int s = 0; s < K; ++s) {
     const int t = R.t[s];
     const bool ok = R.ok[s];
-    const float* At = ABb + (size_t)t * NX * NZ;
-    const float* Ap = ABb + (size_t)(t - 1) * NX * NZ;
+    // clamped to blocks and rows that exist, so that every load is issued
+    // unconditionally in one batch; lanes without the block discard them
+    const int li = lane < NX ? lane : 0;
+    const float* At = ABb + (size_t)(t < N ? t : N - 1) * NX * NZ;
+    const float* Ap = ABb + (size_t)(t >= 1 ? t - 1 : 0) * NX * NZ;
+    float ac[NX], ar[NX], br[NU];
 #pragma unroll
-    for (int j = 0; j < NX; ++j) R.acol[s][j] = (ok && t < N) ? At[j * NZ + lane] : 0.0f;
+    for (int j = 0; j < NX; ++j) ac[j] = At[j * NZ + li];
 #pragma unroll
-    for (int k = 0; k < NX; ++k) R.arow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + k] : 0.0f;
+    for (int k = 0; k < NX; ++k) ar[k] = Ap[li * NZ + k];
 #pragma unroll
-    for (int k = 0; k < NU; ++k) R.brow[s][k] = (ok && t >= 1) ? Ap[lane * NZ + NX + (k ^ bq)] : 0.0f;
+    for (int k = 0; k < NU; ++k) br[k] = Ap[li * NZ + NX + (k ^ bq)];
+#pragma unroll
+    for (int j = 0; j < NX; ++j) R.acol[s][j] = (ok && t < N) ? ac[j] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) R.arow[s][k] = (ok && t >= 1) ? ar[k] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < NU; ++k) R.brow[s][k] = (ok && t >= 1) ? br[k] : 0.0f;
     R.qi[s] = ok ? L.qinv[t * NX + lane] : 0.0;
   }
 }
@@ -170,7 +180,12 @@ __device__ __forceinline__ void theta_row16x13(const float* Ap, const double* qp
   double acc[NX];
 #pragma unroll
   for (int j = 0; j < NX; ++j) acc[j] = (j == lane) ? qi : 0.0;
+  // the whole row in one batch of loads (Ap is a valid block for every lane;
+  // lanes without a predecessor discard the values)
   const float* own = Ap + (size_t)(lane < NX ? lane : 0) * NZ;
+  float ab[NZ];
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) ab[k] = own[k];
 #define DMPC_TH(j) "v_fmac_f64_dpp %" #j ", %13, %14 row_newbcast:" #j " row_mask:0xf bank_mask:0xf\n"
 #define DMPC_TH_BLOCK(src, scale)                                                                            \
   asm("s_nop 4\n" DMPC_TH(0) DMPC_TH(1) DMPC_TH(2) DMPC_TH(3) DMPC_TH(4) DMPC_TH(5) DMPC_TH(6) DMPC_TH(7)    \
@@ -180,13 +195,13 @@ __device__ __forceinline__ void theta_row16x13(const float* Ap, const double* qp
       : "v"(src), "v"(scale))
 #pragma unroll
   for (int k = 0; k < NX; ++k) {
-    const double a = has_prev ? (double)own[k] : 0.0;
+    const double a = has_prev ? (double)ab[k] : 0.0;
     const double aq = a * qprev[k];
     DMPC_TH_BLOCK(a, aq);
   }
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
-    const double bk = has_prev ? (double)own[NX + k] : 0.0;
+    const double bk = has_prev ? (double)ab[NX + k] : 0.0;
     const double bq = bk * rinv[k];
     DMPC_TH_BLOCK(bk, bq);
   }
@@ -698,31 +713,47 @@ __device__ void zero_lds_vectors(const Lds& L, int N) {
   }
 }
 
-// Closed-form quadratic model of the diagonal cost (sqp.py:160-197,
+// Staging of one instance for the forward QP: iterate, multipliers, f_d,
+// x0 and the closed-form quadratic model of the diagonal cost (sqp.py:160-197,
 // optimal_control_problem.py:249-323): Qinv = 1 / max(2 w, 1e-9),
 // q = -2 w ref (terminal: w_x + w_f and final_state), r = -2 w_u u_ref.
+// One pass over the knots in which each thread issues all its global loads
+// at once (addresses clamped into the arrays, loads unconditional), then
+// stores to LDS.  N: this instance's horizon; Np: the padded one.
 template <int NX, int NU>
-__device__ void cost_terms(const Lds& L, int N, int NBs, int b, const double* wx, const double* wu, const double* wf,
-                           const double* xref, const double* uref, const double* xf, int penalize) {
-  // N: this instance's horizon; NBs: knot stride of the global arrays
-  const int NB = N + 1;
-  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
+__device__ void stage_forward(const SolveArgs& a, const Lds& L, int b, int N, int Np) {
+  const int NB = N + 1, NBs = Np + 1;
+  const int nxe = NB * NX, nue = NB * NU, nfd = N * NX;
+  for (int k = threadIdx.x; k < nxe; k += blockDim.x) {
     const int t = k / NX, i = k % NX;
-    const double w = wx[b * NX + i];
+    const int ku = k < nue ? k : nue - 1, kf = k < nfd ? k : nfd - 1;
+    const int tu = ku / NU, iu = ku % NU;
+    const size_t xo = (size_t)b * NBs * NX + k;
+    const double xv = a.x[xo], lv = a.lam[xo];
+    const double uv = a.u[(size_t)b * NBs * NU + ku];
+    const double fv = a.fd[(size_t)b * Np * NX + kf];
+    const double w = a.wx[b * NX + i], wfi = a.wf[b * NX + i], xfi = a.xf[b * NX + i], x0i = a.x0[b * NX + i];
+    const double xr = a.xref[((size_t)b * NBs + t) * NX + i];
+    const double wu = a.wu[b * NU + iu];
+    const double ur = a.uref[((size_t)b * NBs + tu) * NU + iu];
+    L.xb[k] = xv;
+    L.lamS[NX + k] = lv;
     if (t < N) {
       L.qinv[k] = 1.0 / fmax(2.0 * w, kPsdMinEig);
-      L.qv[k] = -2.0 * w * xref[((size_t)b * NBs + t) * NX + i];
+      L.qv[k] = -2.0 * w * xr;
     } else {
-      const double wt = w + wf[b * NX + i];
+      const double wt = w + wfi;
       L.qinv[k] = 1.0 / fmax(2.0 * wt, kPsdMinEig);
-      L.qv[k] = -2.0 * wt * xf[b * NX + i];
+      L.qv[k] = -2.0 * wt * xfi;
     }
+    if (k < nue) {
+      L.ub[k] = uv;
+      L.rv[k] = (tu < N && a.penalize) ? -2.0 * wu * ur : 0.0;
+    }
+    if (k < nfd) L.fd[k] = fv;
+    if (k < NX) L.x0[k] = x0i;
+    if (k < NU) L.rinv[k] = 1.0 / fmax(2.0 * wu, kPsdMinEig);  // iu == k here
   }
-  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) {
-    const int t = k / NU, i = k % NU;
-    L.rv[k] = (t < N && penalize) ? -2.0 * wu[b * NU + i] * uref[((size_t)b * NBs + t) * NU + i] : 0.0;
-  }
-  for (int k = threadIdx.x; k < NU; k += blockDim.x) L.rinv[k] = 1.0 / fmax(2.0 * wu[b * NU + k], kPsdMinEig);
 }
 
 // ---------------------------------------------------------------------------
@@ -884,14 +915,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
 
   zero_lds_vectors<NX, NU>(L, Np);
   __syncthreads();
-  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
-    L.xb[k] = a.x[(size_t)b * NBs * NX + k];
-    L.lamS[NX + k] = a.lam[(size_t)b * NBs * NX + k];
-  }
-  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) L.ub[k] = a.u[(size_t)b * NBs * NU + k];
-  for (int k = threadIdx.x; k < N * NX; k += blockDim.x) L.fd[k] = a.fd[(size_t)b * Np * NX + k];
-  for (int k = threadIdx.x; k < NX; k += blockDim.x) L.x0[k] = a.x0[b * NX + k];
-  cost_terms<NX, NU>(L, N, NBs, b, a.wx, a.wu, a.wf, a.xref, a.uref, a.xf, a.penalize);
+  stage_forward<NX, NU>(a, L, b, N, Np);
   __syncthreads();
 
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
@@ -1005,18 +1029,24 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
 
   zero_lds_vectors<NX, NU>(L, Np);
   __syncthreads();
+  // one pass, all global loads of a thread issued at once (clamped
+  // addresses), as stage_forward
   for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
-    L.qinv[k] = a.qinv[(size_t)b * NBs * NX + k];
-    L.qv[k] = -a.dl_dx[(size_t)b * NBs * NX + k];
-    L.lamS[NX + k] = a.lam_bwd[(size_t)b * NBs * NX + k];
-    L.xb[k] = a.x[(size_t)b * NBs * NX + k];
+    const int ku = k < NB * NU ? k : NB * NU - 1;
+    const size_t xo = (size_t)b * NBs * NX + k, uo = (size_t)b * NBs * NU + ku;
+    const double qi = a.qinv[xo], dx = a.dl_dx[xo], lb = a.lam_bwd[xo], xv = a.x[xo];
+    const double du = a.dl_du[uo], uv = a.u[uo];
+    const double ri = a.rinv[(size_t)b * NU + (ku < NU ? ku : 0)];
+    L.qinv[k] = qi;
+    L.qv[k] = -dx;
+    L.lamS[NX + k] = lb;
+    L.xb[k] = xv;
+    if (k < NB * NU) {
+      L.rv[k] = ku / NU < N ? -du : 0.0;
+      L.ub[k] = uv;
+    }
+    if (k < NU) L.rinv[k] = ri;
   }
-  for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) {
-    const int t = k / NU;
-    L.rv[k] = t < N ? -a.dl_du[(size_t)b * NBs * NU + k] : 0.0;
-    L.ub[k] = a.u[(size_t)b * NBs * NU + k];
-  }
-  for (int k = threadIdx.x; k < NU; k += blockDim.x) L.rinv[k] = a.rinv[(size_t)b * NU + k];
   __syncthreads();
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
   assemble_dinv<G>(R, ABb, L, lane, gbase);
