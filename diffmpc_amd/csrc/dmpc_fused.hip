@@ -785,26 +785,40 @@ constexpr int kOrderThreads = 1024;
 constexpr int kOrderPerGroup = kOrderThreads / 16;
 constexpr int kOrderMaxBatch = 32768;  // keys staged in LDS: 128 KB
 
-__device__ __forceinline__ int order_key(const SolveArgs& a, int c) {
-  if (!sqp_active(a, c)) return -1;
-  const int ic = a.num_iter[c];
-  const long long n = horizon_of(a.horizons, a.N, c) + 1;
-  const long long k = ic < 1 ? n : (long long)a.pcg_iters[(size_t)c * a.maxit + ic - 1] * n;
-  return k > 0x7fffffffLL ? 0x7fffffff : (int)k;
-}
+// Work estimate of instance c before QP launch it of a solve.
+struct SolveOrderKeys {
+  SolveArgs a;
+  __device__ int operator()(int c) const {
+    if (!sqp_active(a, c)) return -1;
+    const int ic = a.num_iter[c];
+    const long long n = horizon_of(a.horizons, a.N, c) + 1;
+    const long long k = ic < 1 ? n : (long long)a.pcg_iters[(size_t)c * a.maxit + ic - 1] * n;
+    return k > 0x7fffffffLL ? 0x7fffffff : (int)k;
+  }
+};
 
-__global__ void __launch_bounds__(kOrderThreads) order_kernel(SolveArgs a, int32_t* order) {
+// Work estimate of instance c of an adjoint solve: its blocks (the PCG
+// iterations are not known yet).
+struct HorizonOrderKeys {
+  const int32_t* horizons;
+  int N;
+  __device__ int operator()(int c) const { return horizon_of(horizons, N, c) + 1; }
+};
+
+// order[i * stride] = instance of workgroup i.
+template <class Keys>
+__global__ void __launch_bounds__(kOrderThreads) order_kernel(Keys keys, int B, int32_t* order, size_t stride) {
   extern __shared__ int keys_[];
   __shared__ int rank[kOrderPerGroup];
-  for (int c = threadIdx.x; c < a.B; c += blockDim.x) keys_[c] = order_key(a, c);
+  for (int c = threadIdx.x; c < B; c += blockDim.x) keys_[c] = keys(c);
   if (threadIdx.x < kOrderPerGroup) rank[threadIdx.x] = 0;
   __syncthreads();
   const int b = blockIdx.x * kOrderPerGroup + (threadIdx.x >> 4);
   const int part = threadIdx.x & 15;
-  if (b < a.B) {
+  if (b < B) {
     const int kb = keys_[b];
-    const int chunk = (a.B + 15) / 16;
-    const int lo = part * chunk, hi = min(a.B, lo + chunk);
+    const int chunk = (B + 15) / 16;
+    const int lo = part * chunk, hi = min(B, lo + chunk);
     int r = 0;
     for (int c = lo; c < hi; ++c) {
       const int kc = keys_[c];
@@ -813,7 +827,7 @@ __global__ void __launch_bounds__(kOrderThreads) order_kernel(SolveArgs a, int32
     atomicAdd(&rank[threadIdx.x >> 4], r);
   }
   __syncthreads();
-  if (part == 0 && b < a.B) order[rank[threadIdx.x >> 4]] = b;
+  if (part == 0 && b < B) order[(size_t)rank[threadIdx.x >> 4] * stride] = b;
 }
 
 // Whether ordering pays: more instances than CUs (then instances share CUs
@@ -830,22 +844,28 @@ int order_grid(int B) {
   return (B + kOrderPerGroup - 1) / kOrderPerGroup;
 }
 
+template <class Keys>
+bool enqueue_order(const Keys& keys, int B, int32_t* order, size_t stride, hipStream_t st) {
+  const int grid = order_grid(B);
+  if (grid == 0 || !order) return false;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&order_kernel<Keys>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kOrderMaxBatch * (int)sizeof(int)) != hipSuccess)
+      return false;
+    attr = true;
+  }
+  hipLaunchKernelGGL(order_kernel<Keys>, dim3(grid), dim3(kOrderThreads), (size_t)B * sizeof(int), st, keys, B, order,
+                     stride);
+  return true;
+}
+
 // Enqueues the order kernel for QP launch `it` into `order`, or returns
 // false when the identity order is kept (also for the first launch of a
 // batch with one horizon, whose keys are all equal).
 bool launch_order(const SolveArgs& a, int it, int32_t* order, hipStream_t st) {
-  if ((it < 1 && !a.horizons) || !order) return false;
-  const int grid = order_grid(a.B);
-  if (grid == 0) return false;
-  static bool attr = false;
-  if (!attr) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&order_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            kOrderMaxBatch * (int)sizeof(int)) != hipSuccess)
-      return false;
-    attr = true;
-  }
-  hipLaunchKernelGGL(order_kernel, dim3(grid), dim3(kOrderThreads), (size_t)a.B * sizeof(int), st, a, order);
-  return true;
+  if (it < 1 && !a.horizons) return false;
+  return enqueue_order(SolveOrderKeys{a}, a.B, order, 1, st);
 }
 
 // Column j of [A_t B_t], one thread per (b, t, j), then f_d(x_t, u_t), one
@@ -1296,6 +1316,13 @@ size_t fused_work_stride(int model, int nx, int nu, int N, int ls_eval, int n_al
 const int32_t* stream_launch_order(const SolveArgs& a, int it, hipStream_t st) {
   // the solution's status array is free until sqp_finish_kernel writes it
   return launch_order(a, it, a.status, st) ? a.status : nullptr;
+}
+
+// Longest-first order of an adjoint launch with ragged horizons, into
+// order[i * stride]; false (identity order) for a batch with one horizon.
+bool stream_backward_order(const BackwardArgs& a, int32_t* order, size_t stride, hipStream_t st) {
+  if (!a.horizons) return false;
+  return enqueue_order(HorizonOrderKeys{a.horizons, a.N}, a.B, order, stride, st);
 }
 
 int stream_launch_reset(const SolveArgs& a, hipStream_t st) {

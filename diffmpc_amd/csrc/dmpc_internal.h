@@ -65,6 +65,10 @@ struct BackwardArgs {
   int32_t* pcg_iters;                  // (B,)
   char* work;                          // streamed variant workspace
   size_t work_stride;
+  // launch order: workgroup i solves instance order[i * order_stride], or
+  // null for the identity
+  const int32_t* order;
+  size_t order_stride;
 };
 
 // SQP continue rule of sqp.py:460-466 for instance b:

@@ -50,6 +50,8 @@ __host__ __device__ constexpr bool compact_blocks(int nx) { return nx <= 16; }
 // both ends), D_t = theta_t^-1 (N+1 blocks, packed or full), 1/L_kk (N+1,
 // nx), the line-search step and scratch, and (large blocks) AT (N, nx+nu, nx)
 // float.  Each array starts on 256 bytes.
+constexpr size_t kOrderSlot = 256;
+
 struct Work {
   double* v[kStreamVectors];
   double *DT, *ID, *DX, *DU, *LS;
@@ -75,6 +77,10 @@ __host__ __device__ inline size_t carve_work(char* base, int nx, int nu, int N, 
   off += align256((size_t)ls_doubles * 8);
   if (w) w->AT = reinterpret_cast<float*>(base + off);
   if (!compact_blocks(nx)) off += align256((size_t)N * (nx + nu) * nx * 4);
+  // last kOrderSlot bytes: one int of a launch order spread over the
+  // instances' regions (entry i in region i), which no kernel of the
+  // instance writes
+  off += kOrderSlot;
   return off;
 }
 
@@ -659,7 +665,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
   extern __shared__ double stream_vlds[];
-  const int b = blockIdx.x;
+  const int b = a.order ? a.order[blockIdx.x * a.order_stride] : blockIdx.x;
   const int Np = a.N, NBs = Np + 1;
   const int N = horizon_of(a.horizons, Np, b), NB = N + 1;
   Work w;
@@ -813,6 +819,7 @@ size_t stream_work_stride(int nx, int nu, int N, int ls_eval, int n_alphas) {
 int stream_launch_linearize(int model, const SolveArgs& a, hipStream_t st);
 int stream_launch_reset(const SolveArgs& a, hipStream_t st);
 const int32_t* stream_launch_order(const SolveArgs& a, int it, hipStream_t st);
+bool stream_backward_order(const BackwardArgs& a, int32_t* order, size_t stride, hipStream_t st);
 int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st);
 
 // How many PCG vectors the streamed kernels keep in LDS for padded horizon N:
@@ -893,18 +900,25 @@ int stream_backward_t(const BackwardArgs& a, hipStream_t st) {
   aa.work = a.work;
   aa.work_stride = a.work_stride;
   aa.ls_doubles = 0;
+  // ragged batches: longest-first order in the instances' order slots
+  BackwardArgs ab = a;
+  int32_t* slots = reinterpret_cast<int32_t*>(a.work + a.work_stride - kOrderSlot);
+  if (stream_backward_order(a, slots, a.work_stride / sizeof(int32_t), st)) {
+    ab.order = slots;
+    ab.order_stride = a.work_stride / sizeof(int32_t);
+  }
   prof_begin(PROF_BACKWARD, st);
   hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0, st,
                      aa);
   const int nv = lds_vectors<Model, G>(a.N, false);
   const size_t lds = vector_lds_bytes(nv, NX, a.N);
+  const int threads = stream_threads(a.N, G, stream_max_threads<Model>());
   if (nv == kStreamVectors)
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(stream_threads(a.N, G, stream_max_threads<Model>())), lds, st,
-                       a);
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(threads), lds, st, ab);
   else if (nv == kHotVectors)
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(stream_threads(a.N, G, stream_max_threads<Model>())), lds, st, a);
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(threads), lds, st, ab);
   else
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0>), dim3(a.B), dim3(stream_threads(a.N, G, stream_max_threads<Model>())), 0, st, a);
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0>), dim3(a.B), dim3(threads), 0, st, ab);
   prof_end(PROF_BACKWARD, st);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
 }
