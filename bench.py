@@ -23,8 +23,8 @@ METRIC = "SQP solves/s & fwd+bwd grads/s, quadrotor T=50 B=1024, fused vs plain 
 MI355X_FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per qp_iteration_kernel launch at the default workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scale factors
-# from a 256 MiB device copy): 102.7 MB read + 77.3 MB written
-QP_TRAFFIC_BYTES = 180.0e6
+# from a 256 MiB device copy): 115.1 MB read + 95.6 MB written
+QP_TRAFFIC_BYTES = 210.7e6
 QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
 # HBM bytes per forward SQP iteration of the chain workload (stream_assemble_kernel
 # + stream_qp_kernel, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0): 16.90 TB read + 0.77 TB written
@@ -32,8 +32,8 @@ CHAIN_TRAFFIC_BYTES = 17.67e12
 CHAIN_TRAFFIC_SOURCE = "profiles/chain_nx48_B512_N256.md"
 # HBM bytes per forward launch (stream_assemble_kernel + stream_qp_kernel, one SQP
 # iteration, mean over the three) of the ragged mix's dominant group, the streamed
-# quadrotor instances with N in 64..200: 267.2 GB read + 22.3 GB written
-RAGGED_TRAFFIC_BYTES = 289.4e9
+# quadrotor instances with N in 64..200: 275.3 GB read + 22.4 GB written
+RAGGED_TRAFFIC_BYTES = 297.7e9
 RAGGED_TRAFFIC_GROUP = "quadrotor/1"
 RAGGED_TRAFFIC_SOURCE = "profiles/ragged_mixed_B8192_T10-200.md"
 
