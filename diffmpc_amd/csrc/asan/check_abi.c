@@ -171,7 +171,7 @@ int main(void) {
   EXPECT(dmpc_pcg_recover(1, 1, 1, 1, NULL, d, d, d, d, d, d, d, d, d, NULL), DMPC_ERR_ARG);
   EXPECT(dmpc_set_blocks_per_group(3), DMPC_ERR_ARG);
   EXPECT(dmpc_set_blocks_per_group(0), DMPC_OK);
-  EXPECT(dmpc_set_fused_variant(3), DMPC_ERR_ARG);
+  EXPECT(dmpc_set_fused_variant(4), DMPC_ERR_ARG);
   EXPECT(dmpc_set_fused_variant(0), DMPC_OK);
   {
     double ms = 0.0;

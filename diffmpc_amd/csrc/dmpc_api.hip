@@ -105,7 +105,7 @@ int check_problem(const dmpc_problem* p) {
 
 bool tuning_ok(const dmpc_solver_params* s) {
   const int k = s->blocks_per_group, v = s->fused_variant;
-  return (k == 0 || k == 1 || k == 2 || k == 4) && v >= 0 && v <= 2;
+  return (k == 0 || k == 1 || k == 2 || k == 4) && v >= 0 && v <= 3;
 }
 
 bool linesearch_is_noop(const dmpc_solver_params* s) {
@@ -172,7 +172,7 @@ int dmpc_set_blocks_per_group(int32_t k) {
 }
 
 int dmpc_set_fused_variant(int32_t v) {
-  if (v < 0 || v > 2) return DMPC_ERR_ARG;
+  if (v < 0 || v > 3) return DMPC_ERR_ARG;
   g_fused_variant = v;
   return DMPC_OK;
 }

@@ -1289,7 +1289,7 @@ bool register_fits(int model, int nx, int nu, int N, int lsd) {
 bool use_stream(int model, int nx, int nu, int N, int lsd, int variant) {
   const int v = variant > 0 ? variant : g_fused_variant;
   if (v == 1) return false;
-  if (v == 2) return stream_supported(model, nx, nu);
+  if (v >= 2) return stream_supported(model, nx, nu);
   return !register_fits(model, nx, nu, N, lsd) && stream_supported(model, nx, nu);
 }
 

@@ -47,6 +47,9 @@ struct SolveArgs {
   // launch order of the per-instance workgroups of one QP launch (workgroup
   // i solves instance order[i]), or null for the identity
   const int32_t* order;
+  // streamed variant, time-invariant model: one copy of the blocks (see
+  // shared_block in dmpc_stream.hip)
+  int shared_blocks;
 };
 
 struct BackwardArgs {
@@ -69,6 +72,7 @@ struct BackwardArgs {
   // null for the identity
   const int32_t* order;
   size_t order_stride;
+  int shared_blocks;  // as in SolveArgs
 };
 
 // SQP continue rule of sqp.py:460-466 for instance b:

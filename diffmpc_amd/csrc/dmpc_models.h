@@ -12,6 +12,7 @@
 // Dual, double for double, HD for HD).  The flat parameter layouts are the
 // ones documented in include/dmpc.h.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 namespace dmpc {
@@ -147,10 +148,14 @@ enum ModelId : int {
 
 // ---------------------------------------------------------------------------
 // xdot = A x + B u + b   (diffmpc/dynamics/linear_dynamics.py:46-70)
+// Linear in (x, u): the forward-mode Jacobian's tangent parts never involve
+// the point's values, so [A_t | B_t] comes out bit for bit the same at every
+// knot (kTimeInvariant).
 template <int NX_, int NU_>
 struct LinearModel {
   static constexpr int NX = NX_, NU = NU_;
   static constexpr int NPARAM = NX * NX + NX * NU + NX;
+  static constexpr bool kTimeInvariant = true;
   template <class T, class PA>
   __device__ static void f(const T* x, const T* u, PA p, T* xd) {
 #pragma unroll
@@ -276,6 +281,7 @@ struct CartpoleModel {
 struct ChainModel {
   static constexpr int M = 12;
   static constexpr int NX = 4 * M, NU = M, NPARAM = 2 * M;
+  static constexpr bool kTimeInvariant = true;  // linear, as LinearModel
   template <class T, class PA>
   __device__ static void f(const T* x, const T* u, PA p, T* xd) {
 #pragma unroll
@@ -300,6 +306,13 @@ struct ChainModel {
     }
   }
 };
+
+// Models whose linearization is the same at every knot declare
+// kTimeInvariant = true; the others are treated as time-varying.
+template <class M, class = void>
+struct time_invariant : std::false_type {};
+template <class M>
+struct time_invariant<M, std::void_t<decltype(M::kTimeInvariant)>> : std::integral_constant<bool, M::kTimeInvariant> {};
 
 // ---------------------------------------------------------------------------
 // One integration step (diffmpc/dynamics/integrators.py:20-71):

@@ -56,10 +56,29 @@ struct Work {
   double* v[kStreamVectors];
   double *DT, *ID, *DX, *DU, *LS;
   float* AT;
+  // time-invariant model: knots share one copy of their blocks (shared_block)
+  int shared, N;
 };
+
+// Block a knot's data is read from.  A time-invariant model (linear in x and
+// u, kTimeInvariant) has the same [A_t | B_t] at every knot, and with the
+// reference's time-invariant weights theta_t -- and so D_t -- is the same for
+// 1 <= t < N (theta_0 = Qinv_0 and theta_N has the terminal weight).  Such
+// instances store and read block 0 of [A|B] (and of its transposed copy)
+// and D_0, D_1, D_N only: the blocks a PCG round reads then stay in cache
+// instead of streaming from HBM, and the results are the same bits, since
+// the shared copies are the values every knot would compute.
+__device__ __forceinline__ int shared_ab(const Work& w, int tp) { return w.shared ? 0 : tp; }
+__device__ __forceinline__ int shared_d(const Work& w, int t) {
+  return w.shared ? (t == 0 ? 0 : (t < w.N ? 1 : w.N)) : t;
+}
 
 __host__ __device__ inline size_t carve_work(char* base, int nx, int nu, int N, int ls_doubles, Work* w) {
   size_t off = 0;
+  if (w) {
+    w->shared = 0;
+    w->N = N;
+  }
   const size_t vbytes = align256((size_t)(N + 3) * nx * 8);
   for (int k = 0; k < kStreamVectors; ++k) {
     if (w) w->v[k] = reinterpret_cast<double*>(base + off);
@@ -180,6 +199,7 @@ struct AssembleArgs {
   char* work;
   size_t work_stride;
   int ls_doubles;
+  int shared_blocks;  // assemble knots 0, 1 and N only (see shared_d)
 };
 
 template <int NX, int NU>
@@ -199,8 +219,11 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
   }
   const int N = horizon_of(a.horizons, Np, b);
   if (t > N) return;
+  if (a.shared_blocks && t > 1 && t < N) return;  // the same as knot 1
   Work w;
   carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, a.ls_doubles, &w);
+  w.shared = a.shared_blocks;
+  w.N = N;
   const int i = threadIdx.x;
   const double* wx = a.wx + (size_t)b * NX;
   const double* wu = a.wu + (size_t)b * NU;
@@ -314,6 +337,7 @@ template <int G, int NX>
 __device__ __forceinline__ double dmul_mem(const Work& w, int t, double v, int lane, bool ok) {
   const int l = lane < NX ? lane : 0;
   double acc;
+  t = shared_d(w, t);
   if constexpr (compact_blocks(NX)) {
     const double* D = w.DT + (size_t)t * tri(NX, 0);
     const int lrow = tri(l, 0);
@@ -331,9 +355,10 @@ __device__ __forceinline__ double dmul_mem(const Work& w, int t, double v, int l
 // next column, in whichever copy the block layout reads; B_{t-1} follows at
 // NX strides.
 template <int NX, int NU>
-__device__ __forceinline__ const float* arow_mem(const float* ABb, const float* AT, int t, int li) {
+__device__ __forceinline__ const float* arow_mem(const float* ABb, const Work& w, int t, int li) {
   constexpr int NZ = NX + NU;
-  const size_t tp = (size_t)(t >= 1 ? t - 1 : 0);
+  const float* AT = w.AT;
+  const size_t tp = (size_t)shared_ab(w, t >= 1 ? t - 1 : 0);
   if constexpr (compact_blocks(NX)) return ABb + tp * NX * NZ + li * NZ;
   else return AT + tp * NZ * NX + li;
 }
@@ -379,8 +404,8 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   const double* GAM = w.v[V_GAM];
   const int lane = K.lane, N = K.N;
   const int li = lane < NX ? lane : 0;
-  auto acol = [&](int t) { return ABb + (size_t)(t < N ? t : 0) * NX * NZ + li; };        // stride NZ
-  auto arow = [&](int t) { return arow_mem<NX, NU>(ABb, w.AT, t, li); };
+  auto acol = [&](int t) { return ABb + (size_t)shared_ab(w, t < N ? t : 0) * NX * NZ + li; };        // stride NZ
+  auto arow = [&](int t) { return arow_mem<NX, NU>(ABb, w, t, li); };
   constexpr int AS = arow_stride<NX>();
   auto has_a = [&](int t) { return t < N; };
   auto has_p = [&](int t) { return t >= 1; };
@@ -529,9 +554,9 @@ __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r
   const double* LAM = w.v[V_LAM];
   const double lt = vget<NX>(LAM, t, lane);
   const double ln = vget<NX>(LAM, t + 1, lane);
-  const double ax = gdot_mem<G, NX>(ABb + (size_t)(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
+  const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab(w, t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
   constexpr int AS = arow_stride<NX>();
-  const float* brow = arow_mem<NX, NU>(ABb, w.AT, t, li) + NX * AS;
+  const float* brow = arow_mem<NX, NU>(ABb, w, t, li) + NX * AS;
   double bl = 0.0;
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
@@ -576,6 +601,8 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, Np);
   Work w;
   carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, lsd, &w);
+  w.shared = a.shared_blocks;
+  w.N = N;
   vectors_in_lds<NV, NX>(&w, stream_vlds, Np);
   Blocks<G, NX, NU> K;
   K.init(N);
@@ -670,6 +697,8 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
   const int N = horizon_of(a.horizons, Np, b), NB = N + 1;
   Work w;
   carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, 0, &w);
+  w.shared = a.shared_blocks;
+  w.N = N;
   vectors_in_lds<NV, NX>(&w, stream_vlds, Np);
   Blocks<G, NX, NU> K;
   K.init(N);
@@ -711,9 +740,9 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
     const double* LAM = w.v[V_LAM];
     const double lt = vget<NX>(LAM, t, lane);
     const double ln = vget<NX>(LAM, t + 1, lane);
-    const double ax = gdot_mem<G, NX>(ABb + (size_t)(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
+    const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab(w, t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
     constexpr int AS = arow_stride<NX>();
-    const float* brow = arow_mem<NX, NU>(ABb, w.AT, t, li) + NX * AS;
+    const float* brow = arow_mem<NX, NU>(ABb, w, t, li) + NX * AS;
     double bl = 0.0;
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
@@ -805,7 +834,16 @@ AssembleArgs assemble_args(const SolveArgs& a, int lsd) {
   s.work = a.work;
   s.work_stride = a.work_stride;
   s.ls_doubles = lsd;
+  s.shared_blocks = a.shared_blocks;
   return s;
+}
+
+// One copy of the blocks for time-invariant models, unless the caller asked
+// for the streamed variant with a copy per knot (fused variant 3).
+template <class Model>
+int shares_blocks(int variant) {
+  const int v = variant > 0 ? variant : g_fused_variant;
+  return time_invariant<Model>::value && v != 3 ? 1 : 0;
 }
 
 }  // namespace
@@ -852,8 +890,10 @@ int lds_vectors(int N, bool forward) {
 }
 
 template <class Model, int G>
-int stream_forward_t(int model, const SolveArgs& a, hipStream_t st) {
+int stream_forward_t(int model, const SolveArgs& a0, hipStream_t st) {
   constexpr int NX = Model::NX, NU = Model::NU;
+  SolveArgs a = a0;
+  a.shared_blocks = shares_blocks<Model>(a.variant);
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
   const AssembleArgs aa = assemble_args<Model>(a, lsd);
   const int threads = stream_threads(a.N, G, stream_max_threads<Model>());
@@ -883,8 +923,10 @@ int stream_forward_t(int model, const SolveArgs& a, hipStream_t st) {
 }
 
 template <class Model, int G>
-int stream_backward_t(const BackwardArgs& a, hipStream_t st) {
+int stream_backward_t(const BackwardArgs& a0, hipStream_t st) {
   constexpr int NX = Model::NX, NU = Model::NU;
+  BackwardArgs a = a0;
+  a.shared_blocks = shares_blocks<Model>(a.variant);
   AssembleArgs aa;
   aa.B = a.B;
   aa.N = a.N;
@@ -900,6 +942,7 @@ int stream_backward_t(const BackwardArgs& a, hipStream_t st) {
   aa.work = a.work;
   aa.work_stride = a.work_stride;
   aa.ls_doubles = 0;
+  aa.shared_blocks = a.shared_blocks;
   // ragged batches: longest-first order in the instances' order slots
   BackwardArgs ab = a;
   int32_t* slots = reinterpret_cast<int32_t*>(a.work + a.work_stride - kOrderSlot);

@@ -17,7 +17,8 @@ The solve runs in libdmpc.so: mode "fused" is one workgroup per instance per
 SQP iteration, "plain" is the per-launch op graph of the reference, and
 "auto" (the default) takes fused where the library builds it for the model
 and size, else plain (linear models of any size).  Solver parameters use the reference's keys plus the optional
-``"mode"``, ``"fused_variant"`` ("auto", "register", "streamed") and
+``"mode"``, ``"fused_variant"`` ("auto", "register", "streamed",
+"streamed_per_knot") and
 ``"blocks_per_group"`` (0 = automatic, 1, 2, 4), which diffmpc_amd.autotune
 sets.  ``problem_params["horizons"]`` (B,) makes a ragged batch.
 """
@@ -102,7 +103,8 @@ def _c_solver_params(params: Dict[str, Any]) -> "_lib.SolverParams":
     sp.warm_start_backward = int(bool(params["warm_start_backward"]))
     # fused-mode tuning (keys with no reference counterpart; 0 = library default)
     sp.blocks_per_group = int(params.get("blocks_per_group", 0))
-    sp.fused_variant = {"auto": 0, "register": 1, "streamed": 2}[params.get("fused_variant", "auto")]
+    sp.fused_variant = {"auto": 0, "register": 1, "streamed": 2, "streamed_per_knot": 3}[
+        params.get("fused_variant", "auto")]
     return sp
 
 

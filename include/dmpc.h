@@ -77,7 +77,8 @@ typedef struct dmpc_solver_params { /* keys of diffmpc/solvers/params/sqp.yaml *
   /* Fused-mode tuning for this call (no reference counterpart; 0 = the
    * library default, see dmpc_set_blocks_per_group / dmpc_set_fused_variant):
    * time blocks per lane group of the register-resident kernels (1, 2, 4) and
-   * the kernel variant (1 register-resident, 2 streamed). */
+   * the kernel variant (1 register-resident, 2 streamed, 3 streamed with a
+   * copy of the blocks per knot even for time-invariant models). */
   int32_t blocks_per_group;
   int32_t fused_variant;
 } dmpc_solver_params;
@@ -236,7 +237,11 @@ int dmpc_workspace_bytes(int32_t mode, const dmpc_problem* prob, const dmpc_solv
 int dmpc_set_blocks_per_group(int32_t k);
 
 /* Tuning: fused variant, 0 = automatic (register-resident blocks when they fit
- * on chip, else streamed), 1 = register-resident only, 2 = streamed only. */
+ * on chip, else streamed), 1 = register-resident only, 2 = streamed only,
+ * 3 = streamed only, and a copy of the preconditioner blocks per knot even for
+ * models whose linearization is the same at every knot (linear and chain
+ * models, which otherwise store and read one copy: same results, for
+ * comparison). */
 int dmpc_set_fused_variant(int32_t v);
 
 /* Instrumentation (no reference counterpart): when enabled, the library

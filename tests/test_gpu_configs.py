@@ -237,3 +237,37 @@ def test_longest_first_launch_order_keeps_every_result(cuda, variant, hmin, hmax
         b = np.concatenate([p[j] for p in parts])
         assert np.array_equal(a, b), j
     say("  identical")
+
+
+@pytest.mark.parametrize("name,horizon,batch", [("chain", 64, 8), ("linear", 300, 16)])
+def test_time_invariant_shared_blocks_match_per_knot_blocks(cuda, name, horizon, batch):
+    """Linear models linearize to the same [A_t | B_t] at every knot, so the
+    streamed variant stores and reads one copy of the blocks (and D_0, D_1,
+    D_N of the preconditioner) instead of one per knot.  The results are the
+    same bits as the streamed variant with a copy per knot (variant 3):
+    trajectories, multipliers, PCG counts and the gradient w.r.t. the state
+    weights, on a ragged batch."""
+    say(f"[shared blocks] {name} N={horizon} B={batch}")
+    if name == "chain":
+        _, prob, sp = W.chain(batch=batch, seed=4, horizon=horizon)
+    else:
+        _, prob, sp = W.lti(batch=batch, seed=8, horizon=horizon)
+        sp = dict(sp)
+        sp["pcg"] = dict(sp["pcg"], max_iter=2000)
+    hs = np.random.default_rng(3).integers(horizon // 2, horizon + 1, size=batch).astype(np.int32)
+    key = "weights_penalization_reference_state_trajectory"
+
+    def run(v):
+        solver, params, guess = W.make_solver(name, prob, sp, device=cuda, horizons=hs)
+        wx = params[key].clone().requires_grad_(True)
+        with _variant(v):
+            sol = solver.solve(guess, params, {key: wx})
+            loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+            (g,) = torch.autograd.grad(loss, [wx])
+        torch.cuda.synchronize()
+        return [_np(sol.states), _np(sol.controls), _np(sol.dual), _np(sol.num_pcg_iters), _np(g)]
+
+    shared, per_knot = run(2), run(3)
+    for j, (a, b) in enumerate(zip(shared, per_knot)):
+        assert np.array_equal(a, b), j
+    say("  identical")
