@@ -701,3 +701,38 @@ def test_graph_captured_rollout_gradient_matches_eager(cuda):
         total, g = graphed.replay()
         torch.cuda.synchronize()
         assert torch.equal(total, e_total) and torch.equal(g, e_g)
+
+
+@pytest.mark.parametrize("name", ["spacecraft7", "quadrotor"])
+def test_float32_dynamics_parameters_against_float64_reference_parameters(cuda, name):
+    """The C ABI carries the dynamics parameters as float32 (the Jacobians are
+    float32 dual numbers; f_d and the QP are float64), while the reference
+    computes with float64 parameters.  Against the float64 oracle run with the
+    unrounded float64 parameters (attitude inertias ~ U(0.5, 2), the
+    quadrotor's J = diag(0.01, 0.01, 0.02): neither is a float32 value), the
+    GPU trajectories stay within 1e-5 and the gradients within 1e-4 relative;
+    the oracle itself moves by the rounding of its parameters to float32 by
+    far less (printed), so the narrowing is not what the tolerance absorbs."""
+    gen = {"spacecraft7": lambda: W.spacecraft7(batch=16, seed=21, horizon=60, sqp_iters=3),
+           "quadrotor": lambda: W.quadrotor(batch=8, seed=21, horizon=30, sqp_iters=3)}[name]
+    _, prob, sp = gen()
+    assert not np.array_equal(prob["dyn"], prob["dyn"].astype(np.float32).astype(np.float64))
+    say(f"[float32 parameters] {name} B={prob['x0'].shape[0]} N={prob['horizon']}")
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    key = "weights_penalization_reference_state_trajectory"
+    wx = params[key].clone().requires_grad_(True)
+    sol = solver.solve(guess, params, {key: wx})
+    loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+    (g,) = torch.autograd.grad(loss, [wx])
+    torch.cuda.synchronize()
+    lg = lambda p, r: (r["states"] - p["x_ref"], np.zeros_like(r["controls"]))  # noqa: E731
+    idx = list(range(prob["x0"].shape[0]))
+    _, ref64, g64 = _sampled_oracle(name, prob, sp, idx, lg)
+    p32 = dict(prob, dyn=prob["dyn"].astype(np.float32).astype(np.float64))
+    _, ref32, g32 = _sampled_oracle(name, p32, sp, idx, lg)
+    ex, eg = rel(_np(sol.states), ref64["states"]), rel(_np(g), g64["w_x"])
+    say(f"  GPU vs float64-parameter oracle: states {ex:.2e}, dl/dw_x {eg:.2e}; "
+        f"oracle float32 vs float64 parameters: states {rel(ref32['states'], ref64['states']):.2e}, "
+        f"dl/dw_x {rel(g32['w_x'], g64['w_x']):.2e}")
+    assert ex <= 1e-5 and rel(_np(sol.controls), ref64["controls"]) <= 1e-5
+    assert eg <= 1e-4
