@@ -211,6 +211,38 @@ __device__ __forceinline__ double gbcast_at(double v, int src, int lane) {
   }
 }
 
+// acc += (value of `src` in lane n of the group) * scale, n known after
+// unrolling.  16-lane groups: one v_fmac_f64_dpp with row_newbcast:n (no
+// broadcast temporary; the s_nop covers the DPP read-after-VALU-write wait);
+// other groups: gbcast and an FMA.
+template <int n>
+__device__ __forceinline__ void dpp_fmac16(double& acc, double src, double scale) {
+  asm("s_nop 1\n"
+      "v_fmac_f64_dpp %0, %1, %2 row_newbcast:%c3 row_mask:0xf bank_mask:0xf"
+      : "+v"(acc)
+      : "v"(src), "v"(scale), "i"(n));
+}
+template <int G, int n = 0>
+__device__ __forceinline__ void bcast_fmac_at(double& acc, double src, double scale, int from, int lane) {
+  if constexpr (n == G - 1) {
+    if constexpr (G == 16) {
+      dpp_fmac16<n>(acc, src, scale);
+    } else {
+      acc = fma(gbcast<G, n>(src, lane), scale, acc);
+    }
+  } else {
+    if (from == n) {
+      if constexpr (G == 16) {
+        dpp_fmac16<n>(acc, src, scale);
+      } else {
+        acc = fma(gbcast<G, n>(src, lane), scale, acc);
+      }
+    } else {
+      bcast_fmac_at<G, n + 1>(acc, src, scale, from, lane);
+    }
+  }
+}
+
 // sum_j a[j] * v_j where v_j lives in lane j of the group (j < M <= NX).
 template <int G, int M, int j = 0, class AT>
 __device__ __forceinline__ double gdot_chain(const AT* a, double v, int lane, int salt, double acc) {

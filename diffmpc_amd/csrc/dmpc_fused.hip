@@ -46,6 +46,18 @@ int stream_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_
 
 namespace {
 
+// Where D_t = theta_t^-1 lives during PCG: in registers (row `lane` per lane)
+// for small blocks, in LDS for nx > 8, where the registers are needed for the
+// A/B rows.  The LDS copy is stored transposed, DT[t][j][i] = D_t[i][j] as
+// computed by lane i, so the lanes of a group read consecutive words.
+template <int NX>
+__host__ __device__ constexpr bool d_in_lds() { return NX > 8; }
+
+// Whether the block's triangular solves for D_t run in registers (4- and
+// 16-lane groups) or against the factor parked in LDS (8-lane groups).
+template <int NX>
+__host__ __device__ constexpr bool solve_in_registers() { return !(NX > 4 && NX <= 8); }
+
 // LDS carve-up.  Vectors over time blocks use stride NX per block; arrays
 // marked "padded" have a zero block before t = 0 and after t = N, so
 // neighbour reads need no branches: index (t + 1) * NX + i.
@@ -55,7 +67,8 @@ struct LdsPlan {
     const size_t NB = N + 1, NBP = N + 3;
     return NB * NX /*xb*/ + NB * NU /*ub*/ + NBP * NX /*lamS*/ + NB * NX /*qinv*/ + NB * NX /*qv*/ +
            NB * NU /*rv*/ + NU /*rinv*/ + NX /*x0*/ + 3 * 64 /*red*/ + 4 * NBP * NX /*RT Y YQ YX*/ +
-           NB * NX * NX /*LF*/ + NB * NX /*ID*/ + (size_t)N * NX /*fd*/;
+           (d_in_lds<NX>() || !solve_in_registers<NX>() ? NB * NX * NX : 0) /*LF*/ +
+           (solve_in_registers<NX>() ? 0 : NB * NX) /*ID*/ + (size_t)N * NX /*fd*/;
   }
   static __host__ __device__ size_t floats(int N, int P) { return (size_t)P; }
   // doubles(N) plus `ls` doubles of line-search scratch
@@ -67,7 +80,8 @@ struct LdsPlan {
 struct Lds {
   double *xb, *ub, *lamS, *qinv, *qv, *rv, *rinv, *x0, *redA, *redB, *redC;
   double *RT, *Y, *YQ, *YX;
-  double *LF, *ID;  // Cholesky factor rows of theta_t and 1 / L_kk
+  double* LF;       // D_t = theta_t^-1, transposed (nx > 8); 8-lane groups: factor rows
+  double* ID;       // 8-lane groups: 1 / L_kk
   double* fd;       // f_d(x_t, u_t), t < N
   double* LS;       // line-search scratch (see linesearch_doubles)
   float* prm;
@@ -93,20 +107,14 @@ __device__ Lds carve(void* base, int N, int ls = 0) {
   L.Y = d; d += NBP * NX;
   L.YQ = d; d += NBP * NX;
   L.YX = d; d += NBP * NX;
-  L.LF = d; d += NB * NX * NX;
-  L.ID = d; d += NB * NX;
+  L.LF = d; d += d_in_lds<NX>() || !solve_in_registers<NX>() ? NB * NX * NX : 0;
+  L.ID = d; d += solve_in_registers<NX>() ? 0 : NB * NX;
   L.fd = d; d += N * NX;
   L.LS = d; d += ls;
   L.prm = reinterpret_cast<float*>(d);
   return L;
 }
 
-// Where D_t = theta_t^-1 lives during PCG: in registers (row `lane` per lane)
-// for small blocks, in LDS for nx > 8, where the registers are needed for the
-// A/B rows.  The LDS copy is stored transposed, DT[t][j][i] = D_t[i][j] as
-// computed by lane i, so the lanes of a group read consecutive words.
-template <int NX>
-__host__ __device__ constexpr bool d_in_lds() { return NX > 8; }
 
 // 16-lane groups with four controls keep B's rows column-permuted per quad
 // of lanes and form the B-terms by reduce-scatter (rs16/gather16); other
@@ -216,8 +224,8 @@ __device__ __forceinline__ void theta_row16x13(const float* Ap, const double* qp
 //   theta_0 = Qinv_0,
 //   theta_t = A_{t-1} Qinv_{t-1} A_{t-1}^T + B_{t-1} Rinv B_{t-1}^T + Qinv_t
 // (pcg.py:182-196 with As_next = -I).  The factor is computed in registers
-// with cross-lane shuffles, parked in LDS, and each lane then solves
-// L L^T d = e_lane for its row of D_t.
+// with cross-lane broadcasts, and each lane then solves L L^T d = e_lane for
+// its row of D_t, also in registers.
 template <int G, int NX, int NU, int K>
 __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* ABb, const Lds& L, int lane,
                                               int gbase) {
@@ -252,13 +260,15 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* A
       }
     }
     // right-looking Cholesky: lane i ends with row[k] = L[i][k], k <= i;
-    // pivots and column entries reach the group's lanes by DPP broadcast
+    // pivots and column entries reach the group's lanes by DPP broadcast;
+    // id[k] = 1 / L_kk is the same in every lane of the group
+    double id[NX];
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
       const double dkk = gbcast_at<G>(row[k], k, lane);
       const double lkk = sqrt(dkk);
       const double il = 1.0 / lkk;
-      if (ok && lane == k) L.ID[t * NX + k] = il;
+      id[k] = il;
       const double lik = (lane == k) ? lkk : row[k] * il;
       row[k] = lik;
 #pragma unroll
@@ -267,42 +277,80 @@ __device__ __forceinline__ void assemble_dinv(Regs<NX, NU, K>& R, const float* A
         row[j] -= lik * ljk;
       }
     }
-    if (ok) {
+    if constexpr (solve_in_registers<NX>()) {
+      // column `lane` of theta^-1 = L^-T L^-1 e_lane by forward and back
+      // substitution in registers: L[j][k] is entry k of lane j's row,
+      // broadcast by DPP, so no factor goes through LDS.  Forward column by
+      // column, back substitution row by row; each unknown's terms are
+      // subtracted in ascending index order, as the substitution through
+      // LDS for 8-lane groups does, with the same roundings.
+      double x[NX];
 #pragma unroll
-      for (int k = 0; k < NX; ++k) L.LF[(t * NX + lane) * NX + k] = row[k];
-    }
-  }
-  __syncthreads();
+      for (int j = 0; j < NX; ++j) x[j] = (j == lane) ? 1.0 : 0.0;
 #pragma unroll
-  for (int s = 0; s < K; ++s) {
-    const int t = R.t[s];
-    const double* LF = L.LF + (size_t)t * NX * NX;
-    const double* ID = L.ID + t * NX;
-    // column `lane` of theta^-1 = L^-T L^-1 e_lane
-    double w[NX], d[NX];
+      for (int k = 0; k < NX; ++k) {
+        x[k] *= id[k];
+        const double nx_k = -x[k];
 #pragma unroll
-    for (int j = 0; j < NX; ++j) {
-      const int rj = j == 0 ? 0 : after(j * NX, w[j - 1]);  // row j of L
-      double acc = (j == lane) ? 1.0 : 0.0;
+        for (int j = k + 1; j < NX; ++j) bcast_fmac_at<G>(x[j], row[k], nx_k, j, lane);
+      }
 #pragma unroll
-      for (int k = 0; k < j; ++k) acc -= LF[rj + k] * w[k];
-      w[j] = acc * ID[j];
-    }
+      for (int j = NX - 1; j >= 0; --j) {
+        double acc = x[j];
 #pragma unroll
-    for (int j = NX - 1; j >= 0; --j) {
-      const int cj = j == NX - 1 ? j : after(j, d[j + 1]);  // column j of L
-      double acc = w[j];
+        for (int k = j + 1; k < NX; ++k) bcast_fmac_at<G>(acc, row[j], -x[k], k, lane);
+        x[j] = acc * id[j];
+      }
+      if constexpr (d_in_lds<NX>()) {
+        // transposed, DT[t][j][i] = D_t[i][j], so a group reads consecutive
+        // words; no other lane reads this block before the barrier below
+        if (ok) {
 #pragma unroll
-      for (int k = j + 1; k < NX; ++k) acc -= LF[k * NX + cj] * d[k];
-      d[j] = acc * ID[j];
-    }
-    if constexpr (d_in_lds<NX>()) {
-      __syncthreads();  // every lane is done reading the factors
-      if (R.ok[s]) {
+          for (int j = 0; j < NX; ++j) L.LF[((size_t)t * NX + j) * NX + lane] = x[j];
+        }
+      } else {
 #pragma unroll
-        for (int j = 0; j < NX; ++j) L.LF[((size_t)t * NX + j) * NX + lane] = d[j];
+        for (int j = 0; j < NX; ++j) R.drow[s][j] = ok ? x[j] : 0.0;
       }
     } else {
+      // 8-lane groups: the factor rows and 1 / L_kk go through LDS and each
+      // lane substitutes against them (their two-move DPP broadcasts would
+      // hold more registers than the 8-lane kernels have to spare)
+      if (ok) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) L.LF[(t * NX + lane) * NX + k] = row[k];
+        if (lane == 0) {
+#pragma unroll
+          for (int k = 0; k < NX; ++k) L.ID[t * NX + k] = id[k];
+        }
+      }
+    }
+  }
+  if constexpr (!solve_in_registers<NX>()) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+      const int t = R.t[s];
+      const double* LF = L.LF + (size_t)t * NX * NX;
+      const double* ID = L.ID + t * NX;
+      // column `lane` of theta^-1 = L^-T L^-1 e_lane
+      double w[NX], d[NX];
+#pragma unroll
+      for (int j = 0; j < NX; ++j) {
+        const int rj = j == 0 ? 0 : after(j * NX, w[j - 1]);  // row j of L
+        double acc = (j == lane) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < j; ++k) acc -= LF[rj + k] * w[k];
+        w[j] = acc * ID[j];
+      }
+#pragma unroll
+      for (int j = NX - 1; j >= 0; --j) {
+        const int cj = j == NX - 1 ? j : after(j, d[j + 1]);  // column j of L
+        double acc = w[j];
+#pragma unroll
+        for (int k = j + 1; k < NX; ++k) acc -= LF[k * NX + cj] * d[k];
+        d[j] = acc * ID[j];
+      }
 #pragma unroll
       for (int j = 0; j < NX; ++j) R.drow[s][j] = R.ok[s] ? d[j] : 0.0;
     }
