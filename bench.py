@@ -23,8 +23,8 @@ METRIC = "SQP solves/s & fwd+bwd grads/s, quadrotor T=50 B=1024, fused vs plain 
 MI355X_FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per qp_iteration_kernel launch at the default workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scale factors
-# from a 256 MiB device copy): 115.1 MB read + 95.6 MB written
-QP_TRAFFIC_BYTES = 210.7e6
+# from a 256 MiB device copy): 72.6 MB read + 18.2 MB written
+QP_TRAFFIC_BYTES = 90.8e6
 QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
 # HBM bytes per forward SQP iteration of the chain workload (stream_assemble_kernel
 # + stream_qp_kernel, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0): 1.034 TB read + 0.757 TB

@@ -967,7 +967,11 @@ __global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
 // One SQP iteration of one instance per workgroup: Schur/preconditioner
 // assembly, gamma, the whole PCG loop, primal recovery and the step test
 // (sqp.py:262-299, 506-528).
-template <class Model, int G, int K, int MAXT>
+// LS: the build with the float64 merit line search (a.ls_eval).  Its model
+// evaluations hold enough registers to make the whole kernel spill at the
+// 128-VGPR cap of the one-block-per-group variant, so solves that do not
+// evaluate the line search run a build without it, which spills nothing.
+template <class Model, int G, int K, int MAXT, bool LS>
 __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   extern __shared__ double smem_[];
@@ -997,7 +1001,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   double xn[K], un[K];
   recover_rows<G>(R, L, N, lane, xn, un);
   double alpha = 1.0;
-  if (a.ls_eval) {  // uniform over the workgroup
+  if constexpr (LS) {
     double* DX = L.YX;  // (N+1, nx): QP step of the states
     double* DU = L.Y;   // (N+1, nu): QP step of the controls
 #pragma unroll
@@ -1237,11 +1241,11 @@ Launch plan(int N, int P, int ls = 0, int forced_K = 0) {
   return l;  // K = 0: does not fit
 }
 
-template <class Model, int G, int K, int MAXT>
+template <class Model, int G, int K, int MAXT, bool LS>
 int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&qp_iteration_kernel<Model, G, K, MAXT>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&qp_iteration_kernel<Model, G, K, MAXT, LS>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) != hipSuccess)
       return DMPC_ERR_HIP;
     attr = true;
@@ -1256,7 +1260,7 @@ int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
     SolveArgs ai = a;
     ai.order = launch_order(a, it, a.status, st) ? a.status : nullptr;
     prof_begin(PROF_QP_ITERATION, st);
-    hipLaunchKernelGGL((qp_iteration_kernel<Model, G, K, MAXT>), dim3(a.B), dim3(l.threads), l.lds, st, ai);
+    hipLaunchKernelGGL((qp_iteration_kernel<Model, G, K, MAXT, LS>), dim3(a.B), dim3(l.threads), l.lds, st, ai);
     prof_end(PROF_QP_ITERATION, st);
   }
   hipLaunchKernelGGL(sqp_finish_kernel, dim3(a.B), dim3(256), 0, st, a, Model::NX, Model::NU);
@@ -1283,10 +1287,13 @@ int fwd_model(const SolveArgs& a, hipStream_t st) {
   Launch l = plan<Model, G>(a.N, a.P, linesearch_doubles(a.ls_eval, a.n_alphas, a.N), a.forced_K);
   if (l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
   switch (l.K) {
-    case 1: return launch_fwd_k<Model, G, 1, 1024>(a, l, st);
-    case 2: return launch_fwd_k<Model, G, 2, 512>(a, l, st);
+    case 1:
+      return a.ls_eval ? launch_fwd_k<Model, G, 1, 1024, true>(a, l, st) : launch_fwd_k<Model, G, 1, 1024, false>(a, l, st);
+    case 2:
+      return a.ls_eval ? launch_fwd_k<Model, G, 2, 512, true>(a, l, st) : launch_fwd_k<Model, G, 2, 512, false>(a, l, st);
     case 4:
-      if constexpr (max_blocks_per_group<Model>() >= 4) return launch_fwd_k<Model, G, 4, 256>(a, l, st);
+      if constexpr (max_blocks_per_group<Model>() >= 4)
+        return a.ls_eval ? launch_fwd_k<Model, G, 4, 256, true>(a, l, st) : launch_fwd_k<Model, G, 4, 256, false>(a, l, st);
       return DMPC_ERR_UNSUPPORTED;
     default: return DMPC_ERR_UNSUPPORTED;
   }
