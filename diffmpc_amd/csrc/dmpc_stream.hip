@@ -56,8 +56,6 @@ struct Work {
   double* v[kStreamVectors];
   double *DT, *ID, *DX, *DU, *LS;
   float* AT;
-  // time-invariant model: knots share one copy of their blocks (shared_block)
-  int shared, N;
 };
 
 // Block a knot's data is read from.  A time-invariant model (linear in x and
@@ -67,18 +65,31 @@ struct Work {
 // instances store and read block 0 of [A|B] (and of its transposed copy)
 // and D_0, D_1, D_N only: the blocks a PCG round reads then stay in cache
 // instead of streaming from HBM, and the results are the same bits, since
-// the shared copies are the values every knot would compute.
-__device__ __forceinline__ int shared_ab(const Work& w, int tp) { return w.shared ? 0 : tp; }
-__device__ __forceinline__ int shared_d(const Work& w, int t) {
-  return w.shared ? (t == 0 ? 0 : (t < w.N ? 1 : w.N)) : t;
+// the shared copies are the values every knot would compute.  SH is a
+// template flag of the streamed kernels (instantiated both ways for
+// time-invariant models only), so the other kernels keep their plain
+// per-knot addressing.  In the SH kernels the choice is the runtime flag
+// `on` (the launch's shared_blocks), passed through an empty asm at each use:
+// with the block addresses known loop-invariant the compiler versions the
+// PCG rounds on the flag and hoists the block loads out of them, and then
+// spills them (the 48-state chain's forward launch took 2.4x as long so).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+template <bool SH>
+__device__ __forceinline__ int shared_ab(int tp, int on) {
+  if constexpr (SH) return opaque(on) ? 0 : tp;
+  return tp;
+}
+template <bool SH>
+__device__ __forceinline__ int shared_d(int t, int N, int on) {
+  if constexpr (SH) return opaque(on) ? (t == 0 ? 0 : (t < N ? 1 : N)) : t;
+  return t;
 }
 
 __host__ __device__ inline size_t carve_work(char* base, int nx, int nu, int N, int ls_doubles, Work* w) {
   size_t off = 0;
-  if (w) {
-    w->shared = 0;
-    w->N = N;
-  }
   const size_t vbytes = align256((size_t)(N + 3) * nx * 8);
   for (int k = 0; k < kStreamVectors; ++k) {
     if (w) w->v[k] = reinterpret_cast<double*>(base + off);
@@ -222,8 +233,6 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
   if (a.shared_blocks && t > 1 && t < N) return;  // the same as knot 1
   Work w;
   carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, a.ls_doubles, &w);
-  w.shared = a.shared_blocks;
-  w.N = N;
   const int i = threadIdx.x;
   const double* wx = a.wx + (size_t)b * NX;
   const double* wu = a.wu + (size_t)b * NU;
@@ -333,11 +342,11 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
 }
 
 // D_t v: sum_j D_t[lane][j] v_j (v_j in lane j of the group), 0 when !ok.
-template <int G, int NX>
-__device__ __forceinline__ double dmul_mem(const Work& w, int t, double v, int lane, bool ok) {
+template <int G, int NX, bool SH>
+__device__ __forceinline__ double dmul_mem(const Work& w, int t, int N, int on, double v, int lane, bool ok) {
   const int l = lane < NX ? lane : 0;
   double acc;
-  t = shared_d(w, t);
+  t = shared_d<SH>(t, N, on);
   if constexpr (compact_blocks(NX)) {
     const double* D = w.DT + (size_t)t * tri(NX, 0);
     const int lrow = tri(l, 0);
@@ -354,11 +363,10 @@ __device__ __forceinline__ double dmul_mem(const Work& w, int t, double v, int l
 // The lane's entry of column 0 of A_{t-1} (row `li`) and the stride to the
 // next column, in whichever copy the block layout reads; B_{t-1} follows at
 // NX strides.
-template <int NX, int NU>
-__device__ __forceinline__ const float* arow_mem(const float* ABb, const Work& w, int t, int li) {
+template <int NX, int NU, bool SH>
+__device__ __forceinline__ const float* arow_mem(const float* ABb, const float* AT, int t, int li, int on) {
   constexpr int NZ = NX + NU;
-  const float* AT = w.AT;
-  const size_t tp = (size_t)shared_ab(w, t >= 1 ? t - 1 : 0);
+  const size_t tp = (size_t)shared_ab<SH>(t >= 1 ? t - 1 : 0, on);
   if constexpr (compact_blocks(NX)) return ABb + tp * NX * NZ + li * NZ;
   else return AT + tp * NZ * NX + li;
 }
@@ -387,9 +395,9 @@ struct Blocks {
 // PCG on S lam = gamma (pcg.py:350-445) with the vectors in the workspace.
 // gamma in V_GAM, the guess in V_LAM; the solution is left in V_LAM.  Same
 // phases, products and reductions as pcg() of the register variant.
-template <int G, int NX, int NU>
+template <int G, int NX, int NU, bool SH>
 __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostView<NX, NU>& c, const float* ABb,
-                          double* red3, double eps, int max_iter, double* eta_out) {
+                          double* red3, double eps, int max_iter, double* eta_out, int on) {
   constexpr int NZ = NX + NU;
   double* LAM = w.v[V_LAM];
   double* P = w.v[V_P];
@@ -404,8 +412,8 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   const double* GAM = w.v[V_GAM];
   const int lane = K.lane, N = K.N;
   const int li = lane < NX ? lane : 0;
-  auto acol = [&](int t) { return ABb + (size_t)shared_ab(w, t < N ? t : 0) * NX * NZ + li; };        // stride NZ
-  auto arow = [&](int t) { return arow_mem<NX, NU>(ABb, w, t, li); };
+  auto acol = [&](int t) { return ABb + (size_t)shared_ab<SH>(t < N ? t : 0, on) * NX * NZ + li; };  // stride NZ
+  auto arow = [&](int t) { return arow_mem<NX, NU, SH>(ABb, w.AT, t, li, on); };
   constexpr int AS = arow_stride<NX>();
   auto has_a = [&](int t) { return t < N; };
   auto has_p = [&](int t) { return t >= 1; };
@@ -438,7 +446,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     const double g = (t == 0 ? 1.0 : -1.0) * vget<NX>(YXO, t, lane) + ((ok && has_p(t)) ? au : 0.0) +
                      vget<NX>(GB, t, lane);
     const double rv = ok ? vget<NX>(GAM, t, lane) + g : 0.0;
-    const double y = dmul_mem<G, NX>(w, t, rv, lane, ok);
+    const double y = dmul_mem<G, NX, SH>(w, t, N, on, rv, lane, ok);
     if (ok) {
       R[(t + 1) * NX + lane] = rv;
       Y[(t + 1) * NX + lane] = y;
@@ -456,7 +464,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     const double rv = vget<NX>(R, t, lane);
     const double qi = ok ? c.qinv(t, lane) : 0.0;
     const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
-    const double rt = dmul_mem<G, NX>(w, t, z, lane, ok);
+    const double rt = dmul_mem<G, NX, SH>(w, t, N, on, z, lane, ok);
     if (ok) {
       part += rv * rt;
       RT[(t + 1) * NX + lane] = rt;
@@ -507,7 +515,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
         LAM[(t + 1) * NX + lane] = vget<NX>(LAM, t, lane) + alpha * vget<NX>(P, t, lane);
         rv += alpha * g;
       }
-      const double y = dmul_mem<G, NX>(w, t, rv, lane, ok);
+      const double y = dmul_mem<G, NX, SH>(w, t, N, on, rv, lane, ok);
       if (ok) {
         R[(t + 1) * NX + lane] = rv;
         Y[(t + 1) * NX + lane] = y;
@@ -526,7 +534,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
       const double rv = vget<NX>(R, t, lane);
       const double qi = ok ? c.qinv(t, lane) : 0.0;
       const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
-      const double rt = dmul_mem<G, NX>(w, t, z, lane, ok);
+      const double rt = dmul_mem<G, NX, SH>(w, t, N, on, z, lane, ok);
       if (ok) {
         epart += rv * rt;
         RT[(t + 1) * NX + lane] = rt;
@@ -543,10 +551,10 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
 }
 
 // x_t and u_{t-1} of block t (pcg.py:316-348) from V_LAM, see recover_rows.
-template <int G, int NX, int NU>
+template <int G, int NX, int NU, bool SH>
 __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r, const Work& w,
                                                const CostView<NX, NU>& c, const float* ABb, double* xn,
-                                               double* un) {
+                                               double* un, int on) {
   constexpr int NZ = NX + NU;
   const int t = K.t(r), lane = K.lane, N = K.N;
   const bool ok = K.ok(r);
@@ -554,9 +562,9 @@ __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r
   const double* LAM = w.v[V_LAM];
   const double lt = vget<NX>(LAM, t, lane);
   const double ln = vget<NX>(LAM, t + 1, lane);
-  const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab(w, t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
+  const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab<SH>(t < N ? t : 0, on) * NX * NZ + li, NZ, ln, lane);
   constexpr int AS = arow_stride<NX>();
-  const float* brow = arow_mem<NX, NU>(ABb, w, t, li) + NX * AS;
+  const float* brow = arow_mem<NX, NU, SH>(ABb, w.AT, t, li, on) + NX * AS;
   double bl = 0.0;
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
@@ -588,7 +596,7 @@ __device__ void stream_init_vectors(const Work& w, int Np, int N, const double* 
 // One SQP iteration of one instance (streamed): gamma, PCG, recovery, line
 // search, step test and outputs.  The block matrices were assembled by
 // stream_assemble_kernel for this linearization.
-template <class Model, int G, int NV>
+template <class Model, int G, int NV, bool SH>
 __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
@@ -601,8 +609,6 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, Np);
   Work w;
   carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, lsd, &w);
-  w.shared = a.shared_blocks;
-  w.N = N;
   vectors_in_lds<NV, NX>(&w, stream_vlds, Np);
   Blocks<G, NX, NU> K;
   K.init(N);
@@ -633,7 +639,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   }
   __syncthreads();
   double eta;
-  const int npcg = stream_pcg<G, NX, NU>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta);
+  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta, a.shared_blocks);
   // recovery: the QP solution into XN / UN (free PCG vectors) and the step
   // to it into DX / DU; with a line search the new iterate is
   // xbar + alpha * step, otherwise the QP solution itself
@@ -641,7 +647,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   double* UN = w.v[V_AP];  // (N+3) nx >= (N+1) nu doubles
   for (int r = 0; r < K.rounds; ++r) {
     double xn, un;
-    stream_recover<G, NX, NU>(K, r, w, c, ABb, &xn, &un);
+    stream_recover<G, NX, NU, SH>(K, r, w, c, ABb, &xn, &un, a.shared_blocks);
     if (K.ok(r)) {
       const int t = K.t(r), lane = K.lane, tu = ctl_row(t, N);
       XN[t * NX + lane] = xn;
@@ -655,7 +661,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   __syncthreads();
   double alpha = 1.0;
   if (a.ls_eval) {
-    alpha = line_search<Model>(a, b, N, xb, ub, w.DX, w.DU, w.LS, red3);
+    alpha = line_search<Model>(a, b, N, xb, ub, w.DX, w.DU, w.LS, red3, threadIdx.x, blockDim.x);
     for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) XN[k] = xb[k] + alpha * w.DX[k];
     for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) UN[k] = ub[k] + alpha * w.DU[k];
   }
@@ -687,7 +693,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
 // Implicit-differentiation backward (sqp.py:605-798), streamed: adjoint PCG
 // with q = -dl/dx, r = -dl/du[:-1], C = 0 on the saved linearization, then
 // the closed-form contraction of the register variant's backward.
-template <class Model, int G, int NV>
+template <class Model, int G, int NV, bool SH>
 __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel(BackwardArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
@@ -697,8 +703,6 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
   const int N = horizon_of(a.horizons, Np, b), NB = N + 1;
   Work w;
   carve_work(a.work + (size_t)b * a.work_stride, NX, NU, Np, 0, &w);
-  w.shared = a.shared_blocks;
-  w.N = N;
   vectors_in_lds<NV, NX>(&w, stream_vlds, Np);
   Blocks<G, NX, NU> K;
   K.init(N);
@@ -729,7 +733,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
   }
   __syncthreads();
   double eta;
-  const int npcg = stream_pcg<G, NX, NU>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta);
+  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta, a.shared_blocks);
   // recovery of x~, u~ with q~, r~; per-row contraction terms into DX / DU
   const double* xs = a.x + (size_t)b * NBs * NX;
   const double* us = a.u + (size_t)b * NBs * NU;
@@ -740,9 +744,10 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
     const double* LAM = w.v[V_LAM];
     const double lt = vget<NX>(LAM, t, lane);
     const double ln = vget<NX>(LAM, t + 1, lane);
-    const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab(w, t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
+    const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab<SH>(t < N ? t : 0, a.shared_blocks) * NX * NZ + li, NZ,
+                                      ln, lane);
     constexpr int AS = arow_stride<NX>();
-    const float* brow = arow_mem<NX, NU>(ABb, w, t, li) + NX * AS;
+    const float* brow = arow_mem<NX, NU, SH>(ABb, w.AT, t, li, a.shared_blocks) + NX * AS;
     double bl = 0.0;
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
@@ -867,26 +872,58 @@ int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st);
 constexpr int kHotVectors = 7;
 constexpr size_t kStreamLdsLimit = 160 * 1024 - 1024;
 
-template <class Model, int G>
-int lds_vectors(int N, bool forward) {
-  constexpr int NX = Model::NX;
+// Dynamic LDS limit of the LDS-vector kernels of one sharing flag, once.
+template <class Model, int G, bool SH>
+bool set_lds_limit(bool forward) {
   static bool attr[2] = {false, false};
+  if (attr[forward]) return true;
+  const void* k1 = forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kStreamVectors, SH>)
+                           : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kStreamVectors, SH>);
+  const void* k2 = forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors, SH>)
+                           : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kHotVectors, SH>);
+  if (hipFuncSetAttribute(k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess ||
+      hipFuncSetAttribute(k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess)
+    return false;
+  attr[forward] = true;
+  return true;
+}
+
+template <class Model, int G>
+int lds_vectors(int N, bool forward, bool shared) {
+  constexpr int NX = Model::NX;
   int nv = 0;
   // the recovery writes the (N+1) x nu controls into V_AP, which may then
   // run into the following vectors: all eleven in LDS only when nu <= nx
   if (Model::NU <= NX && vector_lds_bytes(kStreamVectors, NX, N) <= kStreamLdsLimit) nv = kStreamVectors;
   else if (vector_lds_bytes(kHotVectors, NX, N) <= kStreamLdsLimit) nv = kHotVectors;
-  if (nv && !attr[forward]) {
-    const void* k1 = forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kStreamVectors>)
-                             : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kStreamVectors>);
-    const void* k2 = forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors>)
-                             : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kHotVectors>);
-    if (hipFuncSetAttribute(k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess ||
-        hipFuncSetAttribute(k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess)
-      return 0;
-    attr[forward] = true;
+  if (!nv) return 0;
+  bool ok;
+  if constexpr (time_invariant<Model>::value) {
+    ok = shared ? set_lds_limit<Model, G, true>(forward) : set_lds_limit<Model, G, false>(forward);
+  } else {
+    ok = set_lds_limit<Model, G, false>(forward);
   }
-  return nv;
+  return ok ? nv : 0;
+}
+
+template <class Model, int G, bool SH>
+void launch_stream_qp(const SolveArgs& a, int nv, int threads, size_t lds, hipStream_t st) {
+  if (nv == kStreamVectors)
+    hipLaunchKernelGGL((stream_qp_kernel<Model, G, kStreamVectors, SH>), dim3(a.B), dim3(threads), lds, st, a);
+  else if (nv == kHotVectors)
+    hipLaunchKernelGGL((stream_qp_kernel<Model, G, kHotVectors, SH>), dim3(a.B), dim3(threads), lds, st, a);
+  else
+    hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0, SH>), dim3(a.B), dim3(threads), 0, st, a);
+}
+
+template <class Model, int G, bool SH>
+void launch_stream_bwd(const BackwardArgs& a, int nv, int threads, size_t lds, hipStream_t st) {
+  if (nv == kStreamVectors)
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kStreamVectors, SH>), dim3(a.B), dim3(threads), lds, st, a);
+  else if (nv == kHotVectors)
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kHotVectors, SH>), dim3(a.B), dim3(threads), lds, st, a);
+  else
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0, SH>), dim3(a.B), dim3(threads), 0, st, a);
 }
 
 template <class Model, int G>
@@ -897,7 +934,7 @@ int stream_forward_t(int model, const SolveArgs& a0, hipStream_t st) {
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
   const AssembleArgs aa = assemble_args<Model>(a, lsd);
   const int threads = stream_threads(a.N, G, stream_max_threads<Model>());
-  const int nv = lds_vectors<Model, G>(a.N, true);
+  const int nv = lds_vectors<Model, G>(a.N, true, a.shared_blocks);
   const size_t lds = vector_lds_bytes(nv, NX, a.N);
   int rc = stream_launch_reset(a, st);
   if (rc) return rc;
@@ -909,12 +946,12 @@ int stream_forward_t(int model, const SolveArgs& a0, hipStream_t st) {
     prof_begin(PROF_QP_ITERATION, st);
     hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0,
                        st, aa);
-    if (nv == kStreamVectors)
-      hipLaunchKernelGGL((stream_qp_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(threads), lds, st, ai);
-    else if (nv == kHotVectors)
-      hipLaunchKernelGGL((stream_qp_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(threads), lds, st, ai);
-    else
-      hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0>), dim3(a.B), dim3(threads), 0, st, ai);
+    if constexpr (time_invariant<Model>::value) {
+      if (a.shared_blocks) launch_stream_qp<Model, G, true>(ai, nv, threads, lds, st);
+      else launch_stream_qp<Model, G, false>(ai, nv, threads, lds, st);
+    } else {
+      launch_stream_qp<Model, G, false>(ai, nv, threads, lds, st);
+    }
     prof_end(PROF_QP_ITERATION, st);
   }
   rc = stream_launch_finish(model, a, st);
@@ -953,15 +990,15 @@ int stream_backward_t(const BackwardArgs& a0, hipStream_t st) {
   prof_begin(PROF_BACKWARD, st);
   hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0, st,
                      aa);
-  const int nv = lds_vectors<Model, G>(a.N, false);
+  const int nv = lds_vectors<Model, G>(a.N, false, a.shared_blocks);
   const size_t lds = vector_lds_bytes(nv, NX, a.N);
   const int threads = stream_threads(a.N, G, stream_max_threads<Model>());
-  if (nv == kStreamVectors)
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kStreamVectors>), dim3(a.B), dim3(threads), lds, st, ab);
-  else if (nv == kHotVectors)
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kHotVectors>), dim3(a.B), dim3(threads), lds, st, ab);
-  else
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0>), dim3(a.B), dim3(threads), 0, st, ab);
+  if constexpr (time_invariant<Model>::value) {
+    if (a.shared_blocks) launch_stream_bwd<Model, G, true>(ab, nv, threads, lds, st);
+    else launch_stream_bwd<Model, G, false>(ab, nv, threads, lds, st);
+  } else {
+    launch_stream_bwd<Model, G, false>(ab, nv, threads, lds, st);
+  }
   prof_end(PROF_BACKWARD, st);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
 }
