@@ -383,6 +383,7 @@ def bench_config0(args):
     import torch
 
     from diffmpc_amd import _lib
+    from diffmpc_amd import autotune
     from diffmpc_amd import workloads as W
     from diffmpc_amd.graph import GraphedStep
 
@@ -391,6 +392,7 @@ def bench_config0(args):
     for B in (1, 4096):
         name, prob, sp = W.lti(batch=B, seed=0)
         solver, params, guess = W.make_solver(name, prob, sp, device=dev)
+        shapes = None
 
         def fwd():
             with torch.no_grad():
@@ -404,6 +406,16 @@ def bench_config0(args):
                                                "initial_state": x0})
             return torch.autograd.grad(0.5 * sol.states.square().sum(), [wx, x0])
 
+        if B > 1:
+            # the batch packs several instances per workgroup (an instance uses
+            # 84 lanes): graph-replayed forward solve per packing, then the
+            # autotuner's choice of launch shape for the timed lines
+            shapes = {}
+            for ipw in (1, 2, 3, 4):
+                solver.params["instances_per_workgroup"] = 0 if ipw == 1 else ipw
+                t, _ = _timed(GraphedStep(fwd).replay, args.steps, args.warmup)
+                shapes[str(ipw)] = t * 1e3
+            best, _ = autotune.tune(solver, params, guess, reps=3)
         roof, sol = _qp_roofline(_lib, fwd, prob["horizon"], 4, 1, lambda o: int(o.num_pcg_iters.sum()))
         steps = max(args.steps, 100) if B == 1 else args.steps
         gf, gb = GraphedStep(fwd), GraphedStep(fwd_bwd)
@@ -411,6 +423,9 @@ def bench_config0(args):
         t_b, _ = _timed(gb.replay, steps, args.warmup)
         res[B] = {"solves_per_s": B / t_f, "grads_per_s": B / t_b, "ms_fwd": t_f * 1e3, "ms_fwd_bwd": t_b * 1e3,
                   "pcg_iterations": int(sol.num_pcg_iters.sum()), "roofline": roof}
+        if shapes:
+            res[B]["launch_shape"] = list(best)
+            res[B]["ms_fwd_by_instances_per_workgroup"] = shapes
     out = {
         "metric": "config 0: LTI MPC QP solves/s, one instance (nx=4 nu=2 T=20, 1 SQP iteration, PCG 1e-10)",
         "value": res[1]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": max(args.steps, 100),
@@ -537,7 +552,7 @@ def bench_ragged(args):
             groups.append((solver, params, guess, w))
             info[f"{name}/{part}"] = {"instances": int(len(hs)), "max_horizon": int(hs.max()),
                                       "launch_shape": list(best),
-                                      "tuning_ms": {f"{k[0]}/{k[1]}": v for k, v in timings.items()}}
+                                      "tuning_ms": {"/".join(map(str, k)): v for k, v in timings.items()}}
     # per-group roofline of the forward QP kernels (one eager forward solve
     # with the library's launch events): register-resident groups against
     # the FP64 VALU peak (algorithmic PCG flops), streamed groups against HBM

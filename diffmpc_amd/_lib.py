@@ -52,7 +52,7 @@ class SolverParams(ctypes.Structure):
         ("n_alphas", ctypes.c_int32), ("linesearch_alphas", ctypes.c_double * 8),
         ("pcg_max_iter", ctypes.c_int32), ("pcg_tol_epsilon", ctypes.c_double),
         ("warm_start_backward", ctypes.c_int32), ("blocks_per_group", ctypes.c_int32),
-        ("fused_variant", ctypes.c_int32),
+        ("fused_variant", ctypes.c_int32), ("instances_per_workgroup", ctypes.c_int32),
     ]
 
 
@@ -123,9 +123,10 @@ PROF_LINEARIZE = 2
 PROF_DYN_GRADS = 3
 
 
-def profile_read(kernel):
-    """(total ms, launches) of one instrumented kernel since the last read."""
-    lib = load()
+def profile_read(kernel, lib=None):
+    """(total ms, launches) of one instrumented kernel since the last read
+    (of libdmpc.so, or of the loaded library `lib`)."""
+    lib = lib or load()
     ms = ctypes.c_double(0.0)
     n = ctypes.c_int32(0)
     check(lib.dmpc_profile_read(kernel, ctypes.byref(ms), ctypes.byref(n)), "dmpc_profile_read")

@@ -2,14 +2,20 @@
 
 A fused solve can run as the register-resident kernels with K = 1, 2 or 4
 time blocks per lane group (K trades threads per instance against registers
-per thread: the on-chip tiling of the block-tridiagonal system), or as the
-streamed variant that keeps the blocks in a device workspace.  ``tune`` times
-every candidate the library accepts on the caller's own problem (forward
-plus backward, HIP events on the current stream), writes the fastest into the
-solver's params (``fused_variant``, ``blocks_per_group``) and caches it per
-problem shape (model, sizes, batch, ragged horizons, mode, line search, SQP
-and PCG settings).  No reference counterpart:
-XLA picks its own launch shapes.
+per thread: the on-chip tiling of the block-tridiagonal system), with one
+block per group also as 2, 3 or 4 instances per workgroup (teams aligned to
+16 lanes, which fill the wavefronts that a short instance of a small model
+leaves mostly idle), or as the streamed variant that keeps the blocks in a
+device workspace.  A candidate is (variant, blocks per group, instances per
+workgroup).  ``tune`` times every candidate the library accepts on the
+caller's own problem (forward plus backward: the library's HIP events
+around the QP-iteration and adjoint launches, i.e. the kernels a launch shape
+changes, so host overhead does not blur small problems; wall clock on the
+current stream where the library records no launches), writes the fastest
+into the solver's params (``fused_variant``,
+``blocks_per_group``, ``instances_per_workgroup``) and caches it per problem
+shape (model, sizes, batch, ragged horizons, mode, line search, SQP and PCG
+settings).  No reference counterpart: XLA picks its own launch shapes.
 """
 import hashlib
 from typing import Dict, List, Tuple
@@ -18,9 +24,12 @@ import torch
 
 from . import _lib
 
-CANDIDATES: List[Tuple[str, int]] = [("register", 1), ("register", 2), ("register", 4), ("streamed", 0)]
+CANDIDATES: List[Tuple[str, int, int]] = [
+    ("register", 1, 1), ("register", 2, 1), ("register", 4, 1), ("streamed", 0, 1),
+    ("register", 1, 2), ("register", 1, 3), ("register", 1, 4),
+]
 
-_cache: Dict[tuple, Tuple[str, int]] = {}
+_cache: Dict[tuple, Tuple[str, int, int]] = {}
 
 
 def register_horizon_limit(model: int, nx: int, nu: int, linesearch: bool = False, n_alphas: int = 1,
@@ -61,7 +70,9 @@ def _key(solver, params, B):
 
 
 def _apply(solver, choice):
-    solver.params["fused_variant"], solver.params["blocks_per_group"] = choice
+    variant, k, ipw = choice
+    solver.params["fused_variant"], solver.params["blocks_per_group"] = variant, k
+    solver.params["instances_per_workgroup"] = 0 if ipw == 1 else ipw
 
 
 def tune(solver, params, guess, reps: int = 2, backward: bool = True, candidates=CANDIDATES):
@@ -87,7 +98,10 @@ def tune(solver, params, guess, reps: int = 2, backward: bool = True, candidates
         return sol
 
     timings = {}
-    saved = (solver.params.get("fused_variant", "auto"), solver.params.get("blocks_per_group", 0))
+    saved = (solver.params.get("fused_variant", "auto"), solver.params.get("blocks_per_group", 0),
+             solver.params.get("instances_per_workgroup", 0) or 1)
+    lib = _lib.load_for(solver.program.dynamics)
+    kernels = (_lib.PROF_QP_ITERATION, _lib.PROF_BACKWARD)
     for choice in candidates:
         _apply(solver, choice)
         try:
@@ -96,13 +110,21 @@ def tune(solver, params, guess, reps: int = 2, backward: bool = True, candidates
             if "unsupported" in str(e):
                 continue
             raise
+        torch.cuda.synchronize()
+        for k in kernels:
+            _lib.profile_read(k, lib)
         start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        start.record()
-        for _ in range(reps):
-            run()
-        end.record()
-        end.synchronize()
-        timings[choice] = start.elapsed_time(end) / reps
+        lib.dmpc_profile_enable(1)
+        try:
+            start.record()
+            for _ in range(reps):
+                run()
+            end.record()
+            end.synchronize()
+        finally:
+            lib.dmpc_profile_enable(0)
+        kernel_ms = sum(_lib.profile_read(k, lib)[0] for k in kernels)
+        timings[choice] = (kernel_ms if kernel_ms > 0.0 else start.elapsed_time(end)) / reps
     if not timings:
         _apply(solver, saved)
         raise _lib.DmpcError("no fused launch shape supports this problem")

@@ -124,6 +124,11 @@ int main(void) {
     t = s;
     t.fused_variant = 5;
     EXPECT(dmpc_sqp_solve(&p, &t, &data, &sol, &saved, DMPC_MODE_FUSED, NULL), DMPC_ERR_ARG);
+    t = s;
+    t.instances_per_workgroup = 9;
+    EXPECT(dmpc_sqp_solve(&p, &t, &data, &sol, &saved, DMPC_MODE_FUSED, NULL), DMPC_ERR_ARG);
+    t.instances_per_workgroup = -1;
+    EXPECT(dmpc_sqp_solve(&p, &t, &data, &sol, &saved, DMPC_MODE_FUSED, NULL), DMPC_ERR_ARG);
   }
   {
     dmpc_problem_data e = data;

@@ -104,8 +104,8 @@ int check_problem(const dmpc_problem* p) {
 }
 
 bool tuning_ok(const dmpc_solver_params* s) {
-  const int k = s->blocks_per_group, v = s->fused_variant;
-  return (k == 0 || k == 1 || k == 2 || k == 4) && v >= 0 && v <= 3;
+  const int k = s->blocks_per_group, v = s->fused_variant, ipw = s->instances_per_workgroup;
+  return (k == 0 || k == 1 || k == 2 || k == 4) && v >= 0 && v <= 3 && ipw >= 0 && ipw <= 8;
 }
 
 bool linesearch_is_noop(const dmpc_solver_params* s) {
@@ -255,6 +255,7 @@ int sqp_solve_impl(const dmpc_problem* prob, const dmpc_solver_params* params, c
   a.nu = prob->nu;
   a.forced_K = params->blocks_per_group;
   a.variant = params->fused_variant;
+  a.ipw = params->instances_per_workgroup;
   a.dt = (float)prob->dt;
   a.dt64 = prob->dt;
   a.maxit = params->num_scp_iteration_max;
@@ -346,6 +347,7 @@ int dmpc_sqp_backward(const dmpc_problem* prob, const dmpc_solver_params* params
   a.pcg_eps = params->pcg_tol_epsilon;
   a.forced_K = params->blocks_per_group;
   a.variant = params->fused_variant;
+  a.ipw = params->instances_per_workgroup;
   a.x0 = data->x0;
   a.xf = data->x_final;
   a.xref = data->x_ref;

@@ -84,10 +84,15 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // partial reads need no EXEC mask.
 // The two halves are separate so that a caller can issue loads that only
 // need the barrier (neighbour vector entries) between them.
-__device__ __forceinline__ void block_sum_rows_publish(double v, double* red) {
+// `row`: the publishing row's index in `red` (threadIdx.x / 16, or the
+// row's index inside its team when a workgroup holds several instances)
+__device__ __forceinline__ void block_sum_rows_publish(double v, double* red, int row) {
   v = gsum<16>(v);
-  if ((threadIdx.x & 15) == 0) red[threadIdx.x >> 4] = v;
+  if ((threadIdx.x & 15) == 0) red[row] = v;
   __syncthreads();
+}
+__device__ __forceinline__ void block_sum_rows_publish(double v, double* red) {
+  block_sum_rows_publish(v, red, threadIdx.x >> 4);
 }
 __device__ __forceinline__ void block_sum_rows_load(const double* red, double (&q)[4]) {
   const double* p = red + (threadIdx.x & 15);
@@ -105,9 +110,12 @@ __device__ __forceinline__ double block_sum_rows_finish(const double* red) {
   block_sum_rows_load(red, q);
   return block_sum_rows_sum(q);
 }
-__device__ __forceinline__ double block_sum_rows(double v, double* red) {
-  block_sum_rows_publish(v, red);
+__device__ __forceinline__ double block_sum_rows(double v, double* red, int row) {
+  block_sum_rows_publish(v, red, row);
   return block_sum_rows_finish(red);
+}
+__device__ __forceinline__ double block_sum_rows(double v, double* red) {
+  return block_sum_rows(v, red, threadIdx.x >> 4);
 }
 
 // a / b for the PCG step lengths: the hardware reciprocal refined by two
@@ -129,6 +137,19 @@ __device__ __forceinline__ double block_max(double v, double* red) {
   const int nw = blockDim.x >> 6;
   double s = red[0];
   for (int k = 1; k < nw; ++k) s = nanmax(s, red[k]);
+  return s;
+}
+
+// block_max over the rows of one team (rows 0 .. nrows - 1 of `red`, `row`
+// this row's index there).  Maxima are exact, so for values without signed
+// zeros (the step norms it takes) the result equals block_max's.
+__device__ __forceinline__ double block_max_rows(double v, double* red, int row, int nrows) {
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) v = nanmax(v, __shfl_xor(v, off, 16));
+  if ((threadIdx.x & 15) == 0) red[row] = v;
+  __syncthreads();
+  double s = red[0];
+  for (int k = 1; k < nrows; ++k) s = nanmax(s, red[k]);
   return s;
 }
 

@@ -116,6 +116,39 @@ __device__ Lds carve(void* base, int N, int ls = 0) {
 }
 
 
+// Instances per workgroup.  The register-resident one-block-per-group
+// kernels can hold `ipw` instances per workgroup as teams of `tt` threads, tt
+// a multiple of 16 so that no DPP row holds two instances: team j of
+// workgroup w solves launch slot w * ipw + j and owns the j-th LDS region.
+// Without teams (TEAMS = false) the workgroup is the single team.  Inside a
+// team, thread `tid` plays the role threadIdx.x plays in a one-instance
+// workgroup: same lane groups, same reduction rows, same sums.
+struct Team {
+  int tid;   // thread index inside the team
+  int nt;    // threads of the team
+  int idx;   // team index inside the workgroup
+  int slot;  // launch slot (instance order[slot], or slot)
+};
+
+// Waves per SIMD the kernels are compiled for: the launch bounds' own floor
+// without teams; five (at most 96 VGPRs) with teams.  The per-thread team
+// bases of the LDS arrays cost the team kernels about 20 VGPRs, which at
+// their 128-VGPR bound gives four waves per SIMD, so that a CU held no more
+// instances than without teams; at five, a CU holds 12 instances of 84 lanes
+// (two per workgroup) instead of 10 (spills stay outside the PCG loop).
+template <bool TEAMS>
+__host__ __device__ constexpr int team_waves_per_simd() { return TEAMS ? 5 : 1; }
+
+template <bool TEAMS>
+__device__ __forceinline__ Team team_of(int ipw, int tt) {
+  if constexpr (TEAMS) {
+    const int j = threadIdx.x / tt;
+    return Team{(int)threadIdx.x - j * tt, tt, j, (int)blockIdx.x * ipw + j};
+  } else {
+    return Team{(int)threadIdx.x, (int)blockDim.x, 0, (int)blockIdx.x};
+  }
+}
+
 // 16-lane groups with four controls keep B's rows column-permuted per quad
 // of lanes and form the B-terms by reduce-scatter (rs16/gather16); other
 // shapes use one group all-reduce per column.
@@ -497,15 +530,24 @@ __device__ __forceinline__ double bgram(const Regs<NX, NU, K>& R, int s, const L
 // caller synchronises before other lanes read L.lamS).  Returns the
 // iteration count.
 //
+// With several instances per workgroup (TEAMS) the loop runs until every
+// team has stopped: a team that has met its stop rule (or is not `live`)
+// keeps passing the barriers with its multipliers, residual, eta and count
+// frozen, so each team's results and count are those of its own solve.
+// Each iteration's team flags go through flags[] with the first workgroup
+// sum's barrier.
+//
 // S = -G H^-1 G^T is applied matrix-free.  Row block t of G H^-1 G^T p is
 //   (t == 0 ? 1 : -1) Qinv_t w_t + A_{t-1} Qinv_{t-1} w_{t-1} + B_{t-1} Rinv B_{t-1}^T p_t
 // with w_t = (t == 0 ? p_t : -p_t) + A_t^T p_{t+1}.  Each lane loads only its
 // own entry of a neighbour block's vector from LDS; products with a block's
 // matrices broadcast the entries inside the lane group with DPP, and the B
 // term is local to the block.  Three workgroup barriers per iteration.
-template <int G, int NX, int NU, int K>
+template <int G, bool TEAMS = false, int NX, int NU, int K>
 __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K], const Lds& L, int N, int lane,
-                   double eps, int max_iter, double* eta_out) {
+                   double eps, int max_iter, double* eta_out, const Team& T, bool live = true,
+                   int* flags = nullptr, int ipw = 1) {
+  const int row = T.tid >> 4;  // this row's workgroup-sum partial
   double r[K], p[K], rt[K], ap[K], yxo[K], gb[K];
   const bool lead = lane == 0;
   // ---- r = gamma - S lam0 -------------------------------------------------
@@ -556,16 +598,20 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
   // A_{t-1} u_{t-1} in phase B), are issued between the barrier and the
   // reading of the partials, so their LDS latency overlaps the sum's.
   double rn[K], apn[K];
-  block_sum_rows_publish(part, L.redC);
+  block_sum_rows_publish(part, L.redC, row);
 #pragma unroll
   for (int s = 0; s < K; ++s) rn[s] = own<NX>(L.RT, R.t[s] + 1, lane);
   gdot_k<G, NX>(R.acol, rn, lane, 0, apn);
   double eta = block_sum_rows_finish(L.redC);
   double beta = 0.0;
   int it = 0;
+  bool go = live;  // TEAMS: this team still iterates
   while (true) {
+    // (this form, not an if-constexpr block around the break, keeps the
+    // one-instance loop's code as it was: the block changed its scheduling)
     const bool cont = (fabs(eta) > eps && it <= max_iter - 1) || it < 1;
-    if (!cont) break;
+    if (!TEAMS && !cont) break;
+    if constexpr (TEAMS) go = go && cont;
     // ---- phase A: p = rt + beta p; w = G^T p; y = H^-1 w; v = -w.y --------
     // A_t^T p_{t+1} = A_t^T rt_{t+1} + beta A_t^T p_{t+1}(previous), so only
     // the neighbour's rt is read (apn, formed before beta was known); the
@@ -593,7 +639,15 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
         }
       }
     }
-    block_sum_rows_publish(vpart, L.redA);
+    if constexpr (TEAMS) {
+      if (T.tid == 0) flags[T.idx] = go;
+    }
+    block_sum_rows_publish(vpart, L.redA, row);
+    if constexpr (TEAMS) {
+      int any = 0;
+      for (int j = 0; j < ipw; ++j) any |= flags[j];
+      if (!any) break;  // the same flags in every thread: a uniform exit
+    }
     // ---- phase B: ups = S p; lam += alpha p; r -= alpha ups; y = D r -----
     {
       double u[K], au[K], y[K];
@@ -605,7 +659,7 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
 #pragma unroll
       for (int s = 0; s < K; ++s) {
         const double g = (R.t[s] == 0 ? 1.0 : -1.0) * yxo[s] + au[s] + gb[s];
-        if (R.ok[s]) {
+        if (R.ok[s] && (!TEAMS || go)) {
           lam[s] += alpha * p[s];
           r[s] += alpha * g;  // r -= alpha * ups, ups = -g
         }
@@ -642,15 +696,17 @@ __device__ int pcg(Regs<NX, NU, K>& R, const double (&gam)[K], double (&lam)[K],
         L.RT[(R.t[s] + 1) * NX + lane] = rt[s];
       }
     }
-    block_sum_rows_publish(epart, L.redB);
+    block_sum_rows_publish(epart, L.redB, row);
     // next iteration's A_t^T rt_{t+1} (unused if the loop ends here)
 #pragma unroll
     for (int s = 0; s < K; ++s) rn[s] = own<NX>(L.RT, R.t[s] + 1, lane);
     gdot_k<G, NX>(R.acol, rn, lane, it, apn);
     const double eta_new = block_sum_rows_finish(L.redB);
-    beta = div_newton(eta_new, eta);
-    eta = eta_new;
-    ++it;
+    if (!TEAMS || go) {
+      beta = div_newton(eta_new, eta);
+      eta = eta_new;
+      ++it;
+    }
   }
 #pragma unroll
   for (int s = 0; s < K; ++s)
@@ -735,10 +791,11 @@ __device__ __forceinline__ void recover_rows(const Regs<NX, NU, K>& R, const Lds
 __device__ __forceinline__ int control_row(int t, int N) { return t == 0 ? N : t - 1; }
 
 template <int NX, int NU, int K>
-__device__ __forceinline__ void init_blocks(Regs<NX, NU, K>& R, int G, int N, int* lane_out, int* gbase_out) {
-  const int NG = blockDim.x / G;
-  const int grp = threadIdx.x / G;
-  const int lane = threadIdx.x % G;
+__device__ __forceinline__ void init_blocks(Regs<NX, NU, K>& R, int G, int N, const Team& T, int* lane_out,
+                                            int* gbase_out) {
+  const int NG = T.nt / G;
+  const int grp = T.tid / G;
+  const int lane = T.tid % G;
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     R.t[s] = s * NG + grp;
@@ -750,13 +807,13 @@ __device__ __forceinline__ void init_blocks(Regs<NX, NU, K>& R, int G, int N, in
 }
 
 template <int NX, int NU>
-__device__ void zero_lds_vectors(const Lds& L, int N) {
+__device__ void zero_lds_vectors(const Lds& L, int N, const Team& T) {
   const int NBP = N + 3;
-  for (int k = threadIdx.x; k < NBP * NX; k += blockDim.x) {
+  for (int k = T.tid; k < NBP * NX; k += T.nt) {
     L.RT[k] = 0.0; L.Y[k] = 0.0; L.YQ[k] = 0.0; L.YX[k] = 0.0; L.lamS[k] = 0.0;
   }
   // block_sum_rows reads all 64 partials of each array
-  for (int k = threadIdx.x; k < 64; k += blockDim.x) {
+  for (int k = T.tid; k < 64; k += T.nt) {
     L.redA[k] = 0.0; L.redB[k] = 0.0; L.redC[k] = 0.0;
   }
 }
@@ -769,10 +826,10 @@ __device__ void zero_lds_vectors(const Lds& L, int N) {
 // at once (addresses clamped into the arrays, loads unconditional), then
 // stores to LDS.  N: this instance's horizon; Np: the padded one.
 template <int NX, int NU>
-__device__ void stage_forward(const SolveArgs& a, const Lds& L, int b, int N, int Np) {
+__device__ void stage_forward(const SolveArgs& a, const Lds& L, int b, int N, int Np, const Team& T) {
   const int NB = N + 1, NBs = Np + 1;
   const int nxe = NB * NX, nue = NB * NU, nfd = N * NX;
-  for (int k = threadIdx.x; k < nxe; k += blockDim.x) {
+  for (int k = T.tid; k < nxe; k += T.nt) {
     const int t = k / NX, i = k % NX;
     const int ku = k < nue ? k : nue - 1, kf = k < nfd ? k : nfd - 1;
     const int tu = ku / NU, iu = ku % NU;
@@ -971,23 +1028,44 @@ __global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
 // evaluations hold enough registers to make the whole kernel spill at the
 // 128-VGPR cap of the one-block-per-group variant, so solves that do not
 // evaluate the line search run a build without it, which spills nothing.
-template <class Model, int G, int K, int MAXT, bool LS>
-__global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
+template <class Model, int G, int K, int MAXT, bool LS, bool TEAMS>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(team_waves_per_simd<TEAMS>())))
+qp_iteration_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   extern __shared__ double smem_[];
-  const int b = a.order ? a.order[blockIdx.x] : blockIdx.x;
-  if (!sqp_active(a, b)) return;
+  const Team T = team_of<TEAMS>(a.ipw, a.team_threads);
+  int b;
+  bool live = true;  // this team's instance is solved (else its team only keeps the barriers)
+  int* flags = nullptr;
+  char* lds = reinterpret_cast<char*>(smem_);
+  if constexpr (TEAMS) {
+    // a workgroup whose instances have all stopped returns at once
+    bool any = false;
+    for (int j = 0; j < a.ipw; ++j) {
+      const int sl = (int)blockIdx.x * a.ipw + j;
+      if (sl < a.B) any = any || sqp_active(a, a.order ? a.order[sl] : sl);
+    }
+    if (!any) return;
+    const int sl = T.slot < a.B ? T.slot : a.B - 1;  // slots past the batch read the last one
+    b = a.order ? a.order[sl] : sl;
+    live = T.slot < a.B && sqp_active(a, b);
+    flags = reinterpret_cast<int*>(lds + (size_t)a.ipw * a.team_lds);
+    lds += (size_t)T.idx * a.team_lds;
+  } else {
+    b = a.order ? a.order[blockIdx.x] : blockIdx.x;
+    if (!sqp_active(a, b)) return;
+  }
   const int it = a.num_iter[b];
   const int Np = a.N, NBs = Np + 1;                          // padded horizon, knot stride
   const int N = horizon_of(a.horizons, Np, b), NB = N + 1;  // this instance
-  const Lds L = carve<NX, NU>(smem_, Np, linesearch_doubles(a.ls_eval, a.n_alphas, Np));
+  const Lds L = carve<NX, NU>(lds, Np, linesearch_doubles(a.ls_eval, a.n_alphas, Np));
   Regs<NX, NU, K> R;
   int lane, gbase;
-  init_blocks(R, G, N, &lane, &gbase);
+  init_blocks(R, G, N, T, &lane, &gbase);
 
-  zero_lds_vectors<NX, NU>(L, Np);
+  zero_lds_vectors<NX, NU>(L, Np, T);
   __syncthreads();
-  stage_forward<NX, NU>(a, L, b, N, Np);
+  stage_forward<NX, NU>(a, L, b, N, Np, T);
   __syncthreads();
 
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
@@ -996,7 +1074,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
   double gam[K], lam[K];
   gamma_rows<true, G>(R, L, lane, gam);
   double eta;
-  const int npcg = pcg<G>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta);
+  const int npcg = pcg<G, TEAMS>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta, T, live, flags, a.ipw);
   __syncthreads();  // lam (in L.lamS) complete for the neighbour reads
   double xn[K], un[K];
   recover_rows<G>(R, L, N, lane, xn, un);
@@ -1013,7 +1091,7 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
       }
     }
     __syncthreads();
-    alpha = line_search<Model>(a, b, N, L.xb, L.ub, DX, DU, L.LS, L.redA);
+    alpha = line_search<Model, TEAMS>(a, b, N, L.xb, L.ub, DX, DU, L.LS, L.redA, T.tid, T.nt);
 #pragma unroll
     for (int s = 0; s < K; ++s) {
       if (R.ok[s]) {
@@ -1032,7 +1110,13 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
       if (lane < NU) e = nanmax(e, fabs(un[s] - L.ub[control_row(t, N) * NU + lane]));
     }
   }
-  const double err = block_max(e, L.redA);
+  double err;
+  if constexpr (TEAMS) {
+    err = block_max_rows(e, L.redA, T.tid >> 4, T.nt >> 4);
+  } else {
+    err = block_max(e, L.redA);
+  }
+  if (!live) return;  // no barrier follows
 #pragma unroll
   for (int s = 0; s < K; ++s) {
     if (R.ok[s]) {
@@ -1043,14 +1127,14 @@ __global__ void __launch_bounds__(MAXT) qp_iteration_kernel(SolveArgs a) {
     }
   }
   // knots past this instance's horizon (ragged batches) are zero
-  for (int k = NB * NX + threadIdx.x; k < NBs * NX; k += blockDim.x) {
+  for (int k = NB * NX + T.tid; k < NBs * NX; k += T.nt) {
     a.x[(size_t)b * NBs * NX + k] = 0.0;
     a.lam[(size_t)b * NBs * NX + k] = 0.0;
   }
-  for (int k = NB * NU + threadIdx.x; k < NBs * NU; k += blockDim.x) a.u[(size_t)b * NBs * NU + k] = 0.0;
-  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) a.qinv[(size_t)b * NBs * NX + k] = L.qinv[k];
-  for (int k = threadIdx.x; k < NU; k += blockDim.x) a.rinv[(size_t)b * NU + k] = L.rinv[k];
-  if (threadIdx.x == 0) {
+  for (int k = NB * NU + T.tid; k < NBs * NU; k += T.nt) a.u[(size_t)b * NBs * NU + k] = 0.0;
+  for (int k = T.tid; k < NB * NX; k += T.nt) a.qinv[(size_t)b * NBs * NX + k] = L.qinv[k];
+  for (int k = T.tid; k < NU; k += T.nt) a.rinv[(size_t)b * NU + k] = L.rinv[k];
+  if (T.tid == 0) {
     a.conv_errs[(size_t)b * a.maxit + it] = err;
     a.pcg_iters[(size_t)b * a.maxit + it] = npcg;
     if (a.linesearch) a.ls_alphas[(size_t)b * a.maxit + it] = alpha;
@@ -1087,23 +1171,34 @@ __global__ void __launch_bounds__(256) sqp_finish_kernel(SolveArgs a, int nx, in
 // Adjoint backward (sqp.py:668-740): PCG on the saved linearization with
 // q~ = -dl/dx, r~ = -dl/du[:-1], C = 0, recovery of (x~, u~), then the
 // closed-form mixed-derivative contraction of the diagonal tracking cost.
-template <class Model, int G, int K, int MAXT>
-__global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
+template <class Model, int G, int K, int MAXT, bool TEAMS>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(team_waves_per_simd<TEAMS>())))
+fused_backward_kernel(BackwardArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   extern __shared__ double smem_[];
-  const int b = blockIdx.x;
+  const Team T = team_of<TEAMS>(a.ipw, a.team_threads);
+  int b = blockIdx.x;
+  bool live = true;  // as in qp_iteration_kernel
+  int* flags = nullptr;
+  char* lds = reinterpret_cast<char*>(smem_);
+  if constexpr (TEAMS) {
+    b = T.slot < a.B ? T.slot : a.B - 1;  // slots past the batch read the last one
+    live = T.slot < a.B;
+    flags = reinterpret_cast<int*>(lds + (size_t)a.ipw * a.team_lds);
+    lds += (size_t)T.idx * a.team_lds;
+  }
   const int Np = a.N, NBs = Np + 1;                          // padded horizon, knot stride
   const int N = horizon_of(a.horizons, Np, b), NB = N + 1;  // this instance
-  const Lds L = carve<NX, NU>(smem_, Np);
+  const Lds L = carve<NX, NU>(lds, Np);
   Regs<NX, NU, K> R;
   int lane, gbase;
-  init_blocks(R, G, N, &lane, &gbase);
+  init_blocks(R, G, N, T, &lane, &gbase);
 
-  zero_lds_vectors<NX, NU>(L, Np);
+  zero_lds_vectors<NX, NU>(L, Np, T);
   __syncthreads();
   // one pass, all global loads of a thread issued at once (clamped
   // addresses), as stage_forward
-  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
+  for (int k = T.tid; k < NB * NX; k += T.nt) {
     const int ku = k < NB * NU ? k : NB * NU - 1;
     const size_t xo = (size_t)b * NBs * NX + k, uo = (size_t)b * NBs * NU + ku;
     const double qi = a.qinv[xo], dx = a.dl_dx[xo], lb = a.lam_bwd[xo], xv = a.x[xo];
@@ -1126,7 +1221,7 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   double gam[K], lam[K];
   gamma_rows<false, G>(R, L, lane, gam);
   double eta;
-  const int npcg = pcg<G>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta);
+  const int npcg = pcg<G, TEAMS>(R, gam, lam, L, N, lane, a.pcg_eps, a.pcg_max, &eta, T, live, flags, a.ipw);
   __syncthreads();  // lam (in L.lamS) complete for the neighbour reads
   double xt[K], ut[K];
   recover_rows<G>(R, L, N, lane, xt, ut);
@@ -1138,7 +1233,7 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
   double* UT = L.YQ;  // (NB, NU): u~
 #pragma unroll
   for (int s = 0; s < K; ++s) {
-    if (R.ok[s]) {
+    if (R.ok[s] && live) {
       const int t = R.t[s];
       const int i = lane;
       const double ref = (t < N) ? a.xref[((size_t)b * NBs + t) * NX + i] : a.xf[b * NX + i];
@@ -1167,13 +1262,14 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
     }
   }
   // knots past this instance's horizon (ragged batches) are zero
-  for (int k = NB * NX + threadIdx.x; k < NBs * NX; k += blockDim.x) {
+  for (int k = NB * NX + T.tid; live && k < NBs * NX; k += T.nt) {
     a.g_xref[(size_t)b * NBs * NX + k] = 0.0;
     a.lam_bwd[(size_t)b * NBs * NX + k] = 0.0;
   }
-  for (int k = NB * NU + threadIdx.x; k < NBs * NU; k += blockDim.x) a.g_uref[(size_t)b * NBs * NU + k] = 0.0;
+  for (int k = NB * NU + T.tid; live && k < NBs * NU; k += T.nt) a.g_uref[(size_t)b * NBs * NU + k] = 0.0;
   __syncthreads();
-  for (int i = threadIdx.x; i < NX + NU; i += blockDim.x) {
+  if (!live) return;  // no barrier follows
+  for (int i = T.tid; i < NX + NU; i += T.nt) {
     double acc = 0.0;
     if (i < NX) {
       for (int t = 0; t <= N; ++t) acc += TX[t * NX + i];
@@ -1185,19 +1281,23 @@ __global__ void __launch_bounds__(MAXT) fused_backward_kernel(BackwardArgs a) {
     }
   }
   if (a.adj_x) {
-    for (int k = threadIdx.x; k < NBs * NX; k += blockDim.x) a.adj_x[(size_t)b * NBs * NX + k] = k < NB * NX ? XT[k] : 0.0;
+    for (int k = T.tid; k < NBs * NX; k += T.nt) a.adj_x[(size_t)b * NBs * NX + k] = k < NB * NX ? XT[k] : 0.0;
   }
   if (a.adj_u) {
-    for (int k = threadIdx.x; k < NBs * NU; k += blockDim.x) a.adj_u[(size_t)b * NBs * NU + k] = k < N * NU ? UT[k] : 0.0;
+    for (int k = T.tid; k < NBs * NU; k += T.nt) a.adj_u[(size_t)b * NBs * NU + k] = k < N * NU ? UT[k] : 0.0;
   }
-  if (threadIdx.x == 0) a.pcg_iters[b] = npcg;
+  if (T.tid == 0) a.pcg_iters[b] = npcg;
 }
 
 constexpr size_t kMaxLds = 160 * 1024;
+constexpr int kMaxTeams = 8;  // instances per workgroup at most
 
 struct Launch {
   int K, threads;
   size_t lds;
+  int ipw = 1;        // instances per workgroup (teams)
+  int team_threads = 0;
+  size_t team_lds = 0;
 };
 
 // K blocks per thread group -> at most 1024 / 512 / 256 threads, so the
@@ -1241,15 +1341,39 @@ Launch plan(int N, int P, int ls = 0, int forced_K = 0) {
   return l;  // K = 0: does not fit
 }
 
-template <class Model, int G, int K, int MAXT, bool LS>
+// Several instances per workgroup (ipw > 1): teams of one block per lane
+// group, each of tt = (N + 1) G threads rounded up to a DPP row of 16, with
+// its own LDS region (the one-instance plan's bytes rounded to 16) and one
+// int flag per team after the regions.  Only for 4- and 8-lane groups,
+// whose short-horizon instances leave most of a wavefront's lanes idle when
+// alone; K = 0 when the request does not fit one workgroup.
+template <class Model, int G>
+Launch plan_teams(int N, int P, int ls, int forced_K, int ipw) {
+  Launch l{0, 0, 0};
+  const int want = forced_K > 0 ? forced_K : g_forced_K;
+  if (G > 8 || ipw < 2 || ipw > kMaxTeams || (want > 0 && want != 1)) return l;
+  const int tt = ((N + 1) * G + 15) / 16 * 16;
+  const size_t team_lds = (LdsPlan<Model::NX, Model::NU>::bytes(N, P, ls) + 15) / 16 * 16;
+  if (ipw * tt > max_threads(1)) return l;
+  l.K = 1;
+  l.threads = ipw * tt;
+  l.lds = (size_t)ipw * team_lds + kMaxTeams * sizeof(int);
+  l.ipw = ipw;
+  l.team_threads = tt;
+  l.team_lds = team_lds;
+  return l;
+}
+
+template <class Model, int G, int K, int MAXT, bool LS, bool TEAMS = false>
 int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&qp_iteration_kernel<Model, G, K, MAXT, LS>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&qp_iteration_kernel<Model, G, K, MAXT, LS, TEAMS>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) != hipSuccess)
       return DMPC_ERR_HIP;
     attr = true;
   }
+  const unsigned groups = (unsigned)((a.B + l.ipw - 1) / l.ipw);
   hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter);
   const long long nlin = linearize_threads<Model>(a.B, a.N);
   for (int it = 0; it < a.maxit; ++it) {
@@ -1259,32 +1383,52 @@ int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
     // the solution's status array is free until sqp_finish_kernel writes it
     SolveArgs ai = a;
     ai.order = launch_order(a, it, a.status, st) ? a.status : nullptr;
+    ai.ipw = l.ipw;
+    ai.team_threads = l.team_threads;
+    ai.team_lds = (int)l.team_lds;
     prof_begin(PROF_QP_ITERATION, st);
-    hipLaunchKernelGGL((qp_iteration_kernel<Model, G, K, MAXT, LS>), dim3(a.B), dim3(l.threads), l.lds, st, ai);
+    hipLaunchKernelGGL((qp_iteration_kernel<Model, G, K, MAXT, LS, TEAMS>), dim3(groups), dim3(l.threads), l.lds,
+                       st, ai);
     prof_end(PROF_QP_ITERATION, st);
   }
   hipLaunchKernelGGL(sqp_finish_kernel, dim3(a.B), dim3(256), 0, st, a, Model::NX, Model::NU);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
 }
 
-template <class Model, int G, int K, int MAXT>
+template <class Model, int G, int K, int MAXT, bool TEAMS = false>
 int launch_bwd_k(const BackwardArgs& a, const Launch& l, hipStream_t st) {
   static bool attr = false;
   if (!attr) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&fused_backward_kernel<Model, G, K, MAXT>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&fused_backward_kernel<Model, G, K, MAXT, TEAMS>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds) != hipSuccess)
       return DMPC_ERR_HIP;
     attr = true;
   }
+  BackwardArgs ai = a;
+  ai.ipw = l.ipw;
+  ai.team_threads = l.team_threads;
+  ai.team_lds = (int)l.team_lds;
+  const unsigned groups = (unsigned)((a.B + l.ipw - 1) / l.ipw);
   prof_begin(PROF_BACKWARD, st);
-  hipLaunchKernelGGL((fused_backward_kernel<Model, G, K, MAXT>), dim3(a.B), dim3(l.threads), l.lds, st, a);
+  hipLaunchKernelGGL((fused_backward_kernel<Model, G, K, MAXT, TEAMS>), dim3(groups), dim3(l.threads), l.lds, st,
+                     ai);
   prof_end(PROF_BACKWARD, st);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
 }
 
 template <class Model, int G>
 int fwd_model(const SolveArgs& a, hipStream_t st) {
-  Launch l = plan<Model, G>(a.N, a.P, linesearch_doubles(a.ls_eval, a.n_alphas, a.N), a.forced_K);
+  const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
+  if (a.ipw > 1) {
+    if constexpr (G <= 8) {
+      const Launch l = plan_teams<Model, G>(a.N, a.P, lsd, a.forced_K, a.ipw);
+      if (l.K != 1 || l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
+      return a.ls_eval ? launch_fwd_k<Model, G, 1, 1024, true, true>(a, l, st)
+                       : launch_fwd_k<Model, G, 1, 1024, false, true>(a, l, st);
+    }
+    return DMPC_ERR_UNSUPPORTED;
+  }
+  Launch l = plan<Model, G>(a.N, a.P, lsd, a.forced_K);
   if (l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
   switch (l.K) {
     case 1:
@@ -1301,6 +1445,14 @@ int fwd_model(const SolveArgs& a, hipStream_t st) {
 
 template <class Model, int G>
 int bwd_model(const BackwardArgs& a, hipStream_t st) {
+  if (a.ipw > 1) {
+    if constexpr (G <= 8) {
+      const Launch l = plan_teams<Model, G>(a.N, 0, 0, a.forced_K, a.ipw);
+      if (l.K != 1 || l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
+      return launch_bwd_k<Model, G, 1, 1024, true>(a, l, st);
+    }
+    return DMPC_ERR_UNSUPPORTED;
+  }
   Launch l = plan<Model, G>(a.N, 0, 0, a.forced_K);
   if (l.lds > kMaxLds) return DMPC_ERR_UNSUPPORTED;
   switch (l.K) {
@@ -1428,7 +1580,10 @@ int stream_launch_finish(int model, const SolveArgs& a, hipStream_t st) {
 int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st) {
   if (nu > nx) return DMPC_ERR_UNSUPPORTED;  // see fused_supported
   const int lsd = linesearch_doubles(a.ls_eval, a.n_alphas, a.N);
-  if (use_stream(model, nx, nu, a.N, lsd, a.variant)) return stream_forward(model, nx, nu, a, st);
+  if (use_stream(model, nx, nu, a.N, lsd, a.variant)) {
+    if (a.ipw > 1) return DMPC_ERR_UNSUPPORTED;  // teams are a register-resident launch shape
+    return stream_forward(model, nx, nu, a, st);
+  }
   if (!register_fits(model, nx, nu, a.N, lsd)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {
     case MODEL_LINEAR:
@@ -1449,7 +1604,10 @@ int fused_forward(int model, int nx, int nu, const SolveArgs& a, hipStream_t st)
 
 int fused_backward(int model, int nx, int nu, const BackwardArgs& a, hipStream_t st) {
   if (nu > nx) return DMPC_ERR_UNSUPPORTED;  // see fused_supported
-  if (use_stream(model, nx, nu, a.N, 0, a.variant)) return stream_backward(model, nx, nu, a, st);
+  if (use_stream(model, nx, nu, a.N, 0, a.variant)) {
+    if (a.ipw > 1) return DMPC_ERR_UNSUPPORTED;
+    return stream_backward(model, nx, nu, a, st);
+  }
   if (!register_fits(model, nx, nu, a.N, 0)) return DMPC_ERR_UNSUPPORTED;
   switch (model) {
     case MODEL_LINEAR:

@@ -50,6 +50,10 @@ struct SolveArgs {
   // streamed variant, time-invariant model: one copy of the blocks (see
   // shared_block in dmpc_stream.hip)
   int shared_blocks;
+  // instances per workgroup of the register-resident kernels (0 or 1: one);
+  // with more, each instance is a team of team_threads threads with
+  // team_lds bytes of LDS (set at launch)
+  int ipw, team_threads, team_lds;
 };
 
 struct BackwardArgs {
@@ -72,7 +76,8 @@ struct BackwardArgs {
   // null for the identity
   const int32_t* order;
   size_t order_stride;
-  int shared_blocks;  // as in SolveArgs
+  int shared_blocks;                   // as in SolveArgs
+  int ipw, team_threads, team_lds;     // as in SolveArgs
 };
 
 // SQP continue rule of sqp.py:460-466 for instance b:

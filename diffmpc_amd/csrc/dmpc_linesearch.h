@@ -27,11 +27,14 @@ namespace {
 // (candidate k, knot t) are spread over the workgroup and summed in a fixed
 // order, so every thread returns the same alpha.  LS holds
 // linesearch_doubles(1, n_alphas, Nb) doubles; red is a 16-double LDS
-// reduction buffer.  Problem arrays are read with the padded knot stride
-// a.N + 1.  Must be called by every thread of the workgroup.
-template <class Model>
+// reduction buffer (TEAMS: the team's 64-double block_sum_rows buffer, zero
+// past the team's rows).  Problem arrays are read with the padded knot
+// stride a.N + 1.  Must be called by every thread of the workgroup; tid and
+// nt are the thread's index and the thread count of its instance's team
+// (the workgroup's with one instance per workgroup).
+template <class Model, bool TEAMS = false>
 __device__ double line_search(const SolveArgs& a, int b, int Nb, const double* xb, const double* ub,
-                              const double* DX, const double* DU, double* LS, double* red) {
+                              const double* DX, const double* DU, double* LS, double* red, int tid, int nt) {
   constexpr int NX = Model::NX, NU = Model::NU;
   const int NB = Nb + 1, NBs = a.N + 1, nal = a.n_alphas, items = (nal + 1) * NB;
   double* LC = LS;               // cost part of each item
@@ -47,20 +50,25 @@ __device__ double line_search(const SolveArgs& a, int b, int Nb, const double* x
   const float* prm = a.dyn + (size_t)b * a.P;
   // directional derivative of the cost (the last control row has none)
   double part = 0.0;
-  for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) {
+  for (int k = tid; k < NB * NX; k += nt) {
     const int t = k / NX, i = k - t * NX;
     const double gx = (t < Nb) ? 2.0 * wx[i] * (xb[k] - xr[k]) : 2.0 * (wx[i] + wf[i]) * (xb[k] - xf[i]);
     part += gx * DX[k];
   }
-  for (int k = threadIdx.x; k < Nb * NU; k += blockDim.x) {
+  for (int k = tid; k < Nb * NU; k += nt) {
     const int j = k % NU;
     const double ref = a.penalize ? ur[k] : 0.0;
     part += 2.0 * wu[j] * (ub[k] - ref) * DU[k];
   }
-  const double dcdz = block_sum(part, red);
+  double dcdz;
+  if constexpr (TEAMS) {
+    dcdz = block_sum_rows(part, red, tid >> 4);
+  } else {
+    dcdz = block_sum(part, red);
+  }
   // merit parts: item (kc, t) holds the cost of knot t and the constraint
   // block t at alpha = 0 (kc = 0) or alphas[kc - 1]
-  for (int j = threadIdx.x; j < items; j += blockDim.x) {
+  for (int j = tid; j < items; j += nt) {
     const int kc = j / NB, t = j - kc * NB;
     const double al = kc == 0 ? 0.0 : a.alphas[kc - 1];
     double c = 0.0, g = 0.0;
@@ -99,14 +107,14 @@ __device__ double line_search(const SolveArgs& a, int b, int Nb, const double* x
     LG[j] = g;
   }
   __syncthreads();
-  if (threadIdx.x <= nal) {
+  if (tid <= nal) {
     double c = 0.0, g = 0.0;
     for (int t = 0; t < NB; ++t) {
-      c += LC[threadIdx.x * NB + t];
-      g += LG[threadIdx.x * NB + t];
+      c += LC[tid * NB + t];
+      g += LG[tid * NB + t];
     }
-    SUM[2 * threadIdx.x] = c;
-    SUM[2 * threadIdx.x + 1] = g;
+    SUM[2 * tid] = c;
+    SUM[2 * tid + 1] = g;
   }
   __syncthreads();
   const double c0 = SUM[0], g0 = SUM[1];

@@ -18,9 +18,9 @@ SQP iteration, "plain" is the per-launch op graph of the reference, and
 "auto" (the default) takes fused where the library builds it for the model
 and size, else plain (linear models of any size).  Solver parameters use the reference's keys plus the optional
 ``"mode"``, ``"fused_variant"`` ("auto", "register", "streamed",
-"streamed_per_knot") and
-``"blocks_per_group"`` (0 = automatic, 1, 2, 4), which diffmpc_amd.autotune
-sets.  ``problem_params["horizons"]`` (B,) makes a ragged batch.
+"streamed_per_knot"), ``"blocks_per_group"`` (0 = automatic, 1, 2, 4) and
+``"instances_per_workgroup"`` (0 or 1 = one, up to 8, register-resident
+kernels of models with nx <= 8), which diffmpc_amd.autotune sets.  ``problem_params["horizons"]`` (B,) makes a ragged batch.
 """
 import copy
 import enum
@@ -105,6 +105,7 @@ def _c_solver_params(params: Dict[str, Any]) -> "_lib.SolverParams":
     sp.blocks_per_group = int(params.get("blocks_per_group", 0))
     sp.fused_variant = {"auto": 0, "register": 1, "streamed": 2, "streamed_per_knot": 3}[
         params.get("fused_variant", "auto")]
+    sp.instances_per_workgroup = int(params.get("instances_per_workgroup", 0))
     return sp
 
 

@@ -81,6 +81,13 @@ typedef struct dmpc_solver_params { /* keys of diffmpc/solvers/params/sqp.yaml *
    * copy of the blocks per knot even for time-invariant models). */
   int32_t blocks_per_group;
   int32_t fused_variant;
+  /* Instances per workgroup of the register-resident kernels, forward and
+   * adjoint (0 or 1: one; up to 8): each instance is a team of threads
+   * aligned to 16 lanes, for models with nx <= 8 and one block per lane
+   * group.  Without the line search the results are bitwise those of one
+   * instance per workgroup.  DMPC_ERR_UNSUPPORTED where the teams do not fit
+   * one workgroup or the streamed variant runs.  No reference counterpart. */
+  int32_t instances_per_workgroup;
 } dmpc_solver_params;
 
 typedef struct dmpc_problem_data {
