@@ -27,7 +27,28 @@ __device__ __forceinline__ double lane_value(double v, int src) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// G = 64 is a whole wavefront: the four row sums are combined as scalars.
+// gfx950 row exchanges of a 64-bit value (v_permlane16_swap /
+// v_permlane32_swap with both operands the same register, per 32-bit half):
+// with rows r0..r3 of the wavefront,
+//   swap16 -> {[r0, r0, r2, r2], [r1, r1, r3, r3]}
+//   swap32 -> {[r0, r1, r0, r1], [r2, r3, r2, r3]}
+__device__ __forceinline__ void rows_swap16(double v, double& even, double& odd) {
+  const long long x = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(x >> 32), (unsigned)(x >> 32), false, false);
+  even = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  odd = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ void rows_swap32(double v, double& low, double& high) {
+  const long long x = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)x, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(x >> 32), (unsigned)(x >> 32), false, false);
+  low = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  high = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+
+// G = 64 is a whole wavefront: the four row sums are combined across rows
+// with row exchanges as (R0 + R1) + (R2 + R3), in every lane.
 template <int G>
 __device__ __forceinline__ double gsum(double v) {
   static_assert(G == 4 || G == 8 || G == 16 || G == 64, "block groups of 4, 8, 16 or 64 lanes");
@@ -35,7 +56,12 @@ __device__ __forceinline__ double gsum(double v) {
   v += dpp_mov<0x4E>(v);                        // quad_perm [2,3,0,1]
   if constexpr (G >= 8) v += dpp_mov<0x141>(v);  // row_half_mirror
   if constexpr (G >= 16) v += dpp_mov<0x140>(v); // row_mirror
-  if constexpr (G == 64) v = (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+  if constexpr (G == 64) {
+    double e, o, lo, hi;
+    rows_swap16(v, e, o);  // [R0, R0, R2, R2], [R1, R1, R3, R3]
+    rows_swap32(e + o, lo, hi);
+    v = lo + hi;
+  }
   return v;
 }
 
@@ -261,6 +287,47 @@ __device__ __forceinline__ void bcast_fmac_at(double& acc, double src, double sc
     } else {
       bcast_fmac_at<G, n + 1>(acc, src, scale, from, lane);
     }
+  }
+}
+
+// 64-lane groups (the group is the wavefront): DPP broadcasts stay inside a
+// 16-lane row, and a readlane broadcast costs two v_readlane_b32 per term.
+// Three row exchanges per 32-bit half give each row of v in every row, and
+// each term is then one v_fmac_f64_dpp row_newbcast of the right copy, in
+// the order j = 0..M-1 with one rounding per term, as gdot_chain.
+__device__ __forceinline__ void rows_to_all(double v, double (&b)[4]) {
+  double l, h;
+  rows_swap32(v, l, h);     // [r0, r1, r0, r1], [r2, r3, r2, r3]
+  rows_swap16(l, b[0], b[1]);
+  rows_swap16(h, b[2], b[3]);
+}
+// acc + sum_{k<16} s[k] * (lane k of the row in b); the leading s_nop 4
+// covers the DPP read-after-VALU-write wait states of b
+__device__ __forceinline__ double fmac_row16(double acc, double b, const double (&s)[16]) {
+#define DMPC_FMAC16(op, k) "v_fmac_f64_dpp %0, %1, %" #op " row_newbcast:" #k " row_mask:0xf bank_mask:0xf\n"
+  asm("s_nop 4\n" DMPC_FMAC16(2, 0) DMPC_FMAC16(3, 1) DMPC_FMAC16(4, 2) DMPC_FMAC16(5, 3) DMPC_FMAC16(6, 4)
+          DMPC_FMAC16(7, 5) DMPC_FMAC16(8, 6) DMPC_FMAC16(9, 7) DMPC_FMAC16(10, 8) DMPC_FMAC16(11, 9)
+              DMPC_FMAC16(12, 10) DMPC_FMAC16(13, 11) DMPC_FMAC16(14, 12) DMPC_FMAC16(15, 13) DMPC_FMAC16(16, 14)
+                  DMPC_FMAC16(17, 15)
+      : "+v"(acc)
+      : "v"(b), "v"(s[0]), "v"(s[1]), "v"(s[2]), "v"(s[3]), "v"(s[4]), "v"(s[5]), "v"(s[6]), "v"(s[7]),
+        "v"(s[8]), "v"(s[9]), "v"(s[10]), "v"(s[11]), "v"(s[12]), "v"(s[13]), "v"(s[14]), "v"(s[15]));
+#undef DMPC_FMAC16
+  return acc;
+}
+template <int M, int j = 0, class AT>
+__device__ __forceinline__ double gdot64_rows(const AT* a, const double (&b)[4], int salt, double acc) {
+  if constexpr (j == M) {
+    return acc;
+  } else if constexpr (j % 16 == 0 && j + 16 <= M) {
+    double s[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s[k] = cvt_in_loop(a[j + k], salt);
+    acc = fmac_row16(acc, b[j / 16], s);
+    return gdot64_rows<M, j + 16>(a, b, salt, acc);
+  } else {
+    dpp_fmac16<j % 16>(acc, b[j / 16], cvt_in_loop(a[j], salt));
+    return gdot64_rows<M, j + 1>(a, b, salt, acc);
   }
 }
 
@@ -806,6 +873,10 @@ __device__ __forceinline__ double gdot(const AT* a, double v, int lane, int salt
     return gdot8d_fused<M>(a, v, 0.0);
   } else if constexpr (G == 8 && M <= 8 && std::is_same<AT, float>::value) {
     return gdot8f_fused<M>(a, v, 0.0);
+  } else if constexpr (G == 64) {
+    double b[4];
+    rows_to_all(v, b);
+    return gdot64_rows<M>(a, b, salt, 0.0);
   } else {
     return gdot_chain<G, M>(a, v, lane, salt, 0.0);
   }
