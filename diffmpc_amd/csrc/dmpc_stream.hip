@@ -596,7 +596,7 @@ __device__ void stream_init_vectors(const Work& w, int Np, int N, const double* 
 // One SQP iteration of one instance (streamed): gamma, PCG, recovery, line
 // search, step test and outputs.  The block matrices were assembled by
 // stream_assemble_kernel for this linearization.
-template <class Model, int G, int NV, bool SH>
+template <class Model, int G, int NV, bool SH, bool LS>
 __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
@@ -660,7 +660,9 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   }
   __syncthreads();
   double alpha = 1.0;
-  if (a.ls_eval) {
+  // LS: the build that evaluates the merit line search (a.ls_eval); the
+  // other one has none of its float64 model evaluations and their registers
+  if constexpr (LS) {
     alpha = line_search<Model>(a, b, N, xb, ub, w.DX, w.DU, w.LS, red3, threadIdx.x, blockDim.x);
     for (int k = threadIdx.x; k < NB * NX; k += blockDim.x) XN[k] = xb[k] + alpha * w.DX[k];
     for (int k = threadIdx.x; k < NB * NU; k += blockDim.x) UN[k] = ub[k] + alpha * w.DU[k];
@@ -877,13 +879,16 @@ template <class Model, int G, bool SH>
 bool set_lds_limit(bool forward) {
   static bool attr[2] = {false, false};
   if (attr[forward]) return true;
-  const void* k1 = forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kStreamVectors, SH>)
-                           : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kStreamVectors, SH>);
-  const void* k2 = forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors, SH>)
-                           : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kHotVectors, SH>);
-  if (hipFuncSetAttribute(k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess ||
-      hipFuncSetAttribute(k2, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess)
-    return false;
+  const void* k[4] = {
+      forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kStreamVectors, SH, false>)
+              : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kStreamVectors, SH>),
+      forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors, SH, false>)
+              : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kHotVectors, SH>),
+      forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kStreamVectors, SH, true>) : nullptr,
+      forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors, SH, true>) : nullptr};
+  for (const void* f : k)
+    if (f && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess)
+      return false;
   attr[forward] = true;
   return true;
 }
@@ -906,14 +911,19 @@ int lds_vectors(int N, bool forward, bool shared) {
   return ok ? nv : 0;
 }
 
+template <class Model, int G, bool SH, bool LS>
+void launch_stream_qp_ls(const SolveArgs& a, int nv, int threads, size_t lds, hipStream_t st) {
+  if (nv == kStreamVectors)
+    hipLaunchKernelGGL((stream_qp_kernel<Model, G, kStreamVectors, SH, LS>), dim3(a.B), dim3(threads), lds, st, a);
+  else if (nv == kHotVectors)
+    hipLaunchKernelGGL((stream_qp_kernel<Model, G, kHotVectors, SH, LS>), dim3(a.B), dim3(threads), lds, st, a);
+  else
+    hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0, SH, LS>), dim3(a.B), dim3(threads), 0, st, a);
+}
 template <class Model, int G, bool SH>
 void launch_stream_qp(const SolveArgs& a, int nv, int threads, size_t lds, hipStream_t st) {
-  if (nv == kStreamVectors)
-    hipLaunchKernelGGL((stream_qp_kernel<Model, G, kStreamVectors, SH>), dim3(a.B), dim3(threads), lds, st, a);
-  else if (nv == kHotVectors)
-    hipLaunchKernelGGL((stream_qp_kernel<Model, G, kHotVectors, SH>), dim3(a.B), dim3(threads), lds, st, a);
-  else
-    hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0, SH>), dim3(a.B), dim3(threads), 0, st, a);
+  if (a.ls_eval) launch_stream_qp_ls<Model, G, SH, true>(a, nv, threads, lds, st);
+  else launch_stream_qp_ls<Model, G, SH, false>(a, nv, threads, lds, st);
 }
 
 template <class Model, int G, bool SH>
