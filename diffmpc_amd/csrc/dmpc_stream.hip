@@ -68,23 +68,24 @@ struct Work {
 // the shared copies are the values every knot would compute.  SH is a
 // template flag of the streamed kernels (instantiated both ways for
 // time-invariant models only), so the other kernels keep their plain
-// per-knot addressing.  In the SH kernels the choice is the runtime flag
-// `on` (the launch's shared_blocks), passed through an empty asm at each use:
-// with the block addresses known loop-invariant the compiler versions the
-// PCG rounds on the flag and hoists the block loads out of them, and then
-// spills them (the 48-state chain's forward launch took 2.4x as long so).
+// per-knot addressing.  The SH kernels stage the shared copies in LDS
+// (stage_shared_blocks): block 0 of [A|B] and of its transposed copy, and
+// D_0, D_1, D_N in slots 0, 1, 2.  Their block index is zero through an
+// empty asm: with the block addresses known loop-invariant the compiler
+// hoists the block loads out of the PCG rounds and spills them (the
+// 48-state chain's forward launch took 2.4x as long so).
 __device__ __forceinline__ int opaque(int v) {
   asm volatile("" : "+s"(v));
   return v;
 }
 template <bool SH>
-__device__ __forceinline__ int shared_ab(int tp, int on) {
-  if constexpr (SH) return opaque(on) ? 0 : tp;
+__device__ __forceinline__ int shared_ab(int tp) {
+  if constexpr (SH) return opaque(0);
   return tp;
 }
 template <bool SH>
-__device__ __forceinline__ int shared_d(int t, int N, int on) {
-  if constexpr (SH) return opaque(on) ? (t == 0 ? 0 : (t < N ? 1 : N)) : t;
+__device__ __forceinline__ int shared_d(int t, int N) {
+  if constexpr (SH) return t == 0 ? 0 : (t < N ? 1 : 2);
   return t;
 }
 
@@ -343,10 +344,10 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
 
 // D_t v: sum_j D_t[lane][j] v_j (v_j in lane j of the group), 0 when !ok.
 template <int G, int NX, bool SH>
-__device__ __forceinline__ double dmul_mem(const Work& w, int t, int N, int on, double v, int lane, bool ok) {
+__device__ __forceinline__ double dmul_mem(const Work& w, int t, int N, double v, int lane, bool ok) {
   const int l = lane < NX ? lane : 0;
   double acc;
-  t = shared_d<SH>(t, N, on);
+  t = shared_d<SH>(t, N);
   if constexpr (compact_blocks(NX)) {
     const double* D = w.DT + (size_t)t * tri(NX, 0);
     const int lrow = tri(l, 0);
@@ -364,9 +365,9 @@ __device__ __forceinline__ double dmul_mem(const Work& w, int t, int N, int on, 
 // next column, in whichever copy the block layout reads; B_{t-1} follows at
 // NX strides.
 template <int NX, int NU, bool SH>
-__device__ __forceinline__ const float* arow_mem(const float* ABb, const float* AT, int t, int li, int on) {
+__device__ __forceinline__ const float* arow_mem(const float* ABb, const float* AT, int t, int li) {
   constexpr int NZ = NX + NU;
-  const size_t tp = (size_t)shared_ab<SH>(t >= 1 ? t - 1 : 0, on);
+  const size_t tp = (size_t)shared_ab<SH>(t >= 1 ? t - 1 : 0);
   if constexpr (compact_blocks(NX)) return ABb + tp * NX * NZ + li * NZ;
   else return AT + tp * NZ * NX + li;
 }
@@ -392,12 +393,47 @@ struct Blocks {
   __device__ bool ok(int r) const { return r * NG + grp <= N && lane < NX; }
 };
 
+// LDS bytes the SH kernels stage the shared blocks in, after the PCG
+// vectors that live in LDS: D_0, D_1, D_N, block 0 of [A|B] and, for large
+// blocks, its transposed copy.
+template <int NX, int NU>
+__host__ __device__ constexpr size_t shared_block_bytes() {
+  return 3 * (size_t)(compact_blocks(NX) ? tri(NX, 0) : NX * NX) * 8 +
+         (size_t)NX * (NX + NU) * 4 * (compact_blocks(NX) ? 1 : 2);
+}
+
+// Copies an instance's shared blocks from the workspace into LDS at `lds`
+// (slots of shared_d / shared_ab) and points ABb, w.AT and w.DT there: every
+// PCG round then reads them from LDS instead of the cache hierarchy.  Called
+// by every thread; one barrier.
+template <int NX, int NU>
+__device__ void stage_shared_blocks(double* lds, const float*& ABb, Work& w, int N) {
+  constexpr int NZ = NX + NU;
+  constexpr int DB = compact_blocks(NX) ? tri(NX, 0) : NX * NX;
+  double* D = lds;
+  float* AB = reinterpret_cast<float*>(lds + 3 * DB);
+  float* AT = AB + NX * NZ;
+  for (int k = threadIdx.x; k < DB; k += blockDim.x) {
+    D[k] = w.DT[k];
+    D[DB + k] = w.DT[DB + k];
+    D[2 * DB + k] = w.DT[(size_t)N * DB + k];
+  }
+  for (int k = threadIdx.x; k < NX * NZ; k += blockDim.x) {
+    AB[k] = ABb[k];
+    if constexpr (!compact_blocks(NX)) AT[k] = w.AT[k];
+  }
+  __syncthreads();
+  ABb = AB;
+  w.AT = AT;
+  w.DT = D;
+}
+
 // PCG on S lam = gamma (pcg.py:350-445) with the vectors in the workspace.
 // gamma in V_GAM, the guess in V_LAM; the solution is left in V_LAM.  Same
 // phases, products and reductions as pcg() of the register variant.
 template <int G, int NX, int NU, bool SH>
 __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostView<NX, NU>& c, const float* ABb,
-                          double* red3, double eps, int max_iter, double* eta_out, int on) {
+                          double* red3, double eps, int max_iter, double* eta_out) {
   constexpr int NZ = NX + NU;
   double* LAM = w.v[V_LAM];
   double* P = w.v[V_P];
@@ -412,8 +448,8 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   const double* GAM = w.v[V_GAM];
   const int lane = K.lane, N = K.N;
   const int li = lane < NX ? lane : 0;
-  auto acol = [&](int t) { return ABb + (size_t)shared_ab<SH>(t < N ? t : 0, on) * NX * NZ + li; };  // stride NZ
-  auto arow = [&](int t) { return arow_mem<NX, NU, SH>(ABb, w.AT, t, li, on); };
+  auto acol = [&](int t) { return ABb + (size_t)shared_ab<SH>(t < N ? t : 0) * NX * NZ + li; };  // stride NZ
+  auto arow = [&](int t) { return arow_mem<NX, NU, SH>(ABb, w.AT, t, li); };
   constexpr int AS = arow_stride<NX>();
   auto has_a = [&](int t) { return t < N; };
   auto has_p = [&](int t) { return t >= 1; };
@@ -446,7 +482,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     const double g = (t == 0 ? 1.0 : -1.0) * vget<NX>(YXO, t, lane) + ((ok && has_p(t)) ? au : 0.0) +
                      vget<NX>(GB, t, lane);
     const double rv = ok ? vget<NX>(GAM, t, lane) + g : 0.0;
-    const double y = dmul_mem<G, NX, SH>(w, t, N, on, rv, lane, ok);
+    const double y = dmul_mem<G, NX, SH>(w, t, N, rv, lane, ok);
     if (ok) {
       R[(t + 1) * NX + lane] = rv;
       Y[(t + 1) * NX + lane] = y;
@@ -464,7 +500,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     const double rv = vget<NX>(R, t, lane);
     const double qi = ok ? c.qinv(t, lane) : 0.0;
     const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
-    const double rt = dmul_mem<G, NX, SH>(w, t, N, on, z, lane, ok);
+    const double rt = dmul_mem<G, NX, SH>(w, t, N, z, lane, ok);
     if (ok) {
       part += rv * rt;
       RT[(t + 1) * NX + lane] = rt;
@@ -515,7 +551,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
         LAM[(t + 1) * NX + lane] = vget<NX>(LAM, t, lane) + alpha * vget<NX>(P, t, lane);
         rv += alpha * g;
       }
-      const double y = dmul_mem<G, NX, SH>(w, t, N, on, rv, lane, ok);
+      const double y = dmul_mem<G, NX, SH>(w, t, N, rv, lane, ok);
       if (ok) {
         R[(t + 1) * NX + lane] = rv;
         Y[(t + 1) * NX + lane] = y;
@@ -534,7 +570,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
       const double rv = vget<NX>(R, t, lane);
       const double qi = ok ? c.qinv(t, lane) : 0.0;
       const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
-      const double rt = dmul_mem<G, NX, SH>(w, t, N, on, z, lane, ok);
+      const double rt = dmul_mem<G, NX, SH>(w, t, N, z, lane, ok);
       if (ok) {
         epart += rv * rt;
         RT[(t + 1) * NX + lane] = rt;
@@ -554,7 +590,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
 template <int G, int NX, int NU, bool SH>
 __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r, const Work& w,
                                                const CostView<NX, NU>& c, const float* ABb, double* xn,
-                                               double* un, int on) {
+                                               double* un) {
   constexpr int NZ = NX + NU;
   const int t = K.t(r), lane = K.lane, N = K.N;
   const bool ok = K.ok(r);
@@ -562,9 +598,9 @@ __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r
   const double* LAM = w.v[V_LAM];
   const double lt = vget<NX>(LAM, t, lane);
   const double ln = vget<NX>(LAM, t + 1, lane);
-  const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab<SH>(t < N ? t : 0, on) * NX * NZ + li, NZ, ln, lane);
+  const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab<SH>(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
   constexpr int AS = arow_stride<NX>();
-  const float* brow = arow_mem<NX, NU, SH>(ABb, w.AT, t, li, on) + NX * AS;
+  const float* brow = arow_mem<NX, NU, SH>(ABb, w.AT, t, li) + NX * AS;
   double bl = 0.0;
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
@@ -638,8 +674,9 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
     w.v[V_GAM][(t + 1) * NX + lane] = g;
   }
   __syncthreads();
+  if constexpr (SH) stage_shared_blocks<NX, NU>(stream_vlds + (size_t)NV * (Np + 3) * NX, ABb, w, N);
   double eta;
-  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta, a.shared_blocks);
+  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta);
   // recovery: the QP solution into XN / UN (free PCG vectors) and the step
   // to it into DX / DU; with a line search the new iterate is
   // xbar + alpha * step, otherwise the QP solution itself
@@ -647,7 +684,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   double* UN = w.v[V_AP];  // (N+3) nx >= (N+1) nu doubles
   for (int r = 0; r < K.rounds; ++r) {
     double xn, un;
-    stream_recover<G, NX, NU, SH>(K, r, w, c, ABb, &xn, &un, a.shared_blocks);
+    stream_recover<G, NX, NU, SH>(K, r, w, c, ABb, &xn, &un);
     if (K.ok(r)) {
       const int t = K.t(r), lane = K.lane, tu = ctl_row(t, N);
       XN[t * NX + lane] = xn;
@@ -734,8 +771,9 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
     w.v[V_GAM][(t + 1) * NX + lane] = g;
   }
   __syncthreads();
+  if constexpr (SH) stage_shared_blocks<NX, NU>(stream_vlds + (size_t)NV * (Np + 3) * NX, ABb, w, N);
   double eta;
-  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta, a.shared_blocks);
+  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta);
   // recovery of x~, u~ with q~, r~; per-row contraction terms into DX / DU
   const double* xs = a.x + (size_t)b * NBs * NX;
   const double* us = a.u + (size_t)b * NBs * NU;
@@ -746,10 +784,9 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
     const double* LAM = w.v[V_LAM];
     const double lt = vget<NX>(LAM, t, lane);
     const double ln = vget<NX>(LAM, t + 1, lane);
-    const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab<SH>(t < N ? t : 0, a.shared_blocks) * NX * NZ + li, NZ,
-                                      ln, lane);
+    const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab<SH>(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
     constexpr int AS = arow_stride<NX>();
-    const float* brow = arow_mem<NX, NU, SH>(ABb, w.AT, t, li, a.shared_blocks) + NX * AS;
+    const float* brow = arow_mem<NX, NU, SH>(ABb, w.AT, t, li) + NX * AS;
     double bl = 0.0;
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
@@ -879,18 +916,28 @@ template <class Model, int G, bool SH>
 bool set_lds_limit(bool forward) {
   static bool attr[2] = {false, false};
   if (attr[forward]) return true;
-  const void* k[4] = {
+  const void* k[6] = {
       forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kStreamVectors, SH, false>)
               : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kStreamVectors, SH>),
       forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors, SH, false>)
               : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, kHotVectors, SH>),
+      forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, 0, SH, false>)
+              : reinterpret_cast<const void*>(&stream_bwd_kernel<Model, G, 0, SH>),
       forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kStreamVectors, SH, true>) : nullptr,
-      forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors, SH, true>) : nullptr};
+      forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, kHotVectors, SH, true>) : nullptr,
+      forward ? reinterpret_cast<const void*>(&stream_qp_kernel<Model, G, 0, SH, true>) : nullptr};
   for (const void* f : k)
     if (f && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStreamLdsLimit) != hipSuccess)
       return false;
   attr[forward] = true;
   return true;
+}
+
+// Dynamic LDS of a streamed launch: its LDS vectors, and the staged shared
+// blocks with sharing.
+template <class Model>
+size_t stream_lds_bytes(int nv, int N, bool shared) {
+  return vector_lds_bytes(nv, Model::NX, N) + (shared ? shared_block_bytes<Model::NX, Model::NU>() : 0);
 }
 
 template <class Model, int G>
@@ -899,16 +946,16 @@ int lds_vectors(int N, bool forward, bool shared) {
   int nv = 0;
   // the recovery writes the (N+1) x nu controls into V_AP, which may then
   // run into the following vectors: all eleven in LDS only when nu <= nx
-  if (Model::NU <= NX && vector_lds_bytes(kStreamVectors, NX, N) <= kStreamLdsLimit) nv = kStreamVectors;
-  else if (vector_lds_bytes(kHotVectors, NX, N) <= kStreamLdsLimit) nv = kHotVectors;
-  if (!nv) return 0;
+  if (Model::NU <= NX && stream_lds_bytes<Model>(kStreamVectors, N, shared) <= kStreamLdsLimit) nv = kStreamVectors;
+  else if (stream_lds_bytes<Model>(kHotVectors, N, shared) <= kStreamLdsLimit) nv = kHotVectors;
+  if (!nv && !shared) return 0;
   bool ok;
   if constexpr (time_invariant<Model>::value) {
     ok = shared ? set_lds_limit<Model, G, true>(forward) : set_lds_limit<Model, G, false>(forward);
   } else {
     ok = set_lds_limit<Model, G, false>(forward);
   }
-  return ok ? nv : 0;
+  return ok ? nv : -1;
 }
 
 template <class Model, int G, bool SH, bool LS>
@@ -918,7 +965,7 @@ void launch_stream_qp_ls(const SolveArgs& a, int nv, int threads, size_t lds, hi
   else if (nv == kHotVectors)
     hipLaunchKernelGGL((stream_qp_kernel<Model, G, kHotVectors, SH, LS>), dim3(a.B), dim3(threads), lds, st, a);
   else
-    hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0, SH, LS>), dim3(a.B), dim3(threads), 0, st, a);
+    hipLaunchKernelGGL((stream_qp_kernel<Model, G, 0, SH, LS>), dim3(a.B), dim3(threads), lds, st, a);
 }
 template <class Model, int G, bool SH>
 void launch_stream_qp(const SolveArgs& a, int nv, int threads, size_t lds, hipStream_t st) {
@@ -933,7 +980,7 @@ void launch_stream_bwd(const BackwardArgs& a, int nv, int threads, size_t lds, h
   else if (nv == kHotVectors)
     hipLaunchKernelGGL((stream_bwd_kernel<Model, G, kHotVectors, SH>), dim3(a.B), dim3(threads), lds, st, a);
   else
-    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0, SH>), dim3(a.B), dim3(threads), 0, st, a);
+    hipLaunchKernelGGL((stream_bwd_kernel<Model, G, 0, SH>), dim3(a.B), dim3(threads), lds, st, a);
 }
 
 template <class Model, int G>
@@ -945,7 +992,8 @@ int stream_forward_t(int model, const SolveArgs& a0, hipStream_t st) {
   const AssembleArgs aa = assemble_args<Model>(a, lsd);
   const int threads = stream_threads(a.N, G, stream_max_threads<Model>());
   const int nv = lds_vectors<Model, G>(a.N, true, a.shared_blocks);
-  const size_t lds = vector_lds_bytes(nv, NX, a.N);
+  if (nv < 0) return DMPC_ERR_HIP;
+  const size_t lds = stream_lds_bytes<Model>(nv, a.N, a.shared_blocks);
   int rc = stream_launch_reset(a, st);
   if (rc) return rc;
   for (int it = 0; it < a.maxit; ++it) {
@@ -1001,7 +1049,8 @@ int stream_backward_t(const BackwardArgs& a0, hipStream_t st) {
   hipLaunchKernelGGL((stream_assemble_kernel<NX, NU>), dim3((unsigned)((long long)a.B * (a.N + 1))), dim3(64), 0, st,
                      aa);
   const int nv = lds_vectors<Model, G>(a.N, false, a.shared_blocks);
-  const size_t lds = vector_lds_bytes(nv, NX, a.N);
+  if (nv < 0) return DMPC_ERR_HIP;
+  const size_t lds = stream_lds_bytes<Model>(nv, a.N, a.shared_blocks);
   const int threads = stream_threads(a.N, G, stream_max_threads<Model>());
   if constexpr (time_invariant<Model>::value) {
     if (a.shared_blocks) launch_stream_bwd<Model, G, true>(ab, nv, threads, lds, st);
