@@ -509,18 +509,33 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   double eta = block_sum(part, red3 + 32);
   double beta = 0.0;
   int it = 0;
+  // Each phase loads the next round's vector entries before it works on
+  // the current round, so that their latency (workspace vectors of large
+  // instances come from the cache hierarchy) overlaps the products.  A round
+  // stores only entries of its own knots, which the next round does not load
+  // (a lane that would see its own knot again is masked there).
   while (true) {
     const bool cont = (fabs(eta) > eps && it <= max_iter - 1) || it < 1;
     if (!cont) break;
     // ---- phase A: p = rt + beta p; w = G^T p; y = H^-1 w; v = -w.y ------
     double vpart = 0.0;
+    double nrt = vget<NX>(RT, K.t(0), lane), np = vget<NX>(P, K.t(0), lane);
+    double nrn = vget<NX>(RT, K.t(0) + 1, lane), nap = vget<NX>(AP, K.t(0), lane);
     for (int r = 0; r < K.rounds; ++r) {
       const int t = K.t(r);
       const bool ok = K.ok(r);
-      const double pv = ok ? vget<NX>(RT, t, lane) + beta * vget<NX>(P, t, lane) : 0.0;
-      const double rn = vget<NX>(RT, t + 1, lane);
+      const double crt = nrt, cp = np, crn = nrn, cap = nap;
+      {
+        const int tn = K.t(r + 1);
+        nrt = vget<NX>(RT, tn, lane);
+        np = vget<NX>(P, tn, lane);
+        nrn = vget<NX>(RT, tn + 1, lane);
+        nap = vget<NX>(AP, tn, lane);
+      }
+      const double pv = ok ? crt + beta * cp : 0.0;
+      const double rn = crn;
       const double arn = gdot_mem<G, NX>(acol(t), NZ, rn, lane);
-      const double apv = ok ? ((has_a(t) ? arn : 0.0) + beta * vget<NX>(AP, t, lane)) : 0.0;
+      const double apv = ok ? ((has_a(t) ? arn : 0.0) + beta * cap) : 0.0;
       double q;
       const double gb = bgram_mem<G, NX, NU>(arow(t) + NX * AS, AS, c, pv, &q);
       const double qi = ok ? c.qinv(t, lane) : 0.0;
@@ -539,16 +554,27 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     const double v = -block_sum(vpart, red3);
     const double alpha = eta / v;
     // ---- phase B: ups = S p; lam += alpha p; r -= alpha ups; y = D r -----
+    double nu_ = vget<NX>(YX, K.t(0) - 1, lane), nyxo = vget<NX>(YXO, K.t(0), lane);
+    double ngb = vget<NX>(GB, K.t(0), lane), nr = vget<NX>(R, K.t(0), lane);
+    double nlam = vget<NX>(LAM, K.t(0), lane), npb = vget<NX>(P, K.t(0), lane);
     for (int r = 0; r < K.rounds; ++r) {
       const int t = K.t(r);
       const bool ok = K.ok(r);
-      const double u = vget<NX>(YX, t - 1, lane);
+      const double u = nu_, cyxo = nyxo, cgb = ngb, cr = nr, clam = nlam, cpb = npb;
+      {
+        const int tn = K.t(r + 1);
+        nu_ = vget<NX>(YX, tn - 1, lane);
+        nyxo = vget<NX>(YXO, tn, lane);
+        ngb = vget<NX>(GB, tn, lane);
+        nr = vget<NX>(R, tn, lane);
+        nlam = vget<NX>(LAM, tn, lane);
+        npb = vget<NX>(P, tn, lane);
+      }
       const double au = gdot_mem<G, NX>(arow(t), AS, u, lane);
-      const double g = (t == 0 ? 1.0 : -1.0) * vget<NX>(YXO, t, lane) + ((ok && has_p(t)) ? au : 0.0) +
-                       vget<NX>(GB, t, lane);
-      double rv = vget<NX>(R, t, lane);
+      const double g = (t == 0 ? 1.0 : -1.0) * cyxo + ((ok && has_p(t)) ? au : 0.0) + cgb;
+      double rv = cr;
       if (ok) {
-        LAM[(t + 1) * NX + lane] = vget<NX>(LAM, t, lane) + alpha * vget<NX>(P, t, lane);
+        LAM[(t + 1) * NX + lane] = clam + alpha * cpb;
         rv += alpha * g;
       }
       const double y = dmul_mem<G, NX, SH>(w, t, N, rv, lane, ok);
@@ -561,13 +587,21 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     __syncthreads();
     // ---- phase C: rt = Phi^-1 r; eta' = r.rt --------------------------------
     double epart = 0.0;
+    double nyq = vget<NX>(YQ, K.t(0) - 1, lane), ny = vget<NX>(Y, K.t(0) + 1, lane), nrc = vget<NX>(R, K.t(0), lane);
     for (int r = 0; r < K.rounds; ++r) {
       const int t = K.t(r);
       const bool ok = K.ok(r);
-      const double a1v = gdot_mem<G, NX>(arow(t), AS, vget<NX>(YQ, t - 1, lane), lane);
-      const double a2v = gdot_mem<G, NX>(acol(t), NZ, vget<NX>(Y, t + 1, lane), lane);
+      const double cyq = nyq, cy = ny, crc = nrc;
+      {
+        const int tn = K.t(r + 1);
+        nyq = vget<NX>(YQ, tn - 1, lane);
+        ny = vget<NX>(Y, tn + 1, lane);
+        nrc = vget<NX>(R, tn, lane);
+      }
+      const double a1v = gdot_mem<G, NX>(arow(t), AS, cyq, lane);
+      const double a2v = gdot_mem<G, NX>(acol(t), NZ, cy, lane);
       const double a1 = (ok && has_p(t)) ? a1v : 0.0, a2 = (ok && has_a(t)) ? a2v : 0.0;
-      const double rv = vget<NX>(R, t, lane);
+      const double rv = crc;
       const double qi = ok ? c.qinv(t, lane) : 0.0;
       const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
       const double rt = dmul_mem<G, NX, SH>(w, t, N, z, lane, ok);
