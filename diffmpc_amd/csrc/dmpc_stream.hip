@@ -150,6 +150,12 @@ template <int NX, int NU>
 struct CostView {
   const double *wx, *wu, *wf, *xref, *uref, *xf;  // instance rows / bases
   int penalize, N;                                 // N: the instance's horizon
+  // the reciprocals a PCG round needs, computed once per instance
+  // (cache_reciprocals; the same expressions, so the same bits): Qinv of
+  // the lane's row before and at the terminal knot, Rinv in LDS
+  double qi_run = 0.0, qi_end = 0.0;
+  const double* rtab = nullptr;
+  __device__ double qinv_lane(int t) const { return t < N ? qi_run : qi_end; }
   __device__ double qinv(int t, int i) const {
     const double w = t < N ? wx[i] : wx[i] + wf[i];
     return 1.0 / fmax(2.0 * w, kPsdMinEig);
@@ -162,6 +168,17 @@ struct CostView {
     return (t < N && penalize) ? -2.0 * wu[k] * uref[t * NU + k] : 0.0;
   }
 };
+
+// Fills c's cached reciprocals; rtab must be read only after a barrier.
+template <int NX, int NU>
+__device__ __forceinline__ void cache_reciprocals(CostView<NX, NU>& c, double* rtab, int lane) {
+  if (threadIdx.x < NU) rtab[threadIdx.x] = c.rinv(threadIdx.x);
+  c.rtab = rtab;
+  if (lane < NX) {
+    c.qi_run = c.qinv(0, lane);  // t = 0 < N
+    c.qi_end = c.qinv(c.N, lane);
+  }
+}
 
 // sum_j a[j * stride] * v_j (a in global memory), v_j in lane j of the group.
 template <int G, int M, class AT>
@@ -183,7 +200,7 @@ __device__ __forceinline__ double bgram_mem(const float* brow, int stride, const
   for (int k = 0; k < NU; ++k) {
     const double bk = brow[k * stride];
     const double sk = gsum<G>(bk * v);
-    const double yk = c.rinv(k) * sk;
+    const double yk = c.rtab[k] * sk;
     g += bk * yk;
     q += sk * yk;
   }
@@ -345,7 +362,7 @@ __global__ void __launch_bounds__(64) stream_assemble_kernel(AssembleArgs a) {
 // D_t v: sum_j D_t[lane][j] v_j (v_j in lane j of the group), 0 when !ok.
 template <int G, int NX, bool SH>
 __device__ __forceinline__ double dmul_mem(const Work& w, int t, int N, double v, int lane, bool ok) {
-  const int l = lane < NX ? lane : 0;
+  const int l = lane < NX ? lane : NX - 1;  // idle lanes read row NX-1 with lane NX-1: an LDS broadcast, not a bank conflict
   double acc;
   t = shared_d<SH>(t, N);
   if constexpr (compact_blocks(NX)) {
@@ -447,7 +464,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   double* GB = w.v[V_GB];
   const double* GAM = w.v[V_GAM];
   const int lane = K.lane, N = K.N;
-  const int li = lane < NX ? lane : 0;
+  const int li = lane < NX ? lane : NX - 1;  // (see dmul_mem)
   auto acol = [&](int t) { return ABb + (size_t)shared_ab<SH>(t < N ? t : 0) * NX * NZ + li; };  // stride NZ
   auto arow = [&](int t) { return arow_mem<NX, NU, SH>(ABb, w.AT, t, li); };
   constexpr int AS = arow_stride<NX>();
@@ -457,7 +474,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   for (int r = 0; r < K.rounds; ++r) {
     const int t = K.t(r);
     const bool ok = K.ok(r);
-    const double qi = ok ? c.qinv(t, lane) : 0.0;
+    const double qi = ok ? c.qinv_lane(t) : 0.0;
     const double pv = vget<NX>(LAM, t, lane);
     const double pn = vget<NX>(LAM, t + 1, lane);
     const double ax = gdot_mem<G, NX>(acol(t), NZ, pn, lane);
@@ -486,7 +503,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     if (ok) {
       R[(t + 1) * NX + lane] = rv;
       Y[(t + 1) * NX + lane] = y;
-      YQ[(t + 1) * NX + lane] = c.qinv(t, lane) * y;
+      YQ[(t + 1) * NX + lane] = c.qinv_lane(t) * y;
     }
   }
   __syncthreads();
@@ -498,7 +515,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
     const double a2v = gdot_mem<G, NX>(acol(t), NZ, vget<NX>(Y, t + 1, lane), lane);
     const double a1 = (ok && has_p(t)) ? a1v : 0.0, a2 = (ok && has_a(t)) ? a2v : 0.0;
     const double rv = vget<NX>(R, t, lane);
-    const double qi = ok ? c.qinv(t, lane) : 0.0;
+    const double qi = ok ? c.qinv_lane(t) : 0.0;
     const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
     const double rt = dmul_mem<G, NX, SH>(w, t, N, z, lane, ok);
     if (ok) {
@@ -538,7 +555,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
       const double apv = ok ? ((has_a(t) ? arn : 0.0) + beta * cap) : 0.0;
       double q;
       const double gb = bgram_mem<G, NX, NU>(arow(t) + NX * AS, AS, c, pv, &q);
-      const double qi = ok ? c.qinv(t, lane) : 0.0;
+      const double qi = ok ? c.qinv_lane(t) : 0.0;
       const double wx = ((t == 0) ? pv : -pv) + apv;
       const double yxo = qi * wx;
       if (ok) {
@@ -581,7 +598,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
       if (ok) {
         R[(t + 1) * NX + lane] = rv;
         Y[(t + 1) * NX + lane] = y;
-        YQ[(t + 1) * NX + lane] = c.qinv(t, lane) * y;
+        YQ[(t + 1) * NX + lane] = c.qinv_lane(t) * y;
       }
     }
     __syncthreads();
@@ -602,7 +619,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
       const double a2v = gdot_mem<G, NX>(acol(t), NZ, cy, lane);
       const double a1 = (ok && has_p(t)) ? a1v : 0.0, a2 = (ok && has_a(t)) ? a2v : 0.0;
       const double rv = crc;
-      const double qi = ok ? c.qinv(t, lane) : 0.0;
+      const double qi = ok ? c.qinv_lane(t) : 0.0;
       const double z = ok ? rv - (t == 1 ? 1.0 : -1.0) * a1 - (t == 0 ? 1.0 : -1.0) * qi * a2 : 0.0;
       const double rt = dmul_mem<G, NX, SH>(w, t, N, z, lane, ok);
       if (ok) {
@@ -628,7 +645,7 @@ __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r
   constexpr int NZ = NX + NU;
   const int t = K.t(r), lane = K.lane, N = K.N;
   const bool ok = K.ok(r);
-  const int li = lane < NX ? lane : 0;
+  const int li = lane < NX ? lane : NX - 1;  // (see dmul_mem)
   const double* LAM = w.v[V_LAM];
   const double lt = vget<NX>(LAM, t, lane);
   const double ln = vget<NX>(LAM, t + 1, lane);
@@ -685,6 +702,8 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   CostView<NX, NU> c{a.wx + (size_t)b * NX, a.wu + (size_t)b * NU, a.wf + (size_t)b * NX,
                      a.xref + (size_t)b * NBs * NX, a.uref + (size_t)b * NBs * NU, a.xf + (size_t)b * NX,
                      a.penalize, N};
+  __shared__ double rinv_lds[NU];
+  cache_reciprocals(c, rinv_lds, K.lane);
   const double* xb = a.x + (size_t)b * NBs * NX;
   const double* ub = a.u + (size_t)b * NBs * NU;
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
@@ -784,6 +803,8 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
   CostView<NX, NU> c{a.wx + (size_t)b * NX, a.wu + (size_t)b * NU, a.wf + (size_t)b * NX,
                      a.xref + (size_t)b * NBs * NX, a.uref + (size_t)b * NBs * NU, a.xf + (size_t)b * NX,
                      a.penalize, N};
+  __shared__ double rinv_lds[NU];
+  cache_reciprocals(c, rinv_lds, K.lane);  // visible after the barrier that follows gamma
   const double* dlx = a.dl_dx + (size_t)b * NBs * NX;
   const double* dlu = a.dl_du + (size_t)b * NBs * NU;
   const float* ABb = a.AB + (size_t)b * Np * NX * NZ;
@@ -814,7 +835,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
   for (int r = 0; r < K.rounds; ++r) {
     const int t = K.t(r), lane = K.lane;
     const bool ok = K.ok(r);
-    const int li = lane < NX ? lane : 0;
+    const int li = lane < NX ? lane : NX - 1;  // (see dmul_mem)
     const double* LAM = w.v[V_LAM];
     const double lt = vget<NX>(LAM, t, lane);
     const double ln = vget<NX>(LAM, t + 1, lane);
