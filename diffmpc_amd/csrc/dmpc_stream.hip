@@ -192,8 +192,8 @@ __device__ __forceinline__ double gdot_mem(const AT* a, int stride, double v, in
 // Group sum of b_k * v for the NU columns k of B_{t-1} (brow: the lane's
 // entry of column 0, the next at stride `stride`), and the B_{t-1} Rinv
 // B_{t-1}^T v product (see bgram in the register variant).
-template <int G, int NX, int NU>
-__device__ __forceinline__ double bgram_mem(const float* brow, int stride, const CostView<NX, NU>& c, double v,
+template <int G, int NX, int NU, class T>
+__device__ __forceinline__ double bgram_mem(const T* brow, int stride, const CostView<NX, NU>& c, double v,
                                             double* quad) {
   double g = 0.0, q = 0.0;
 #pragma unroll
@@ -381,8 +381,8 @@ __device__ __forceinline__ double dmul_mem(const Work& w, int t, int N, double v
 // The lane's entry of column 0 of A_{t-1} (row `li`) and the stride to the
 // next column, in whichever copy the block layout reads; B_{t-1} follows at
 // NX strides.
-template <int NX, int NU, bool SH>
-__device__ __forceinline__ const float* arow_mem(const float* ABb, const float* AT, int t, int li) {
+template <int NX, int NU, bool SH, class T>
+__device__ __forceinline__ const T* arow_mem(const T* ABb, const T* AT, int t, int li) {
   constexpr int NZ = NX + NU;
   const size_t tp = (size_t)shared_ab<SH>(t >= 1 ? t - 1 : 0);
   if constexpr (compact_blocks(NX)) return ABb + tp * NX * NZ + li * NZ;
@@ -411,46 +411,53 @@ struct Blocks {
 };
 
 // LDS bytes the SH kernels stage the shared blocks in, after the PCG
-// vectors that live in LDS: D_0, D_1, D_N, block 0 of [A|B] and, for large
-// blocks, its transposed copy.
+// vectors that live in LDS: D_0, D_1, D_N, and block 0 of [A|B] and, for
+// large blocks, its transposed copy, converted to float64 (exact, so the
+// products keep their bits and need no conversion per term).
 template <int NX, int NU>
 __host__ __device__ constexpr size_t shared_block_bytes() {
   return 3 * (size_t)(compact_blocks(NX) ? tri(NX, 0) : NX * NX) * 8 +
-         (size_t)NX * (NX + NU) * 4 * (compact_blocks(NX) ? 1 : 2);
+         (size_t)NX * (NX + NU) * 8 * (compact_blocks(NX) ? 1 : 2);
 }
 
 // Copies an instance's shared blocks from the workspace into LDS at `lds`
-// (slots of shared_d / shared_ab) and points ABb, w.AT and w.DT there: every
-// PCG round then reads them from LDS instead of the cache hierarchy.  Called
-// by every thread; one barrier.
+// (slots of shared_d / shared_ab; [A|B] and its transpose as float64) and
+// points ABd, ATd and w.DT there: every PCG round then reads them from LDS
+// instead of the cache hierarchy.  Called by every thread; one barrier.
 template <int NX, int NU>
-__device__ void stage_shared_blocks(double* lds, const float*& ABb, Work& w, int N) {
+__device__ void stage_shared_blocks(double* lds, const float* ABb, Work& w, int N, const double*& ABd,
+                                    const double*& ATd) {
   constexpr int NZ = NX + NU;
   constexpr int DB = compact_blocks(NX) ? tri(NX, 0) : NX * NX;
   double* D = lds;
-  float* AB = reinterpret_cast<float*>(lds + 3 * DB);
-  float* AT = AB + NX * NZ;
+  double* AB = lds + 3 * DB;
+  double* AT = AB + NX * NZ;
   for (int k = threadIdx.x; k < DB; k += blockDim.x) {
     D[k] = w.DT[k];
     D[DB + k] = w.DT[DB + k];
     D[2 * DB + k] = w.DT[(size_t)N * DB + k];
   }
   for (int k = threadIdx.x; k < NX * NZ; k += blockDim.x) {
-    AB[k] = ABb[k];
-    if constexpr (!compact_blocks(NX)) AT[k] = w.AT[k];
+    AB[k] = (double)ABb[k];
+    if constexpr (!compact_blocks(NX)) AT[k] = (double)w.AT[k];
   }
   __syncthreads();
-  ABb = AB;
-  w.AT = AT;
+  ABd = AB;
+  ATd = AT;
   w.DT = D;
 }
+
+// Element type of the blocks a streamed PCG reads: the float32 linearization
+// in the workspace, or the float64 copies the SH kernels stage in LDS.
+template <bool SH>
+using BlockT = typename std::conditional<SH, double, float>::type;
 
 // PCG on S lam = gamma (pcg.py:350-445) with the vectors in the workspace.
 // gamma in V_GAM, the guess in V_LAM; the solution is left in V_LAM.  Same
 // phases, products and reductions as pcg() of the register variant.
-template <int G, int NX, int NU, bool SH>
-__device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostView<NX, NU>& c, const float* ABb,
-                          double* red3, double eps, int max_iter, double* eta_out) {
+template <int G, int NX, int NU, bool SH, class T>
+__device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostView<NX, NU>& c, const T* ABb,
+                          const T* ATb, double* red3, double eps, int max_iter, double* eta_out) {
   constexpr int NZ = NX + NU;
   double* LAM = w.v[V_LAM];
   double* P = w.v[V_P];
@@ -466,7 +473,7 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
   const int lane = K.lane, N = K.N;
   const int li = lane < NX ? lane : NX - 1;  // (see dmul_mem)
   auto acol = [&](int t) { return ABb + (size_t)shared_ab<SH>(t < N ? t : 0) * NX * NZ + li; };  // stride NZ
-  auto arow = [&](int t) { return arow_mem<NX, NU, SH>(ABb, w.AT, t, li); };
+  auto arow = [&](int t) { return arow_mem<NX, NU, SH>(ABb, ATb, t, li); };
   constexpr int AS = arow_stride<NX>();
   auto has_a = [&](int t) { return t < N; };
   auto has_p = [&](int t) { return t >= 1; };
@@ -638,9 +645,9 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
 }
 
 // x_t and u_{t-1} of block t (pcg.py:316-348) from V_LAM, see recover_rows.
-template <int G, int NX, int NU, bool SH>
+template <int G, int NX, int NU, bool SH, class T>
 __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r, const Work& w,
-                                               const CostView<NX, NU>& c, const float* ABb, double* xn,
+                                               const CostView<NX, NU>& c, const T* ABb, const T* ATb, double* xn,
                                                double* un) {
   constexpr int NZ = NX + NU;
   const int t = K.t(r), lane = K.lane, N = K.N;
@@ -651,7 +658,7 @@ __device__ __forceinline__ void stream_recover(const Blocks<G, NX, NU>& K, int r
   const double ln = vget<NX>(LAM, t + 1, lane);
   const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab<SH>(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
   constexpr int AS = arow_stride<NX>();
-  const float* brow = arow_mem<NX, NU, SH>(ABb, w.AT, t, li) + NX * AS;
+  const T* brow = arow_mem<NX, NU, SH>(ABb, ATb, t, li) + NX * AS;
   double bl = 0.0;
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
@@ -727,9 +734,16 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
     w.v[V_GAM][(t + 1) * NX + lane] = g;
   }
   __syncthreads();
-  if constexpr (SH) stage_shared_blocks<NX, NU>(stream_vlds + (size_t)NV * (Np + 3) * NX, ABb, w, N);
+  const BlockT<SH>* ABk;
+  const BlockT<SH>* ATk;
+  if constexpr (SH) {
+    stage_shared_blocks<NX, NU>(stream_vlds + (size_t)NV * (Np + 3) * NX, ABb, w, N, ABk, ATk);
+  } else {
+    ABk = ABb;
+    ATk = w.AT;
+  }
   double eta;
-  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta);
+  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABk, ATk, red3, a.pcg_eps, a.pcg_max, &eta);
   // recovery: the QP solution into XN / UN (free PCG vectors) and the step
   // to it into DX / DU; with a line search the new iterate is
   // xbar + alpha * step, otherwise the QP solution itself
@@ -737,7 +751,7 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
   double* UN = w.v[V_AP];  // (N+3) nx >= (N+1) nu doubles
   for (int r = 0; r < K.rounds; ++r) {
     double xn, un;
-    stream_recover<G, NX, NU, SH>(K, r, w, c, ABb, &xn, &un);
+    stream_recover<G, NX, NU, SH>(K, r, w, c, ABk, ATk, &xn, &un);
     if (K.ok(r)) {
       const int t = K.t(r), lane = K.lane, tu = ctl_row(t, N);
       XN[t * NX + lane] = xn;
@@ -826,9 +840,16 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
     w.v[V_GAM][(t + 1) * NX + lane] = g;
   }
   __syncthreads();
-  if constexpr (SH) stage_shared_blocks<NX, NU>(stream_vlds + (size_t)NV * (Np + 3) * NX, ABb, w, N);
+  const BlockT<SH>* ABk;
+  const BlockT<SH>* ATk;
+  if constexpr (SH) {
+    stage_shared_blocks<NX, NU>(stream_vlds + (size_t)NV * (Np + 3) * NX, ABb, w, N, ABk, ATk);
+  } else {
+    ABk = ABb;
+    ATk = w.AT;
+  }
   double eta;
-  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABb, red3, a.pcg_eps, a.pcg_max, &eta);
+  const int npcg = stream_pcg<G, NX, NU, SH>(K, w, c, ABk, ATk, red3, a.pcg_eps, a.pcg_max, &eta);
   // recovery of x~, u~ with q~, r~; per-row contraction terms into DX / DU
   const double* xs = a.x + (size_t)b * NBs * NX;
   const double* us = a.u + (size_t)b * NBs * NU;
@@ -839,9 +860,9 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel
     const double* LAM = w.v[V_LAM];
     const double lt = vget<NX>(LAM, t, lane);
     const double ln = vget<NX>(LAM, t + 1, lane);
-    const double ax = gdot_mem<G, NX>(ABb + (size_t)shared_ab<SH>(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
+    const double ax = gdot_mem<G, NX>(ABk + (size_t)shared_ab<SH>(t < N ? t : 0) * NX * NZ + li, NZ, ln, lane);
     constexpr int AS = arow_stride<NX>();
-    const float* brow = arow_mem<NX, NU, SH>(ABb, w.AT, t, li) + NX * AS;
+    const BlockT<SH>* brow = arow_mem<NX, NU, SH>(ABk, ATk, t, li) + NX * AS;
     double bl = 0.0;
 #pragma unroll
     for (int k = 0; k < NU; ++k) {
