@@ -192,9 +192,50 @@ __device__ __forceinline__ double gdot_mem(const AT* a, int stride, double v, in
 // Group sum of b_k * v for the NU columns k of B_{t-1} (brow: the lane's
 // entry of column 0, the next at stride `stride`), and the B_{t-1} Rinv
 // B_{t-1}^T v product (see bgram in the register variant).
+// 64-lane groups: the NU <= 16 column sums of B_{t-1}^T v by one
+// reduce-scatter over each 16-lane row instead of NU wavefront sums.  Slot s
+// of lane l holds column l ^ s (zero past NU); the exchanges with lanes
+// l ^ 15, l ^ 7, l ^ 3 and l ^ 1 (row_mirror, row_half_mirror, quad_perm
+// [3,2,1,0] and [1,0,3,2]) keep the slots whose column shares the partner
+// bit with l, so slot 0 ends with column l's sum over the row; two row
+// exchanges add the four rows.  Then y = Rinv s (lane k of every row holds
+// y_k) and g = B_{t-1} y with one DPP FMA per column.  The quadratic form
+// v^T B Rinv B^T v is left to the caller as the row sum of v g.
+template <int NX, int NU, class T>
+__device__ __forceinline__ double bgram64(const T* brow, int stride, const CostView<NX, NU>& c, double v) {
+  static_assert(NU <= 16, "one 16-lane row holds the column sums");
+  const int l = threadIdx.x & 15;
+  double P[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int col = l ^ s;
+    const double b = (double)brow[(col < NU ? col : 0) * stride];
+    P[s] = col < NU ? b * v : 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) P[s] += dpp_mov<kDppMirror>(P[15 - s]);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) P[s] += dpp_mov<kDppHalfMirror>(P[7 - s]);
+#pragma unroll
+  for (int s = 0; s < 2; ++s) P[s] += dpp_mov<0x1B>(P[3 - s]);  // quad_perm [3,2,1,0]
+  P[0] += dpp_mov<0xB1>(P[1]);                                     // quad_perm [1,0,3,2]
+  double e, o, lo, hi;
+  rows_swap16(P[0], e, o);  // rows (R0, R1) and (R2, R3)
+  rows_swap32(e + o, lo, hi);
+  const double y = (l < NU ? c.rtab[l < NU ? l : 0] : 0.0) * (lo + hi);
+  double g = 0.0;
+#pragma unroll
+  for (int k = 0; k < NU; ++k) bcast_fmac_at<16>(g, y, (double)brow[k * stride], k, 0);  // lane k of the row
+  return g;
+}
+
 template <int G, int NX, int NU, class T>
 __device__ __forceinline__ double bgram_mem(const T* brow, int stride, const CostView<NX, NU>& c, double v,
                                             double* quad) {
+  if constexpr (G == 64) {
+    *quad = 0.0;  // the caller adds v g instead
+    return bgram64<NX, NU>(brow, stride, c, v);
+  }
   double g = 0.0, q = 0.0;
 #pragma unroll
   for (int k = 0; k < NU; ++k) {
@@ -572,7 +613,11 @@ __device__ int stream_pcg(const Blocks<G, NX, NU>& K, const Work& w, const CostV
         YXO[(t + 1) * NX + lane] = yxo;
         GB[(t + 1) * NX + lane] = has_p(t) ? gb : 0.0;
         vpart += wx * yxo;
-        if (lane == 0 && has_p(t)) vpart += q;
+        if constexpr (G == 64) {
+          if (has_p(t)) vpart += pv * gb;  // v^T B Rinv B^T v, summed over the rows
+        } else {
+          if (lane == 0 && has_p(t)) vpart += q;
+        }
       }
     }
     const double v = -block_sum(vpart, red3);
