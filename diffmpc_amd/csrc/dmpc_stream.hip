@@ -56,6 +56,7 @@ struct Work {
   double* v[kStreamVectors];
   double *DT, *ID, *DX, *DU, *LS;
   float* AT;
+  const double* DG;  // the workspace's D blocks, when LDS holds D_1 only (d1_only)
 };
 
 // Block a knot's data is read from.  A time-invariant model (linear in x and
@@ -70,7 +71,8 @@ struct Work {
 // time-invariant models only), so the other kernels keep their plain
 // per-knot addressing.  The SH kernels stage the shared copies in LDS
 // (stage_shared_blocks): block 0 of [A|B] and of its transposed copy, and
-// D_0, D_1, D_N in slots 0, 1, 2.  Their block index is zero through an
+// D_0, D_1, D_N in slots 0, 1, 2 -- for 64-lane groups D_1 only (d1_only).
+// Their block index is zero through an
 // empty asm: with the block addresses known loop-invariant the compiler
 // hoists the block loads out of the PCG rounds and spills them (the
 // 48-state chain's forward launch took 2.4x as long so).
@@ -129,6 +131,19 @@ __device__ __forceinline__ double vget(const double* V, int tb, int lane) {
 // the latency of each round's block loads.
 template <class Model>
 __host__ __device__ constexpr int stream_max_threads() { return Model::NX <= 16 ? 1024 : 512; }
+
+// 64-lane groups (more than 32 states) hold one knot per wavefront.  Their
+// shared-block kernels stage only D_1 of the three D blocks in LDS (with the
+// [A|B] copies 64.5 KB for 48 states instead of 101.4 KB) and read D_0 and
+// D_N, which two knots of 257 use, from the workspace: the knot, and so the
+// choice, is uniform over the wavefront.  Two 512-thread workgroups then fit
+// in a CU's LDS, and the kernels without the line search are compiled for
+// the 128 registers per lane that four waves per SIMD leave (the 48-state
+// chain's forward launch: 0.669 -> 0.548 s; the line-search build spills at
+// any budget and keeps the compiler's choice).
+__host__ __device__ constexpr bool d1_only(int nx) { return nx > 32; }
+template <class Model, bool SH, bool LS>
+__host__ __device__ constexpr int stream_waves_per_simd() { return SH && !LS && d1_only(Model::NX) ? 4 : 1; }
 
 // The first NV padded vectors (V_LAM, V_P, V_R, V_RT, V_Y, V_YQ, V_YX, then
 // V_AP, V_YXO, V_GB, V_GAM) in LDS instead of the workspace: every PCG phase
@@ -406,7 +421,11 @@ __device__ __forceinline__ double dmul_mem(const Work& w, int t, int N, double v
   const int l = lane < NX ? lane : NX - 1;  // idle lanes read row NX-1 with lane NX-1: an LDS broadcast, not a bank conflict
   double acc;
   t = shared_d<SH>(t, N);
-  if constexpr (compact_blocks(NX)) {
+  if constexpr (SH && d1_only(NX)) {
+    const int slot = __builtin_amdgcn_readfirstlane(t);
+    if (slot == 1) acc = gdot_mem<G, NX>(w.DT + l, NX, v, lane);
+    else acc = gdot_mem<G, NX>(w.DG + (size_t)(slot == 0 ? 0 : N) * NX * NX + l, NX, v, lane);
+  } else if constexpr (compact_blocks(NX)) {
     const double* D = w.DT + (size_t)t * tri(NX, 0);
     const int lrow = tri(l, 0);
     double r[NX];
@@ -452,32 +471,39 @@ struct Blocks {
 };
 
 // LDS bytes the SH kernels stage the shared blocks in, after the PCG
-// vectors that live in LDS: D_0, D_1, D_N, and block 0 of [A|B] and, for
+// vectors that live in LDS: D_0, D_1, D_N (D_1 only, d1_only), and block 0
+// of [A|B] and, for
 // large blocks, its transposed copy, converted to float64 (exact, so the
 // products keep their bits and need no conversion per term).
 template <int NX, int NU>
 __host__ __device__ constexpr size_t shared_block_bytes() {
-  return 3 * (size_t)(compact_blocks(NX) ? tri(NX, 0) : NX * NX) * 8 +
+  return (d1_only(NX) ? 1 : 3) * (size_t)(compact_blocks(NX) ? tri(NX, 0) : NX * NX) * 8 +
          (size_t)NX * (NX + NU) * 8 * (compact_blocks(NX) ? 1 : 2);
 }
 
 // Copies an instance's shared blocks from the workspace into LDS at `lds`
 // (slots of shared_d / shared_ab; [A|B] and its transpose as float64) and
-// points ABd, ATd and w.DT there: every PCG round then reads them from LDS
-// instead of the cache hierarchy.  Called by every thread; one barrier.
+// points ABd, ATd and w.DT there (w.DG at the workspace's D blocks): every
+// PCG round then reads them from LDS instead of the cache hierarchy.  Called
+// by every thread; one barrier.
 template <int NX, int NU>
 __device__ void stage_shared_blocks(double* lds, const float* ABb, Work& w, int N, const double*& ABd,
                                     const double*& ATd) {
   constexpr int NZ = NX + NU;
   constexpr int DB = compact_blocks(NX) ? tri(NX, 0) : NX * NX;
   double* D = lds;
-  double* AB = lds + 3 * DB;
+  double* AB = lds + (d1_only(NX) ? 1 : 3) * DB;
   double* AT = AB + NX * NZ;
   for (int k = threadIdx.x; k < DB; k += blockDim.x) {
-    D[k] = w.DT[k];
-    D[DB + k] = w.DT[DB + k];
-    D[2 * DB + k] = w.DT[(size_t)N * DB + k];
+    if constexpr (d1_only(NX)) {
+      D[k] = w.DT[DB + k];
+    } else {
+      D[k] = w.DT[k];
+      D[DB + k] = w.DT[DB + k];
+      D[2 * DB + k] = w.DT[(size_t)N * DB + k];
+    }
   }
+  w.DG = w.DT;
   for (int k = threadIdx.x; k < NX * NZ; k += blockDim.x) {
     AB[k] = (double)ABb[k];
     if constexpr (!compact_blocks(NX)) AT[k] = (double)w.AT[k];
@@ -736,7 +762,8 @@ __device__ void stream_init_vectors(const Work& w, int Np, int N, const double* 
 // search, step test and outputs.  The block matrices were assembled by
 // stream_assemble_kernel for this linearization.
 template <class Model, int G, int NV, bool SH, bool LS>
-__global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(SolveArgs a) {
+__global__ void __launch_bounds__(stream_max_threads<Model>())
+__attribute__((amdgpu_waves_per_eu(stream_waves_per_simd<Model, SH, LS>()))) stream_qp_kernel(SolveArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
   extern __shared__ double stream_vlds[];
@@ -845,7 +872,8 @@ __global__ void __launch_bounds__(stream_max_threads<Model>()) stream_qp_kernel(
 // with q = -dl/dx, r = -dl/du[:-1], C = 0 on the saved linearization, then
 // the closed-form contraction of the register variant's backward.
 template <class Model, int G, int NV, bool SH>
-__global__ void __launch_bounds__(stream_max_threads<Model>()) stream_bwd_kernel(BackwardArgs a) {
+__global__ void __launch_bounds__(stream_max_threads<Model>())
+__attribute__((amdgpu_waves_per_eu(stream_waves_per_simd<Model, SH, false>()))) stream_bwd_kernel(BackwardArgs a) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
   __shared__ double red3[48];
   extern __shared__ double stream_vlds[];
