@@ -27,10 +27,10 @@ MI355X_FP64_PEAK_TFLOPS = 78.6
 QP_TRAFFIC_BYTES = 90.8e6
 QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
 # HBM bytes per forward SQP iteration of the chain workload (stream_assemble_kernel
-# + stream_qp_kernel, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0): 0.913 TB read + 0.757 TB
+# + stream_qp_kernel, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0): 0.983 TB read + 0.757 TB
 # written (the linear model's blocks are one shared copy per instance, staged
-# in LDS; the traffic is the PCG vectors in the workspace)
-CHAIN_TRAFFIC_BYTES = 1.670e12
+# in LDS but for D_0 and D_N; the traffic is the PCG vectors in the workspace)
+CHAIN_TRAFFIC_BYTES = 1.740e12
 CHAIN_TRAFFIC_SOURCE = "profiles/chain_nx48_B512_N256.md"
 # HBM bytes per forward launch (stream_assemble_kernel + stream_qp_kernel, one SQP
 # iteration, mean over the three) of the ragged mix's dominant group, the streamed
