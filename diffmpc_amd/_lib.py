@@ -53,6 +53,7 @@ class SolverParams(ctypes.Structure):
         ("pcg_max_iter", ctypes.c_int32), ("pcg_tol_epsilon", ctypes.c_double),
         ("warm_start_backward", ctypes.c_int32), ("blocks_per_group", ctypes.c_int32),
         ("fused_variant", ctypes.c_int32), ("instances_per_workgroup", ctypes.c_int32),
+        ("forward_schedule", ctypes.c_int32),
     ]
 
 

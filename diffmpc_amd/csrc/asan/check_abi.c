@@ -129,6 +129,9 @@ int main(void) {
     EXPECT(dmpc_sqp_solve(&p, &t, &data, &sol, &saved, DMPC_MODE_FUSED, NULL), DMPC_ERR_ARG);
     t.instances_per_workgroup = -1;
     EXPECT(dmpc_sqp_solve(&p, &t, &data, &sol, &saved, DMPC_MODE_FUSED, NULL), DMPC_ERR_ARG);
+    t = s;
+    t.forward_schedule = 2;
+    EXPECT(dmpc_sqp_solve(&p, &t, &data, &sol, &saved, DMPC_MODE_FUSED, NULL), DMPC_ERR_ARG);
   }
   {
     dmpc_problem_data e = data;

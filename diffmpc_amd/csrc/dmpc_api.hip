@@ -105,7 +105,8 @@ int check_problem(const dmpc_problem* p) {
 
 bool tuning_ok(const dmpc_solver_params* s) {
   const int k = s->blocks_per_group, v = s->fused_variant, ipw = s->instances_per_workgroup;
-  return (k == 0 || k == 1 || k == 2 || k == 4) && v >= 0 && v <= 3 && ipw >= 0 && ipw <= 8;
+  return (k == 0 || k == 1 || k == 2 || k == 4) && v >= 0 && v <= 3 && ipw >= 0 && ipw <= 8 &&
+         (s->forward_schedule == 0 || s->forward_schedule == 1);
 }
 
 bool linesearch_is_noop(const dmpc_solver_params* s) {
@@ -256,6 +257,7 @@ int sqp_solve_impl(const dmpc_problem* prob, const dmpc_solver_params* params, c
   a.forced_K = params->blocks_per_group;
   a.variant = params->fused_variant;
   a.ipw = params->instances_per_workgroup;
+  a.schedule = params->forward_schedule;
   a.dt = (float)prob->dt;
   a.dt64 = prob->dt;
   a.maxit = params->num_scp_iteration_max;

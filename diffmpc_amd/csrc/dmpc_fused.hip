@@ -870,9 +870,10 @@ __device__ void stage_forward(const SolveArgs& a, const Lds& L, int b, int N, in
 // at once and keep their last linearization, like the reference's vmapped
 // while_loop keeps frozen instances.
 
-__global__ void sqp_reset_kernel(int B, int32_t* num_iter) {
+__global__ void sqp_reset_kernel(int B, int32_t* num_iter, int32_t* head) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < B) num_iter[b] = 0;
+  if (head && b == 0) *head = 0;
 }
 
 // Longest-first launch order for the next QP launch.  The hardware hands
@@ -1028,33 +1029,10 @@ __global__ void __launch_bounds__(256) linearize_kernel(SolveArgs a) {
 // evaluations hold enough registers to make the whole kernel spill at the
 // 128-VGPR cap of the one-block-per-group variant, so solves that do not
 // evaluate the line search run a build without it, which spills nothing.
-template <class Model, int G, int K, int MAXT, bool LS, bool TEAMS>
-__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(team_waves_per_simd<TEAMS>())))
-qp_iteration_kernel(SolveArgs a) {
+template <class Model, int G, int K, bool LS, bool TEAMS, bool PQ>
+__device__ __forceinline__ void qp_iteration(const SolveArgs& a, int b, const Team& T, bool live, int* flags,
+                                             char* lds) {
   constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
-  extern __shared__ double smem_[];
-  const Team T = team_of<TEAMS>(a.ipw, a.team_threads);
-  int b;
-  bool live = true;  // this team's instance is solved (else its team only keeps the barriers)
-  int* flags = nullptr;
-  char* lds = reinterpret_cast<char*>(smem_);
-  if constexpr (TEAMS) {
-    // a workgroup whose instances have all stopped returns at once
-    bool any = false;
-    for (int j = 0; j < a.ipw; ++j) {
-      const int sl = (int)blockIdx.x * a.ipw + j;
-      if (sl < a.B) any = any || sqp_active(a, a.order ? a.order[sl] : sl);
-    }
-    if (!any) return;
-    const int sl = T.slot < a.B ? T.slot : a.B - 1;  // slots past the batch read the last one
-    b = a.order ? a.order[sl] : sl;
-    live = T.slot < a.B && sqp_active(a, b);
-    flags = reinterpret_cast<int*>(lds + (size_t)a.ipw * a.team_lds);
-    lds += (size_t)T.idx * a.team_lds;
-  } else {
-    b = a.order ? a.order[blockIdx.x] : blockIdx.x;
-    if (!sqp_active(a, b)) return;
-  }
   const int it = a.num_iter[b];
   const int Np = a.N, NBs = Np + 1;                          // padded horizon, knot stride
   const int N = horizon_of(a.horizons, Np, b), NB = N + 1;  // this instance
@@ -1139,7 +1117,138 @@ qp_iteration_kernel(SolveArgs a) {
     a.pcg_iters[(size_t)b * a.maxit + it] = npcg;
     if (a.linesearch) a.ls_alphas[(size_t)b * a.maxit + it] = alpha;
     a.conv_err[b] = err;
-    a.num_iter[b] = it + 1;
+    if constexpr (!PQ) a.num_iter[b] = it + 1;
+  }
+  if constexpr (PQ) {
+    // publish the iteration to the workgroup that takes the instance's next
+    // one (maybe on another XCD): every wave's stores complete at L2, then
+    // one release at agent scope writes them back ahead of the count
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (T.tid == 0) __hip_atomic_store(a.num_iter + b, it + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// One launch for a whole solve instead of one linearization and one QP
+// launch per SQP iteration: B * maxit workgroups, each of which takes the
+// next position of a schedule from a counter (the solution's status[0], free
+// until sqp_finish_kernel).  Position p is SQP iteration p / B of instance
+// p % B; its workgroup waits until the instance has finished the previous
+// iteration, linearizes it and runs the iteration.  So an instance's next
+// iteration starts as soon as its own previous one is done, instead of after
+// the whole batch's, and the tails of the per-iteration launches disappear.
+// No deadlock: a position's predecessor was taken earlier, by a resident
+// workgroup that is running it or has finished it (a waiting workgroup holds
+// no unfinished position), so the lowest waiting position always has a
+// running predecessor.  An instance that stopped (converged, or not finite)
+// skips its remaining positions.  The wait is bounded: past kPqWaitTicks of
+// the 100 MHz wall clock the workgroup ends the schedule (the counter is
+// pushed past the end, so the remaining workgroups exit at once); the
+// instances left unfinished then report their convergence error.  Results
+// are the per-launch schedule's bits (the same per-instance computations).
+// Returns the instance to solve, or -1.
+constexpr long long kPqWaitTicks = 1LL << 29;  // ~5 s
+
+__device__ int pq_take(const SolveArgs& a, int32_t* head) {
+  const int total = a.B * a.maxit;
+  {
+    const int p = atomicAdd(head, 1);
+    if (p < 0 || p >= total) return -1;
+    const int s = p / a.B, c = p - s * a.B;
+    const long long t0 = wall_clock64();
+    bool stopped = false;
+    for (;;) {
+      const int n = __hip_atomic_load(a.num_iter + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (n >= s) break;
+      if (n >= 1) {
+        const long long e = __hip_atomic_load(reinterpret_cast<const long long*>(a.conv_err + c), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        if (!(__longlong_as_double(e) > a.tol)) {
+          stopped = true;
+          break;
+        }
+      }
+      if (wall_clock64() - t0 > kPqWaitTicks) {
+        atomicMax(head, total);
+        return -1;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    if (stopped) return -1;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return sqp_active(a, c) ? c : -1;
+  }
+}
+
+// The linearization of instance b (linearize_kernel's work for one
+// instance, the same per-item computations), by the threads of a workgroup:
+// Jacobian columns first, then the values, so that few waves mix the two.
+template <class Model>
+__device__ void linearize_instance(const SolveArgs& a, int b) {
+  constexpr int NX = Model::NX, NU = Model::NU, NZ = NX + NU;
+  const int N = a.N, Nb = horizon_of(a.horizons, N, b);
+  const int njac = Nb * NZ;
+  for (int k = threadIdx.x; k < njac + Nb; k += blockDim.x) {
+    float prm[Model::NPARAM];
+#pragma unroll
+    for (int i = 0; i < Model::NPARAM; ++i) prm[i] = a.dyn[(size_t)b * a.P + i];
+    if (k >= njac) {
+      const int t = k - njac;
+      fd_value<Model>(a.x, a.u, prm, a.dt64, a.scheme, N, b, t, a.fd + ((size_t)b * N + t) * NX);
+      continue;
+    }
+    const int j = k % NZ, t = k / NZ;
+    float xs[NX], us[NU];
+#pragma unroll
+    for (int i = 0; i < NX; ++i) xs[i] = (float)a.x[((size_t)b * (N + 1) + t) * NX + i];
+#pragma unroll
+    for (int i = 0; i < NU; ++i) us[i] = (float)a.u[((size_t)b * (N + 1) + t) * NU + i];
+    float fv[NX], col[NX];
+    step_jacobian_column<Model>(xs, us, prm, a.dt, a.scheme, j, fv, col);
+    float* At = a.AB + ((size_t)b * N + t) * NX * NZ;
+#pragma unroll
+    for (int i = 0; i < NX; ++i) At[i * NZ + j] = col[i];
+  }
+}
+
+template <class Model, int G, int K, int MAXT, bool LS, bool TEAMS, bool PQ = false>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(team_waves_per_simd<TEAMS>())))
+qp_iteration_kernel(SolveArgs a) {
+  extern __shared__ double smem_[];
+  const Team T = team_of<TEAMS>(a.ipw, a.team_threads);
+  char* lds = reinterpret_cast<char*>(smem_);
+  if constexpr (PQ) {
+    static_assert(!TEAMS, "the schedule runs one instance per workgroup");
+    __shared__ int task;
+    if (threadIdx.x == 0) task = pq_take(a, a.status);
+    __syncthreads();
+    const int b = __builtin_amdgcn_readfirstlane(task);  // uniform, as blockIdx.x is
+    if (b < 0) return;
+    linearize_instance<Model>(a, b);
+    __syncthreads();
+    qp_iteration<Model, G, K, LS, false, true>(a, b, T, true, nullptr, lds);
+  } else {
+    int b;
+    bool live = true;  // this team's instance is solved (else its team only keeps the barriers)
+    int* flags = nullptr;
+    if constexpr (TEAMS) {
+      // a workgroup whose instances have all stopped returns at once
+      bool any = false;
+      for (int j = 0; j < a.ipw; ++j) {
+        const int sl = (int)blockIdx.x * a.ipw + j;
+        if (sl < a.B) any = any || sqp_active(a, a.order ? a.order[sl] : sl);
+      }
+      if (!any) return;
+      const int sl = T.slot < a.B ? T.slot : a.B - 1;  // slots past the batch read the last one
+      b = a.order ? a.order[sl] : sl;
+      live = T.slot < a.B && sqp_active(a, b);
+      flags = reinterpret_cast<int*>(lds + (size_t)a.ipw * a.team_lds);
+      lds += (size_t)T.idx * a.team_lds;
+    } else {
+      b = a.order ? a.order[blockIdx.x] : blockIdx.x;
+      if (!sqp_active(a, b)) return;
+    }
+    qp_iteration<Model, G, K, LS, TEAMS, false>(a, b, T, live, flags, lds);
   }
 }
 
@@ -1364,8 +1473,45 @@ Launch plan_teams(int N, int P, int ls, int forced_K, int ipw) {
   return l;
 }
 
+// The whole solve as one launch of B * maxit workgroups (see pq_take), the
+// schedule's counter in status[0].  DMPC_ERR_UNSUPPORTED when the
+// per-iteration launches are to be used instead (the kernel's static LDS
+// does not fit next to the instance's, or the schedule would not fit an int).
+template <class Model, int G, int K, int MAXT, bool LS>
+int launch_fwd_persistent(const SolveArgs& a, const Launch& l, hipStream_t st) {
+  const void* kern = reinterpret_cast<const void*>(&qp_iteration_kernel<Model, G, K, MAXT, LS, false, true>);
+  constexpr size_t kStatic = 256;  // the kernel's own LDS (the task slot), rounded up
+  if (l.lds + kStatic > kMaxLds || (long long)a.B * a.maxit >= (1LL << 30)) return DMPC_ERR_UNSUPPORTED;
+  static bool attr = false;
+  if (!attr) {
+    if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kMaxLds - kStatic)) != hipSuccess)
+      return DMPC_ERR_HIP;
+    attr = true;
+  }
+  const unsigned groups = (unsigned)(a.B * a.maxit);
+  SolveArgs ai = a;
+  ai.order = nullptr;
+  ai.ipw = 1;
+  ai.team_threads = l.team_threads;
+  ai.team_lds = (int)l.team_lds;
+  hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter, a.status);
+  prof_begin(PROF_QP_ITERATION, st);
+  hipLaunchKernelGGL((qp_iteration_kernel<Model, G, K, MAXT, LS, false, true>), dim3(groups), dim3(l.threads), l.lds,
+                     st, ai);
+  prof_end(PROF_QP_ITERATION, st);
+  hipLaunchKernelGGL(sqp_finish_kernel, dim3(a.B), dim3(256), 0, st, a, Model::NX, Model::NU);
+  return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
+}
+
 template <class Model, int G, int K, int MAXT, bool LS, bool TEAMS = false>
 int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
+  if constexpr (!TEAMS) {
+    // batches of one horizon: the persistent schedule (one launch per solve)
+    if (!a.horizons && a.schedule == 0) {
+      const int rc = launch_fwd_persistent<Model, G, K, MAXT, LS>(a, l, st);
+      if (rc != DMPC_ERR_UNSUPPORTED) return rc;
+    }
+  }
   static bool attr = false;
   if (!attr) {
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&qp_iteration_kernel<Model, G, K, MAXT, LS, TEAMS>),
@@ -1374,7 +1520,7 @@ int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
     attr = true;
   }
   const unsigned groups = (unsigned)((a.B + l.ipw - 1) / l.ipw);
-  hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter);
+  hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter, nullptr);
   const long long nlin = linearize_threads<Model>(a.B, a.N);
   for (int it = 0; it < a.maxit; ++it) {
     prof_begin(PROF_LINEARIZE, st);
@@ -1533,7 +1679,7 @@ bool stream_backward_order(const BackwardArgs& a, int32_t* order, size_t stride,
 }
 
 int stream_launch_reset(const SolveArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter);
+  hipLaunchKernelGGL(sqp_reset_kernel, dim3((a.B + 255) / 256), dim3(256), 0, st, a.B, a.num_iter, nullptr);
   return hipGetLastError() == hipSuccess ? DMPC_OK : DMPC_ERR_HIP;
 }
 

@@ -54,6 +54,9 @@ struct SolveArgs {
   // with more, each instance is a team of team_threads threads with
   // team_lds bytes of LDS (set at launch)
   int ipw, team_threads, team_lds;
+  // forward schedule of the register-resident kernels (0 automatic, 1 one
+  // launch per SQP iteration)
+  int schedule;
 };
 
 struct BackwardArgs {

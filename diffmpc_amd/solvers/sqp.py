@@ -20,7 +20,10 @@ and size, else plain (linear models of any size).  Solver parameters use the ref
 ``"mode"``, ``"fused_variant"`` ("auto", "register", "streamed",
 "streamed_per_knot"), ``"blocks_per_group"`` (0 = automatic, 1, 2, 4) and
 ``"instances_per_workgroup"`` (0 or 1 = one, up to 8, register-resident
-kernels of models with nx <= 8), which diffmpc_amd.autotune sets.  ``problem_params["horizons"]`` (B,) makes a ragged batch.
+kernels of models with nx <= 8), which diffmpc_amd.autotune sets, and
+``"forward_schedule"`` ("auto": one launch per solve for batches of one
+horizon, "per_iteration": one linearization and one QP launch per SQP
+iteration; the same bits).  ``problem_params["horizons"]`` (B,) makes a ragged batch.
 """
 import copy
 import enum
@@ -106,6 +109,7 @@ def _c_solver_params(params: Dict[str, Any]) -> "_lib.SolverParams":
     sp.fused_variant = {"auto": 0, "register": 1, "streamed": 2, "streamed_per_knot": 3}[
         params.get("fused_variant", "auto")]
     sp.instances_per_workgroup = int(params.get("instances_per_workgroup", 0))
+    sp.forward_schedule = {"auto": 0, "per_iteration": 1}[params.get("forward_schedule", "auto")]
     return sp
 
 

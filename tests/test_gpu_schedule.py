@@ -1,0 +1,84 @@
+"""The register-resident forward of a batch of one horizon runs a whole solve
+as one launch: a workgroup per (SQP iteration, instance) takes its position
+from a counter, waits until the instance's previous iteration is done,
+linearizes the instance and runs the iteration.  forward_schedule
+"per_iteration" runs one linearization and one QP launch per SQP iteration
+instead.  The per-instance computations are the same, so both give the same
+bits: trajectories, multipliers, every history, the saved linearization the
+backward reads (through the gradients).  The cases include instances that
+meet the SQP stop rule at different iterations (their later positions are
+skipped), the merit line search, two blocks per lane group, and a batch of
+more instances than the device holds workgroups at once.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from diffmpc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+WKEY = "weights_penalization_reference_state_trajectory"
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _run(name, prob, sp, cuda, schedule, tuning):
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    solver.params.update(tuning, forward_schedule=schedule)
+    wx = params[WKEY].clone().requires_grad_(True)
+    x0 = params["initial_state"].clone().requires_grad_(True)
+    sol = solver.solve(guess, params, {WKEY: wx, "initial_state": x0})
+    loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+    gw, gx = torch.autograd.grad(loss, [wx, x0])
+    torch.cuda.synchronize()
+    out = {"states": sol.states, "controls": sol.controls, "dual": sol.dual, "num_iter": sol.num_iter,
+           "status": sol.status, "pcg": sol.num_pcg_iters, "conv": sol.convergence_errors,
+           "alphas": sol.linesearch_alphas, "grad_wx": gw, "grad_x0": gx}
+    return {k: _np(v) for k, v in out.items()}
+
+
+def _stop_rule(make):
+    def f():
+        name, prob, sp = make()
+        return name, prob, dict(sp, tol_convergence=1e-3)
+    return f
+
+
+CASES = {
+    "quadrotor-N30-B300": (lambda: W.quadrotor(batch=300, seed=3, horizon=30, sqp_iters=4), {}),
+    "quadrotor-N20-B40-K2": (lambda: W.quadrotor(batch=40, seed=4, horizon=20, sqp_iters=3),
+                             {"fused_variant": "register", "blocks_per_group": 2}),
+    "spacecraft7-stop-rule-B64": (_stop_rule(lambda: W.spacecraft7(batch=64, seed=2, horizon=12, sqp_iters=5,
+                                                                   pcg_eps=1e-10)), {}),
+    "cartpole-line-search-B8": (lambda: W.cartpole(batch=8, seed=6), {}),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_one_launch_solve_equals_per_iteration_launches(cuda, case):
+    make, tuning = CASES[case]
+    name, prob, sp = make()
+    if case.startswith("cartpole"):
+        sp = dict(sp, linesearch=True, num_scp_iteration_max=4, tol_convergence=1e-8)
+    B, N = prob["x0"].shape[0], int(prob["horizon"])
+    say(f"[schedule] {case}: B={B} N={N}")
+    one = _run(name, prob, sp, cuda, "auto", tuning)
+    per_iter = _run(name, prob, sp, cuda, "per_iteration", tuning)
+    if case.startswith("spacecraft7"):
+        say("  SQP iterations", np.unique(one["num_iter"]).tolist())
+        assert len(np.unique(one["num_iter"])) > 1  # instances stopped at different iterations
+    if case.startswith("cartpole"):
+        assert not np.all(one["alphas"] == 1.0)  # the line search shortened some steps
+    for k in one:
+        assert np.array_equal(one[k], per_iter[k]), k
+    say("  identical; PCG iterations", int(one["pcg"].sum()))
