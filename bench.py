@@ -21,10 +21,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SQP solves/s & fwd+bwd grads/s, quadrotor T=50 B=1024, fused vs plain GPU"
 MI355X_FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per qp_iteration_kernel launch at the default workload, from two
+# HBM bytes per qp_iteration_kernel launch (one whole solve: five SQP
+# iterations with their linearizations) at the default workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scale factors
-# from a 256 MiB device copy): 72.6 MB read + 18.2 MB written
-QP_TRAFFIC_BYTES = 90.8e6
+# from a 256 MiB device copy): 463.9 MB read + 686.6 MB written, against 731
+# MB algorithmic; the excess is the linearization's register spills
+QP_TRAFFIC_BYTES = 1150.5e6
 # the register-resident forward of a batch of one horizon: one launch per
 # solve (each workgroup linearizes one instance and runs one SQP iteration of
 # it, as soon as the instance's previous iteration is done)

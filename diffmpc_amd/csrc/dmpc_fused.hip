@@ -936,17 +936,22 @@ __global__ void __launch_bounds__(kOrderThreads) order_kernel(Keys keys, int B, 
   if (part == 0 && b < B) order[(size_t)rank[threadIdx.x >> 4] * stride] = b;
 }
 
-// Whether ordering pays: more instances than CUs (then instances share CUs
-// in turn), and few enough for the keys to fit in LDS.
-int order_grid(int B) {
+// CUs of the current device (1 << 30 when unknown).
+int device_cus() {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-      cus = 1 << 30;  // unknown: never order
+      cus = 1 << 30;
   }
-  if (B <= cus || B > kOrderMaxBatch) return 0;
+  return cus;
+}
+
+// Whether ordering pays: more instances than CUs (then instances share CUs
+// in turn), and few enough for the keys to fit in LDS.
+int order_grid(int B) {
+  if (B <= device_cus() || B > kOrderMaxBatch) return 0;
   return (B + kOrderPerGroup - 1) / kOrderPerGroup;
 }
 
@@ -1506,8 +1511,13 @@ int launch_fwd_persistent(const SolveArgs& a, const Launch& l, hipStream_t st) {
 template <class Model, int G, int K, int MAXT, bool LS, bool TEAMS = false>
 int launch_fwd_k(const SolveArgs& a, const Launch& l, hipStream_t st) {
   if constexpr (!TEAMS) {
-    // batches of one horizon: the persistent schedule (one launch per solve)
-    if (!a.horizons && a.schedule == 0) {
+    // batches of one horizon with more instances than CUs: one launch per
+    // solve.  With fewer, every instance runs at once in each per-iteration
+    // launch, so there is no launch tail to recover, and the separate
+    // linearization kernel spreads that work over the whole device
+    // (measured: the 64-instance rollout and the 32-instance imitation step
+    // 3-6 % slower in one launch)
+    if (!a.horizons && a.schedule == 0 && a.B > device_cus()) {
       const int rc = launch_fwd_persistent<Model, G, K, MAXT, LS>(a, l, st);
       if (rc != DMPC_ERR_UNSUPPORTED) return rc;
     }

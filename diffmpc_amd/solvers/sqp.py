@@ -22,8 +22,8 @@ and size, else plain (linear models of any size).  Solver parameters use the ref
 ``"instances_per_workgroup"`` (0 or 1 = one, up to 8, register-resident
 kernels of models with nx <= 8), which diffmpc_amd.autotune sets, and
 ``"forward_schedule"`` ("auto": one launch per solve for batches of one
-horizon, "per_iteration": one linearization and one QP launch per SQP
-iteration; the same bits).  ``problem_params["horizons"]`` (B,) makes a ragged batch.
+horizon with more instances than CUs, "per_iteration": one linearization and
+one QP launch per SQP iteration; the same bits).  ``problem_params["horizons"]`` (B,) makes a ragged batch.
 """
 import copy
 import enum

@@ -89,9 +89,10 @@ typedef struct dmpc_solver_params { /* keys of diffmpc/solvers/params/sqp.yaml *
    * one workgroup or the streamed variant runs.  No reference counterpart. */
   int32_t instances_per_workgroup;
   /* Forward schedule of the register-resident kernels: 0 = automatic (a
-   * batch of one horizon is solved by one launch, in which each instance's
-   * next SQP iteration starts as soon as its previous one is done; a ragged
-   * batch by one linearization and one QP launch per SQP iteration),
+   * batch of one horizon with more instances than the device has CUs is
+   * solved by one launch, in which each instance's next SQP iteration starts
+   * as soon as its previous one is done; other batches by one linearization
+   * and one QP launch per SQP iteration),
    * 1 = one linearization and one QP launch per SQP iteration always.  The
    * results are the same bits.  No reference counterpart. */
   int32_t forward_schedule;

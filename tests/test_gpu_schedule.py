@@ -1,14 +1,15 @@
-"""The register-resident forward of a batch of one horizon runs a whole solve
-as one launch: a workgroup per (SQP iteration, instance) takes its position
-from a counter, waits until the instance's previous iteration is done,
-linearizes the instance and runs the iteration.  forward_schedule
-"per_iteration" runs one linearization and one QP launch per SQP iteration
-instead.  The per-instance computations are the same, so both give the same
-bits: trajectories, multipliers, every history, the saved linearization the
+"""The register-resident forward of a batch of one horizon with more
+instances than the device has CUs runs a whole solve as one launch: a
+workgroup per (SQP iteration, instance) takes its position from a counter,
+waits until the instance's previous iteration is done, linearizes the
+instance and runs the iteration.  forward_schedule "per_iteration" runs one
+linearization and one QP launch per SQP iteration instead.  The
+per-instance computations are the same, so both give the same bits:
+trajectories, multipliers, every history, the saved linearization the
 backward reads (through the gradients).  The cases include instances that
 meet the SQP stop rule at different iterations (their later positions are
-skipped), the merit line search, two blocks per lane group, and a batch of
-more instances than the device holds workgroups at once.
+skipped), the merit line search and two blocks per lane group, each with
+more instances (300) than the device holds workgroups at once.
 """
 import sys
 
@@ -56,11 +57,11 @@ def _stop_rule(make):
 
 CASES = {
     "quadrotor-N30-B300": (lambda: W.quadrotor(batch=300, seed=3, horizon=30, sqp_iters=4), {}),
-    "quadrotor-N20-B40-K2": (lambda: W.quadrotor(batch=40, seed=4, horizon=20, sqp_iters=3),
+    "quadrotor-N20-B300-K2": (lambda: W.quadrotor(batch=300, seed=4, horizon=20, sqp_iters=3),
                              {"fused_variant": "register", "blocks_per_group": 2}),
-    "spacecraft7-stop-rule-B64": (_stop_rule(lambda: W.spacecraft7(batch=64, seed=2, horizon=12, sqp_iters=5,
-                                                                   pcg_eps=1e-10)), {}),
-    "cartpole-line-search-B8": (lambda: W.cartpole(batch=8, seed=6), {}),
+    "spacecraft7-stop-rule-B300": (_stop_rule(lambda: W.spacecraft7(batch=300, seed=2, horizon=12, sqp_iters=5,
+                                                                    pcg_eps=1e-10)), {}),
+    "cartpole-line-search-B300": (lambda: W.cartpole(batch=300, seed=6), {}),
 }
 
 
@@ -71,6 +72,7 @@ def test_one_launch_solve_equals_per_iteration_launches(cuda, case):
     if case.startswith("cartpole"):
         sp = dict(sp, linesearch=True, num_scp_iteration_max=4, tol_convergence=1e-8)
     B, N = prob["x0"].shape[0], int(prob["horizon"])
+    assert B > torch.cuda.get_device_properties(0).multi_processor_count  # the one-launch schedule runs
     say(f"[schedule] {case}: B={B} N={N}")
     one = _run(name, prob, sp, cuda, "auto", tuning)
     per_iter = _run(name, prob, sp, cuda, "per_iteration", tuning)
