@@ -27,11 +27,12 @@ MI355X_FP64_PEAK_TFLOPS = 78.6
 # from a 256 MiB device copy): 463.9 MB read + 686.6 MB written, against 731
 # MB algorithmic; the excess is the linearization's register spills
 QP_TRAFFIC_BYTES = 1150.5e6
-# the register-resident forward of a batch of one horizon: one launch per
-# solve (each workgroup linearizes one instance and runs one SQP iteration of
-# it, as soon as the instance's previous iteration is done)
-QP_KERNEL_TEXT = ("qp_iteration_kernel (linearization + assembly + PCG + recovery; every SQP iteration "
-                  "of the batch in one launch)")
+# the register-resident forward of a batch of one horizon with more instances
+# than CUs: one launch per solve (each workgroup linearizes one instance and
+# runs one SQP iteration of it, as soon as the instance's previous iteration
+# is done); otherwise one linearization and one QP launch per SQP iteration
+QP_KERNEL_TEXT = ("qp_iteration_kernel (assembly + PCG + recovery of every SQP iteration; for a batch of one "
+                  "horizon with more instances than CUs one launch per solve, linearizations included)")
 QP_TRAFFIC_SOURCE = "profiles/qp_iteration_quadrotor_B1024_T50.md"
 # HBM bytes per forward SQP iteration of the chain workload (stream_assemble_kernel
 # + stream_qp_kernel, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0): 0.983 TB read + 0.757 TB
