@@ -8,8 +8,9 @@ per-instance computations are the same, so both give the same bits:
 trajectories, multipliers, every history, the saved linearization the
 backward reads (through the gradients).  The cases include instances that
 meet the SQP stop rule at different iterations (their later positions are
-skipped), the merit line search and two blocks per lane group, each with
-more instances (300) than the device holds workgroups at once.
+skipped), instances whose state is not finite (they stop with the ERROR
+status), the merit line search and two blocks per lane group, each with more
+instances (300) than the device holds workgroups at once.
 """
 import sys
 
@@ -55,7 +56,18 @@ def _stop_rule(make):
     return f
 
 
+def _with_nan_instances():
+    # three instances with a non-finite initial state: they stop after their
+    # first SQP iteration (ERROR status), and their later positions are skipped
+    name, prob, sp = W.quadrotor(batch=300, seed=5, horizon=16, sqp_iters=4)
+    prob = dict(prob)
+    prob["x0"] = np.array(prob["x0"], copy=True)
+    prob["x0"][[0, 150, 299], 2] = np.nan
+    return name, prob, sp
+
+
 CASES = {
+    "quadrotor-nan-instances-B300": (_with_nan_instances, {}),
     "quadrotor-N30-B300": (lambda: W.quadrotor(batch=300, seed=3, horizon=30, sqp_iters=4), {}),
     "quadrotor-N20-B300-K2": (lambda: W.quadrotor(batch=300, seed=4, horizon=20, sqp_iters=3),
                              {"fused_variant": "register", "blocks_per_group": 2}),
@@ -81,6 +93,8 @@ def test_one_launch_solve_equals_per_iteration_launches(cuda, case):
         assert len(np.unique(one["num_iter"])) > 1  # instances stopped at different iterations
     if case.startswith("cartpole"):
         assert not np.all(one["alphas"] == 1.0)  # the line search shortened some steps
+    if "nan" in case:
+        assert (one["status"] != 0).sum() >= 3 and (one["status"][[0, 150, 299]] != 0).all()
     for k in one:
-        assert np.array_equal(one[k], per_iter[k]), k
+        assert np.array_equal(one[k], per_iter[k], equal_nan=True), k
     say("  identical; PCG iterations", int(one["pcg"].sum()))
