@@ -121,7 +121,8 @@ typedef struct dmpc_solution { /* SQPSolution fields, sqp.py:38-57 */
   double* controls;           /* (B, N+1, nu)  in: initial guess, out: solution */
   double* dual;               /* (B, N+1, nx)  in: multiplier guess, out: multipliers */
   int32_t* status;            /* (B,) 0 SUCCESS, -100 ERROR (fused mode: holds the QP
-                                 launches' instance order until the solve's last kernel) */
+                                 launches' instance order, or the one-launch schedule's
+                                 counter, until the solve's last kernel) */
   int32_t* num_iter;          /* (B,) */
   double* convergence_error;  /* (B,) */
   double* convergence_errors; /* (B, maxit) in: carried from the guess, out */
