@@ -1156,33 +1156,28 @@ constexpr long long kPqWaitTicks = 1LL << 29;  // ~5 s
 
 __device__ int pq_take(const SolveArgs& a, int32_t* head) {
   const int total = a.B * a.maxit;
-  {
-    const int p = atomicAdd(head, 1);
-    if (p < 0 || p >= total) return -1;
-    const int s = p / a.B, c = p - s * a.B;
-    const long long t0 = wall_clock64();
-    bool stopped = false;
-    for (;;) {
-      const int n = __hip_atomic_load(a.num_iter + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (n >= s) break;
-      if (n >= 1) {
-        const long long e = __hip_atomic_load(reinterpret_cast<const long long*>(a.conv_err + c), __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-        if (!(__longlong_as_double(e) > a.tol)) {
-          stopped = true;
-          break;
-        }
-      }
-      if (wall_clock64() - t0 > kPqWaitTicks) {
-        atomicMax(head, total);
-        return -1;
-      }
-      __builtin_amdgcn_s_sleep(2);
+  const int p = atomicAdd(head, 1);
+  if (p < 0 || p >= total) return -1;
+  const int s = p / a.B, c = p - s * a.B;
+  const long long t0 = wall_clock64();
+  for (;;) {
+    // iterations done so far, published by the release store that ends each
+    const int n = __hip_atomic_load(a.num_iter + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (n >= s) break;
+    if (n >= 1) {
+      // stopped after n iterations: converged, or its error is not finite
+      const long long e = __hip_atomic_load(reinterpret_cast<const long long*>(a.conv_err + c), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+      if (!(__longlong_as_double(e) > a.tol)) return -1;
     }
-    if (stopped) return -1;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return sqp_active(a, c) ? c : -1;
+    if (wall_clock64() - t0 > kPqWaitTicks) {
+      atomicMax(head, total);
+      return -1;
+    }
+    __builtin_amdgcn_s_sleep(2);
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return sqp_active(a, c) ? c : -1;
 }
 
 // The linearization of instance b (linearize_kernel's work for one
