@@ -9,9 +9,11 @@ leaves mostly idle), or as the streamed variant that keeps the blocks in a
 device workspace.  A candidate is (variant, blocks per group, instances per
 workgroup).  ``tune`` times every candidate the library accepts on the
 caller's own problem (forward plus backward: the library's HIP events
-around the QP-iteration and adjoint launches, i.e. the kernels a launch shape
-changes, so host overhead does not blur small problems; wall clock on the
-current stream where the library records no launches), writes the fastest
+around the linearization, QP-iteration and adjoint launches, i.e. the
+kernels a launch shape changes -- a batch the register-resident kernels
+solve in one launch linearizes inside it -- so host overhead does not blur
+small problems; wall clock on the current stream where the library records
+no launches), writes the fastest
 into the solver's params (``fused_variant``,
 ``blocks_per_group``, ``instances_per_workgroup``) and caches it per problem
 shape (model, sizes, batch, ragged horizons, mode, line search, SQP and PCG
@@ -101,7 +103,7 @@ def tune(solver, params, guess, reps: int = 2, backward: bool = True, candidates
     saved = (solver.params.get("fused_variant", "auto"), solver.params.get("blocks_per_group", 0),
              solver.params.get("instances_per_workgroup", 0) or 1)
     lib = _lib.load_for(solver.program.dynamics)
-    kernels = (_lib.PROF_QP_ITERATION, _lib.PROF_BACKWARD)
+    kernels = (_lib.PROF_LINEARIZE, _lib.PROF_QP_ITERATION, _lib.PROF_BACKWARD)
     for choice in candidates:
         _apply(solver, choice)
         try:
