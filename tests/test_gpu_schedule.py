@@ -98,3 +98,47 @@ def test_one_launch_solve_equals_per_iteration_launches(cuda, case):
     for k in one:
         assert np.array_equal(one[k], per_iter[k], equal_nan=True), k
     say("  identical; PCG iterations", int(one["pcg"].sum()))
+
+
+def test_one_launch_solve_in_a_graph_matches_eager_on_every_replay(cuda):
+    """The one-launch forward (300 quadrotor instances) with its backward,
+    captured as one hipGraph: the schedule's counter is cleared by a kernel
+    at the start of every replay, so each replay, with unrelated eager work
+    and synchronisations in between, equals the eager step bit for bit; a
+    replay after an in-place change of the initial states equals the eager
+    step on the changed inputs."""
+    from diffmpc_amd.graph import GraphedStep
+
+    name, prob, sp = W.quadrotor(batch=300, seed=7, horizon=24, sqp_iters=3)
+    assert prob["x0"].shape[0] > torch.cuda.get_device_properties(0).multi_processor_count
+    say("[schedule, graph] quadrotor B=300 N=24")
+    solver, params, guess = W.make_solver(name, prob, sp, device=cuda)
+    w = params[WKEY].clone().requires_grad_(True)
+
+    def step():
+        guess.dual_backward_guess.zero_()
+        sol = solver.solve(guess, params, {WKEY: w})
+        loss = 0.5 * (sol.states - params["reference_state_trajectory"]).square().sum()
+        (g,) = torch.autograd.grad(loss, [w])
+        return sol.states, sol.num_pcg_iters, g
+
+    def snap(outs):
+        return [o.detach().clone() for o in outs]
+
+    eager = snap(step())
+    graphed = GraphedStep(step)
+    junk = torch.randn(512, 512, device=cuda)
+    for k in range(3):
+        got = snap(graphed.replay())
+        torch.cuda.synchronize()
+        for e, g in zip(eager, got):
+            assert torch.equal(e, g), f"replay {k}"
+        junk = junk @ junk.T / 512.0  # eager work between replays
+        torch.cuda.synchronize()
+    params["initial_state"].mul_(0.5)
+    got = snap(graphed.replay())
+    eager2 = snap(step())
+    for e, g in zip(eager2, got):
+        assert torch.equal(e, g)
+    assert not torch.equal(eager[0], eager2[0])
+    say("  every replay identical")
